@@ -1,0 +1,123 @@
+"""ctypes binding of the C ABI declared in ``include/gcam/denoise.h``.
+
+The product path is the HIP library ``libgcam_denoise.so`` built in-tree by
+``glasscamerapublic_amd.build``.  There is no fallback: if the library is
+missing, ``lib()`` raises and every operation fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgcam_denoise.so")
+
+GCAM_OK = 0
+GCAM_PIXEL_RGBA32F = 0
+GCAM_PIXEL_RG32F = 1
+GCAM_PIXEL_R32F = 2
+GCAM_PIXEL_PCA8_F32 = 3
+GCAM_PIXEL_PCA8_F16 = 4
+
+GCAM_PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
+GCAM_LEVEL_DENOISED, GCAM_LEVEL_SUBTRACTED, GCAM_LEVEL_NOISY, GCAM_LEVEL_FEATURES = 0, 1, 2, 3
+STAGES = ("downsample", "nlf", "subtract", "pca", "block_match")
+
+# Every symbol the header declares (checked by the CPU test suite).
+EXPORTED = (
+    "gcam_abi_version", "gcam_status_string", "gcam_last_error",
+    "gcam_pyramid_create", "gcam_pyramid_destroy", "gcam_pyramid_denoise", "gcam_pyramid_level",
+    "gcam_pyramid_basis", "gcam_pyramid_set_profiling", "gcam_pyramid_stage_times", "gcam_downsample_image", "gcam_subtract_noise_image", "gcam_collect_patches",
+    "gcam_build_pca_space", "gcam_pca_projection", "gcam_block_matching_denoise_image",
+    "gcam_basic_noise_statistics", "gcam_measure_ycbcr_nlf", "gcam_synth_frame",
+)
+
+
+class GcamError(RuntimeError):
+    """A gcam_status other than GCAM_OK, with gcam_last_error()'s message."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"gcam status {status}: {message}")
+        self.status = status
+
+
+class Image(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("stride", ctypes.c_int32), ("format", ctypes.c_int32)]
+
+
+class DenoiseParamsC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("luma", "chroma", "chromaBoost", "gradientBoost", "gradientThreshold", "sharpening")]
+
+
+class NlfC(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_float * 3), ("b", ctypes.c_float * 3)]
+
+
+class ConfigC(ctypes.Structure):
+    _fields_ = [("search_radius", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 6)]
+
+
+class SynthParamsC(ctypes.Structure):
+    _fields_ = [("scene_seed", ctypes.c_uint64), ("noise_seed", ctypes.c_uint64), ("shot_gain", ctypes.c_float),
+                ("read_sigma", ctypes.c_float), ("lo", ctypes.c_float), ("hi", ctypes.c_float),
+                ("period", ctypes.c_float), ("octaves", ctypes.c_int32)]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+_F = ctypes.c_float
+_IP = ctypes.POINTER(Image)
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    L.gcam_abi_version.restype = _I
+    L.gcam_status_string.argtypes = [_I]
+    L.gcam_status_string.restype = ctypes.c_char_p
+    L.gcam_last_error.restype = ctypes.c_char_p
+    L.gcam_pyramid_create.argtypes = [_I, _I, _I, ctypes.POINTER(ConfigC), ctypes.POINTER(_P)]
+    L.gcam_pyramid_destroy.argtypes = [_P]
+    L.gcam_pyramid_destroy.restype = None
+    L.gcam_pyramid_denoise.argtypes = [_P, ctypes.POINTER(DenoiseParamsC), _IP, _IP, ctypes.POINTER(NlfC), _F, _F, _I,
+                                       _P, _IP]
+    L.gcam_pyramid_level.argtypes = [_P, _I, _I, _IP]
+    L.gcam_pyramid_basis.argtypes = [_P, _I, _P, _P]
+    L.gcam_pyramid_set_profiling.argtypes = [_P, _I]
+    L.gcam_pyramid_stage_times.argtypes = [_P, _P]
+    L.gcam_downsample_image.argtypes = [_IP, _IP, _P]
+    L.gcam_subtract_noise_image.argtypes = [_IP, _IP, _IP, _IP, _F, _F, _F, _F, _IP, _P]
+    L.gcam_collect_patches.argtypes = [_IP, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _P]
+    L.gcam_build_pca_space.argtypes = [_P, ctypes.c_int64, _P, _P, _P]
+    L.gcam_pca_projection.argtypes = [_IP, _P, _IP, _P]
+    L.gcam_block_matching_denoise_image.argtypes = [_IP, _IP, _IP, _P, _P, _P, _F, _F, _F, _F, _I, _IP, _P]
+    L.gcam_basic_noise_statistics.argtypes = [_IP, _IP, _P]
+    L.gcam_measure_ycbcr_nlf.argtypes = [_IP, _IP, _F, ctypes.POINTER(NlfC), _P]
+    L.gcam_synth_frame.argtypes = [_I, _I, ctypes.POINTER(SynthParamsC), _P, _IP, _IP, _P]
+    for name in EXPORTED:
+        fn = getattr(L, name)
+        if fn.restype is ctypes.c_int:  # default restype: gcam_status
+            fn.restype = _I
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m glasscamerapublic_amd.build` "
+                "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != GCAM_OK:
+        msg = lib().gcam_last_error()
+        raise GcamError(status, msg.decode() if msg else "")

@@ -1,0 +1,106 @@
+// Internal declarations shared by the gcam HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "gcam/denoise.h"
+
+namespace gcam {
+
+// Patch geometry of the PCA path (MetalPipeline/pyramid_processor.hpp:28-29).
+constexpr int kPatch = 25;      // 5 x 5 luma patch
+constexpr int kComponents = 8;  // PCA components kept
+constexpr int kMoments = kPatch + kPatch * (kPatch + 1) / 2;  // sum x (25) + upper triangle of sum x x^T (325)
+constexpr int kMomentBlocks = 240;  // fixed grid of the moment reduction (deterministic order)
+constexpr int kNlfBlocks = 256;     // fixed grid of the noise-model reductions
+
+// Device view of an image: base pointer, size and row pitch in pixels.
+template <class P>
+struct View {
+    P* ptr;
+    int w, h, stride;
+    __device__ __forceinline__ P& at(int x, int y) const { return ptr[(size_t)y * stride + x]; }
+};
+
+template <class P>
+inline View<P> view(const gcam_image& im) {
+    return View<P>{reinterpret_cast<P*>(im.data), im.width, im.height, im.stride};
+}
+template <class P>
+inline View<P> view(P* p, int w, int h) {
+    return View<P>{p, w, h, w};
+}
+
+// Per-level noise model on the device: var = a + b * Y per channel.
+struct NlfDev {
+    float a[3];
+    float b[3];
+};
+
+// Host-side parameters of one block-matching launch (by value).
+struct BmParams {
+    float thr[3];
+    float chromaBoost, gradientBoost, gradientThreshold, lensShading;
+};
+
+// State of the on-device noise-model fit (MetalPipeline/pyramid_processor.cpp:186-323).
+struct NlfFitState {
+    double varianceMax[3];
+    double A[3], B[3];
+    double err2[3];
+    double hist[6];
+    double n1, n2, discarded;
+    double newErr2[3];
+    int accepted;
+};
+
+// ---- error plumbing ----
+gcam_status set_error(gcam_status st, const std::string& msg);
+#define GCAM_HIP_TRY(expr)                                                                   \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            return ::gcam::set_error(_e == hipErrorOutOfMemory ? GCAM_ERR_OUT_OF_MEMORY      \
+                                                               : GCAM_ERR_HIP,               \
+                                     std::string(#expr) + ": " + hipGetErrorString(_e));     \
+    } while (0)
+#define GCAM_LAUNCH_CHECK(what)                                                              \
+    do {                                                                                     \
+        hipError_t _e = hipGetLastError();                                                   \
+        if (_e != hipSuccess)                                                                \
+            return ::gcam::set_error(GCAM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// ---- launchers (stream-ordered; all return GCAM_OK or set the error) ----
+gcam_status launch_downsample4(View<const float4> in, View<float4> out, hipStream_t s);
+gcam_status launch_downsample2(View<const float2> in, View<float2> out, hipStream_t s);
+// nlf_level: device noise model of this level (or nullptr to use nlf_x/nlf_y as given);
+// when set, nlf = (a.x * thr_y, b.x * thr_y).
+gcam_status launch_subtract_noise(View<const float4> in, View<const float4> in1, View<const float4> den1,
+                                  View<const float2> grad, float luma_weight, float sharpening, float nlf_x,
+                                  float nlf_y, const NlfDev* nlf_level, float thr_y, View<float4> out, hipStream_t s);
+gcam_status launch_noise_stats(View<const float4> in, View<float4> out, hipStream_t s);
+gcam_status launch_nlf_fit(View<const float4> stats, float exposure_multiplier, double* partials,
+                           NlfFitState* state, NlfDev* nlf_out, hipStream_t s);
+gcam_status launch_set_nlf(const gcam_ycbcr_nlf* host_levels, int levels, NlfDev* dev, hipStream_t s);
+
+gcam_status launch_collect_patches(View<const float4> in, float* patches, int64_t n, hipStream_t s);
+// Moments of the patches sampled from an image (never materialized).
+gcam_status launch_image_moments(View<const float4> in, double* partials, hipStream_t s);
+// Moments of a device patch buffer (n x 25).
+gcam_status launch_buffer_moments(const float* patches, int64_t n, double* partials, hipStream_t s);
+// Reduce the partials, form the covariance, eigensolve, write basis (25 x 8 f32) and eigenvalues.
+gcam_status launch_pca_solve(const double* partials, int64_t n, float* basis, double* eigvals, hipStream_t s);
+gcam_status launch_pca_project(View<const float4> in, const float* basis, View<float> feat8, hipStream_t s);
+
+gcam_status launch_block_match(View<const float4> in, View<const float2> grad, View<const float> feat8,
+                               const NlfDev* nlf_level, const BmParams& p, int radius, View<float4> out,
+                               hipStream_t s);
+
+gcam_status launch_synth(int w, int h, const gcam_synth_params& p, uint16_t* raw, View<float4> ycc,
+                         View<float4> clean, hipStream_t s);
+
+}  // namespace gcam

@@ -1,0 +1,319 @@
+// PCA patch basis on the device: patch sampling, FP64 moments, FP64 Jacobi
+// eigensolve and the 25 -> 8 projection, for gfx950.
+//
+//   collect_patches  MetalPipeline/demosaic.metal:755-772 (grid demosaic_kernels.hpp:388)
+//   moments + solve  MetalPipeline/PCA.cpp:23-56 (build_pca_space<25,8>)
+//   pca_project      MetalPipeline/demosaic.metal:774-796
+//
+// The reference maps the patch buffer to the CPU and runs Eigen there, with a
+// host synchronization per level (pyramid_processor.cpp:146-151).  Here the
+// basis never leaves the device: the patches are reduced straight from the
+// image into FP64 first and second moments (sum x, sum x x^T) by a fixed grid
+// of workgroups, one workgroup turns them into the covariance and runs a
+// parallel-ordered cyclic Jacobi, and the projection reads the basis from
+// device memory, so a whole denoise call stays stream-ordered.
+#include "gcam_internal.hpp"
+
+namespace gcam {
+
+namespace {
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Sign reference of the basis vectors: each vector is oriented so that its dot
+// product with this fixed vector is non-negative (any eigenvector solver
+// leaves the sign free; fixing it makes the features reproducible).
+__device__ __forceinline__ double sign_ref(int r) { return 1.0 + (double)((r * r * 7 + r * 3 + 5) % 23) / 16.0; }
+
+// Row/column of moment entry e: entries 0..24 are sum x_r (r = e, s = -1),
+// entries 25.. enumerate the upper triangle r <= s row by row.
+__device__ __forceinline__ void moment_rs(int e, int& r, int& s) {
+    if (e < kPatch) {
+        r = e;
+        s = -1;
+        return;
+    }
+    int t = e - kPatch;
+    r = 0;
+    while (t >= kPatch - r) {
+        t -= kPatch - r;
+        r++;
+    }
+    s = r + t;
+}
+
+// Moment accumulation over rounds of 256 patches staged in LDS.  Each thread
+// owns moment entries tid and tid + 256 (350 in all).  `fetch(k, lds_row)`
+// writes the 25 values of patch k (zeros when k is out of range).
+template <class Fetch>
+__device__ void moments_body(int64_t npatch, double* partials, Fetch fetch) {
+    __shared__ float pt[256 * kPatch];
+    int r0, s0, r1 = 0, s1 = -1;
+    moment_rs(threadIdx.x, r0, s0);
+    const bool has1 = threadIdx.x + 256 < kMoments;
+    if (has1) moment_rs(threadIdx.x + 256, r1, s1);
+    double acc0 = 0.0, acc1 = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t base = (int64_t)blockIdx.x * 256; base < npatch; base += stride) {
+        const int64_t k = base + threadIdx.x;
+        fetch(k, &pt[threadIdx.x * kPatch]);
+        __syncthreads();
+        const int cnt = (int)min<int64_t>(256, npatch - base);
+        for (int i = 0; i < cnt; i++) {
+            const float* row = &pt[i * kPatch];
+            const double a0 = row[r0];
+            acc0 += s0 < 0 ? a0 : a0 * (double)row[s0];
+            if (has1) {
+                const double a1 = row[r1];
+                acc1 += s1 < 0 ? a1 : a1 * (double)row[s1];
+            }
+        }
+        __syncthreads();
+    }
+    partials[(size_t)blockIdx.x * kMoments + threadIdx.x] = acc0;
+    if (has1) partials[(size_t)blockIdx.x * kMoments + threadIdx.x + 256] = acc1;
+}
+
+// Patches sampled from the image at 8*(x, y), x < w/8, y < h/8, luma only,
+// clamp-to-edge reads (collectPatches).  Slots the grid does not reach are
+// zero rows: they add nothing to the moments but count in n.
+__global__ __launch_bounds__(256) void k_image_moments(View<const float4> in, double* partials) {
+    const int gw = in.w / 8, gh = in.h / 8;
+    const int64_t npatch = (int64_t)gw * gh;
+    moments_body(npatch, partials, [&](int64_t k, float* row) {
+        if (k < npatch) {
+            const int x = (int)(k % gw), y = (int)(k / gw);
+            for (int j = -2; j <= 2; j++) {
+                const int sy = clampi(8 * y + j, 0, in.h - 1);
+                for (int i = -2; i <= 2; i++) row[(j + 2) * 5 + (i + 2)] = in.at(clampi(8 * x + i, 0, in.w - 1), sy).x;
+            }
+        } else {
+            for (int e = 0; e < kPatch; e++) row[e] = 0.0f;
+        }
+    });
+}
+
+__global__ __launch_bounds__(256) void k_buffer_moments(const float* patches, int64_t n, double* partials) {
+    moments_body(n, partials, [&](int64_t k, float* row) {
+        for (int e = 0; e < kPatch; e++) row[e] = k < n ? patches[k * kPatch + e] : 0.0f;
+    });
+}
+
+// collectPatches materialized (per-operation entry point): slot = y*w/8 + x.
+__global__ __launch_bounds__(256) void k_collect_patches(View<const float4> in, float* patches, int64_t n) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= in.w / 8 || y >= in.h / 8) return;
+    const int64_t slot = (int64_t)y * in.w / 8 + x;
+    if (slot >= n) return;
+    for (int j = -2; j <= 2; j++) {
+        const int sy = clampi(8 * y + j, 0, in.h - 1);
+        for (int i = -2; i <= 2; i++)
+            patches[slot * kPatch + (j + 2) * 5 + (i + 2)] = in.at(clampi(8 * x + i, 0, in.w - 1), sy).x;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_zero(float* p, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) p[i] = 0.0f;
+}
+
+// Covariance + cyclic Jacobi (parallel circle ordering) + basis selection.
+//
+// C = (sum x x^T - (sum x)(sum x)^T / n) / (n - 1)   -- PCA.cpp:31-32
+// Jacobi: each sweep runs 25 rounds; round r rotates the 12 disjoint pairs
+// ((r + k) mod 25, (r - k) mod 25), k = 1..12, so every off-diagonal pair is
+// annihilated once per sweep (index r idles).  Rotation as in Golub & Van Loan
+// Alg. 8.5.2.  Sweeps stop when the off-diagonal energy is below 1e-30 of the
+// total.  The 8 largest eigenpairs, largest first (PCA.cpp:49-55), are written
+// as basis[r * 8 + c]; ties in eigenvalue keep index order.
+__global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int nblocks, double n, float* basis,
+                                                   double* eigvals) {
+    constexpr int N = kPatch;
+    constexpr int LD = N + 1;
+    __shared__ double mom[kMoments];
+    __shared__ double A[N * LD];
+    __shared__ double V[N * LD];
+    __shared__ double cs[12][2];
+    __shared__ int pq[12][2];
+    __shared__ double red[2][4];
+    __shared__ int rank_of[N];
+    const int tid = threadIdx.x;
+
+    for (int e = tid; e < kMoments; e += 256) {
+        double a = 0.0;
+        for (int b = 0; b < nblocks; b++) a += partials[(size_t)b * kMoments + e];
+        mom[e] = a;
+    }
+    __syncthreads();
+    for (int e = tid; e < kMoments; e += 256) {
+        if (e < N) continue;
+        int r, s;
+        moment_rs(e, r, s);
+        const double c = (mom[e] - mom[r] * mom[s] / n) / (n - 1.0);
+        A[r * LD + s] = c;
+        A[s * LD + r] = c;
+    }
+    for (int e = tid; e < N * N; e += 256) {
+        const int r = e / N, s = e % N;
+        V[r * LD + s] = r == s ? 1.0 : 0.0;
+    }
+    __syncthreads();
+
+    for (int sweep = 0; sweep < 40; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int e = tid; e < N * N; e += 256) {
+            const int r = e / N, s = e % N;
+            const double v = A[r * LD + s] * A[r * LD + s];
+            tot += v;
+            if (r != s) off += v;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            off += __shfl_down(off, o);
+            tot += __shfl_down(tot, o);
+        }
+        if ((tid & 63) == 0) {
+            red[0][tid >> 6] = off;
+            red[1][tid >> 6] = tot;
+        }
+        __syncthreads();
+        off = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        tot = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        __syncthreads();
+        if (off == 0.0 || off <= 1e-30 * tot) break;
+
+        for (int round = 0; round < N; round++) {
+            if (tid < 12) {
+                int p = (round + tid + 1) % N, q = (round - tid - 1 + N) % N;
+                if (p > q) {
+                    const int t = p;
+                    p = q;
+                    q = t;
+                }
+                const double apq = A[p * LD + q];
+                double c = 1.0, s = 0.0;
+                if (apq != 0.0) {
+                    const double theta = (A[q * LD + q] - A[p * LD + p]) / (2.0 * apq);
+                    const double at = fabs(theta);
+                    double t = at > 1e150 ? 0.5 / at : 1.0 / (at + sqrt(theta * theta + 1.0));
+                    if (theta < 0.0) t = -t;
+                    c = 1.0 / sqrt(t * t + 1.0);
+                    s = t * c;
+                }
+                cs[tid][0] = c;
+                cs[tid][1] = s;
+                pq[tid][0] = p;
+                pq[tid][1] = q;
+            }
+            __syncthreads();
+            // A <- J^T A: rows p, q of every pair
+            for (int t = tid; t < 12 * N; t += 256) {
+                const int k = t / N, j = t % N;
+                const int p = pq[k][0], q = pq[k][1];
+                const double c = cs[k][0], s = cs[k][1];
+                const double ap = A[p * LD + j], aq = A[q * LD + j];
+                A[p * LD + j] = c * ap - s * aq;
+                A[q * LD + j] = s * ap + c * aq;
+            }
+            __syncthreads();
+            // A <- A J and V <- V J: columns p, q of every pair
+            for (int t = tid; t < 24 * N; t += 256) {
+                const int u = t / N, j = t % N;
+                const int k = u % 12;
+                double* M = u < 12 ? A : V;
+                const int p = pq[k][0], q = pq[k][1];
+                const double c = cs[k][0], s = cs[k][1];
+                const double mp = M[j * LD + p], mq = M[j * LD + q];
+                M[j * LD + p] = c * mp - s * mq;
+                M[j * LD + q] = s * mp + c * mq;
+            }
+            __syncthreads();
+        }
+    }
+
+    if (tid < N) {
+        const double di = A[tid * LD + tid];
+        int rank = 0;
+        for (int j = 0; j < N; j++) {
+            const double dj = A[j * LD + j];
+            rank += (dj > di || (dj == di && j < tid)) ? 1 : 0;
+        }
+        rank_of[tid] = rank;
+        if (eigvals) eigvals[rank] = di;
+        if (rank < kComponents) {
+            double dot = 0.0;
+            for (int r = 0; r < N; r++) dot += V[r * LD + tid] * sign_ref(r);
+            const double sg = dot < 0.0 ? -1.0 : 1.0;
+            for (int r = 0; r < N; r++) basis[r * kComponents + rank] = (float)(sg * V[r * LD + tid]);
+        }
+    }
+}
+
+// pcaProjection: f[c] = sum_row basis[row][c] * Y(p + off(row)), rows j-outer /
+// i-inner, clamp-to-edge.  A 64 x 4 tile plus its 2-pixel apron of luma is
+// staged in LDS; the basis is wave-uniform and comes in through scalar loads.
+__global__ __launch_bounds__(256) void k_pca_project(View<const float4> in, const float* __restrict__ basis,
+                                                     float* __restrict__ feat, int feat_stride) {
+    constexpr int TW = 64, TH = 4, LW = TW + 4, LH = TH + 4;
+    __shared__ float lum[LH * LW];
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const int tid = threadIdx.y * TW + threadIdx.x;
+    for (int e = tid; e < LH * LW; e += 256) {
+        const int ly = e / LW, lx = e % LW;
+        lum[e] = in.at(clampi(x0 + lx - 2, 0, in.w - 1), clampi(y0 + ly - 2, 0, in.h - 1)).x;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (x >= in.w || y >= in.h) return;
+    float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const float v = lum[(threadIdx.y + j) * LW + threadIdx.x + i];
+            const int row = j * 5 + i;
+#pragma unroll
+            for (int c = 0; c < 8; c++) f[c] += basis[row * 8 + c] * v;
+        }
+    }
+    float4* o = reinterpret_cast<float4*>(feat + ((size_t)y * feat_stride + x) * 8);
+    o[0] = make_float4(f[0], f[1], f[2], f[3]);
+    o[1] = make_float4(f[4], f[5], f[6], f[7]);
+}
+
+}  // namespace
+
+gcam_status launch_collect_patches(View<const float4> in, float* patches, int64_t n, hipStream_t s) {
+    const int64_t count = n * kPatch;
+    int blocks = (int)std::min<int64_t>((count + 255) / 256, 4096);
+    if (blocks < 1) blocks = 1;
+    k_fill_zero<<<blocks, 256, 0, s>>>(patches, count);
+    const int gw = in.w / 8, gh = in.h / 8;
+    if (gw > 0 && gh > 0) k_collect_patches<<<dim3((gw + 63) / 64, (gh + 3) / 4), dim3(64, 4), 0, s>>>(in, patches, n);
+    GCAM_LAUNCH_CHECK("collectPatches");
+    return GCAM_OK;
+}
+
+gcam_status launch_image_moments(View<const float4> in, double* partials, hipStream_t s) {
+    k_image_moments<<<kMomentBlocks, 256, 0, s>>>(in, partials);
+    GCAM_LAUNCH_CHECK("patch moments");
+    return GCAM_OK;
+}
+
+gcam_status launch_buffer_moments(const float* patches, int64_t n, double* partials, hipStream_t s) {
+    k_buffer_moments<<<kMomentBlocks, 256, 0, s>>>(patches, n, partials);
+    GCAM_LAUNCH_CHECK("patch moments");
+    return GCAM_OK;
+}
+
+gcam_status launch_pca_solve(const double* partials, int64_t n, float* basis, double* eigvals, hipStream_t s) {
+    k_pca_solve<<<1, 256, 0, s>>>(partials, kMomentBlocks, (double)n, basis, eigvals);
+    GCAM_LAUNCH_CHECK("PCA eigensolve");
+    return GCAM_OK;
+}
+
+gcam_status launch_pca_project(View<const float4> in, const float* basis, View<float> feat8, hipStream_t s) {
+    k_pca_project<<<dim3((in.w + 63) / 64, (in.h + 3) / 4), dim3(64, 4), 0, s>>>(in, basis, feat8.ptr, feat8.stride);
+    GCAM_LAUNCH_CHECK("pcaProjection");
+    return GCAM_OK;
+}
+
+}  // namespace gcam

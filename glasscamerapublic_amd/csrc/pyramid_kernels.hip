@@ -1,0 +1,413 @@
+// Pyramid resampling, Laplacian reconstruct and the noise-model fit for gfx950.
+//
+//   downsample      MetalPipeline/demosaic.metal:886-920
+//   subtract_noise  MetalPipeline/demosaic.metal:922-959
+//   noise_stats     MetalPipeline/demosaic.metal:1138-1159
+//   nlf_fit         MetalPipeline/pyramid_processor.cpp:186-323
+//
+// The resampling kernels are HBM-bound: one thread per output pixel, 64-wide
+// rows per wavefront so every load and store is a coalesced 16-byte (float4)
+// or 8-byte (float2) access; the neighbouring taps hit L1/L2.
+#include "gcam_internal.hpp"
+
+namespace gcam {
+
+namespace {
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
+    float t = (x - e0) / (e1 - e0);
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+// One axis of a linear-filtered, clamp-to-edge sample at texel position
+// num/den: u = num/den - 1/2 = (2 num - den) / (2 den), taps floor(u) and
+// floor(u)+1 clamped to the image, weight of the second tap frac(u).  All in
+// exact integer arithmetic, so power-of-two pyramids get exact 1/2, 1/4, 3/4.
+struct Tap {
+    int i0, i1;
+    float f;
+};
+__device__ __forceinline__ Tap lin_tap(int num, int den, int size) {
+    int n2 = 2 * num - den, d2 = 2 * den;
+    int fl = n2 >= 0 ? n2 / d2 : -((-n2 + d2 - 1) / d2);
+    int rem = n2 - fl * d2;
+    return Tap{clampi(fl, 0, size - 1), clampi(fl + 1, 0, size - 1), (float)rem / (float)d2};
+}
+
+template <class P>
+__device__ __forceinline__ float3 bilerp3(const View<const P>& im, const Tap& tx, const Tap& ty);
+
+template <>
+__device__ __forceinline__ float3 bilerp3<float4>(const View<const float4>& im, const Tap& tx, const Tap& ty) {
+    float4 t00 = im.at(tx.i0, ty.i0), t01 = im.at(tx.i1, ty.i0);
+    float4 t10 = im.at(tx.i0, ty.i1), t11 = im.at(tx.i1, ty.i1);
+    const float a = tx.f, b = ty.f;
+    float3 top = make_float3((1.0f - a) * t00.x + a * t01.x, (1.0f - a) * t00.y + a * t01.y,
+                             (1.0f - a) * t00.z + a * t01.z);
+    float3 bot = make_float3((1.0f - a) * t10.x + a * t11.x, (1.0f - a) * t10.y + a * t11.y,
+                             (1.0f - a) * t10.z + a * t11.z);
+    return make_float3((1.0f - b) * top.x + b * bot.x, (1.0f - b) * top.y + b * bot.y,
+                       (1.0f - b) * top.z + b * bot.z);
+}
+
+__device__ __forceinline__ float2 bilerp2(const View<const float2>& im, const Tap& tx, const Tap& ty) {
+    float2 t00 = im.at(tx.i0, ty.i0), t01 = im.at(tx.i1, ty.i0);
+    float2 t10 = im.at(tx.i0, ty.i1), t11 = im.at(tx.i1, ty.i1);
+    const float a = tx.f, b = ty.f;
+    float2 top = make_float2((1.0f - a) * t00.x + a * t01.x, (1.0f - a) * t00.y + a * t01.y);
+    float2 bot = make_float2((1.0f - a) * t10.x + a * t11.x, (1.0f - a) * t10.y + a * t11.y);
+    return make_float2((1.0f - b) * top.x + b * bot.x, (1.0f - b) * top.y + b * bot.y);
+}
+
+// downsampleImageXYZ: output o averages 4 linear samples at texel positions
+// (o + k) * in/out, k in {0, 1}, per axis (the reference's +-1/2 output-texel
+// offsets around the output centre).
+__global__ __launch_bounds__(256) void k_downsample4(View<const float4> in, View<float4> out) {
+    const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
+    if (ox >= out.w || oy >= out.h) return;
+    const Tap ty0 = lin_tap(oy * in.h, out.h, in.h), ty1 = lin_tap((oy + 1) * in.h, out.h, in.h);
+    const Tap tx0 = lin_tap(ox * in.w, out.w, in.w), tx1 = lin_tap((ox + 1) * in.w, out.w, in.w);
+    float3 s = bilerp3(in, tx0, ty0);
+    float3 acc = s;
+    s = bilerp3(in, tx1, ty0);
+    acc.x += s.x, acc.y += s.y, acc.z += s.z;
+    s = bilerp3(in, tx0, ty1);
+    acc.x += s.x, acc.y += s.y, acc.z += s.z;
+    s = bilerp3(in, tx1, ty1);
+    acc.x += s.x, acc.y += s.y, acc.z += s.z;
+    out.at(ox, oy) = make_float4(0.25f * acc.x, 0.25f * acc.y, 0.25f * acc.z, 0.0f);
+}
+
+// downsampleImageXY, the 2-channel gradient version.
+__global__ __launch_bounds__(256) void k_downsample2(View<const float2> in, View<float2> out) {
+    const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
+    if (ox >= out.w || oy >= out.h) return;
+    const Tap ty0 = lin_tap(oy * in.h, out.h, in.h), ty1 = lin_tap((oy + 1) * in.h, out.h, in.h);
+    const Tap tx0 = lin_tap(ox * in.w, out.w, in.w), tx1 = lin_tap((ox + 1) * in.w, out.w, in.w);
+    float2 acc = bilerp2(in, tx0, ty0);
+    float2 s = bilerp2(in, tx1, ty0);
+    acc.x += s.x, acc.y += s.y;
+    s = bilerp2(in, tx0, ty1);
+    acc.x += s.x, acc.y += s.y;
+    s = bilerp2(in, tx1, ty1);
+    acc.x += s.x, acc.y += s.y;
+    out.at(ox, oy) = make_float2(0.25f * acc.x, 0.25f * acc.y);
+}
+
+// subtractNoiseImage: Laplacian-style reconstruct of level i from the noisy
+// and the denoised level i+1 (both upsampled bilinearly), with optional
+// gradient-driven sharpening.
+__global__ __launch_bounds__(256) void k_subtract_noise(View<const float4> in, View<const float4> in1,
+                                                        View<const float4> den1, View<const float2> grad,
+                                                        float luma_weight, float sharpening, float nlf_x,
+                                                        float nlf_y, const NlfDev* nlf_level, float thr_y,
+                                                        View<float4> out) {
+    const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
+    if (ox >= out.w || oy >= out.h) return;
+    const Tap tyy = lin_tap((2 * oy + 1) * in1.h, 2 * out.h, in1.h);
+    const Tap txx = lin_tap((2 * ox + 1) * in1.w, 2 * out.w, in1.w);
+    const float4 ip = in.at(ox, oy);
+    const float3 p1 = bilerp3(in1, txx, tyy);
+    const float3 d1 = bilerp3(den1, txx, tyy);
+    float3 dp = make_float3(ip.x - luma_weight * (p1.x - d1.x), ip.y - (p1.y - d1.y), ip.z - (p1.z - d1.z));
+    float alpha = sharpening;
+    if (alpha > 1.0f) {
+        if (nlf_level) {
+            nlf_x = nlf_level->a[0] * thr_y;
+            nlf_y = nlf_level->b[0] * thr_y;
+        }
+        float gradient = 0.0f;
+        if (grad.ptr) {
+            const float2 g = grad.at(ox, oy);
+            gradient = sqrtf(g.x * g.x + g.y * g.y);
+        }
+        const float sigma = sqrtf(nlf_x + nlf_y * d1.x);
+        const float detail = smoothstepf(sigma, 4.0f * sigma, gradient) *
+                             (1.0f - smoothstepf(0.75f, 0.95f, d1.x)) * smoothstepf(0.05f, 0.1f, d1.x);
+        alpha = 1.0f + (alpha - 1.0f) * detail;
+    }
+    float4 o;
+    o.x = fmaxf(d1.x + (dp.x - d1.x) * alpha, 0.0f);
+    o.y = d1.y + (dp.y - d1.y) * alpha;
+    o.z = d1.z + (dp.z - d1.z) * alpha;
+    o.w = ip.w;
+    out.at(ox, oy) = o;
+}
+
+// basicNoiseStatistics: 5x5 mean of Y and population variances of Y, Cb, Cr.
+__global__ __launch_bounds__(256) void k_noise_stats(View<const float4> in, View<float4> out) {
+    const int px = blockIdx.x * 64 + threadIdx.x, py = blockIdx.y * 4 + threadIdx.y;
+    if (px >= out.w || py >= out.h) return;
+    float3 sum = make_float3(0, 0, 0), sq = make_float3(0, 0, 0);
+    for (int y = -2; y <= 2; y++) {
+        const int sy = clampi(py + y, 0, in.h - 1);
+        for (int x = -2; x <= 2; x++) {
+            const float4 s = in.at(clampi(px + x, 0, in.w - 1), sy);
+            sum.x += s.x, sum.y += s.y, sum.z += s.z;
+            sq.x += s.x * s.x, sq.y += s.y * s.y, sq.z += s.z * s.z;
+        }
+    }
+    float4 o;
+    o.x = sum.x / 25.0f;
+    o.y = (sq.x - (sum.x * sum.x) / 25.0f) / 25.0f;
+    o.z = (sq.y - (sum.y * sum.y) / 25.0f) / 25.0f;
+    o.w = (sq.z - (sum.z * sum.z) / 25.0f) / 25.0f;
+    out.at(px, py) = o;
+}
+
+// ---------------- noise-model fit (FP64) ----------------
+
+constexpr int kNlfMaxValues = 13;
+
+__device__ __forceinline__ double floor_max(double v, double fl) { return isnan(v) ? fl : fmax(v, fl); }
+
+__device__ __forceinline__ bool nlf_selected(const float4 ns, const double vmax[3], double& m, double v[3]) {
+    m = ns.x;
+    v[0] = ns.y;
+    v[1] = ns.z;
+    v[2] = ns.w;
+    const bool valid = !(isnan(m) || isnan(v[0]) || isnan(v[1]) || isnan(v[2]));
+    return valid && m >= 0.001 && m <= 0.9 && v[0] <= vmax[0] && v[1] <= vmax[1] && v[2] <= vmax[2];
+}
+
+// Block-wide deterministic sum of `nv` per-thread doubles; thread 0 writes them.
+template <int NV>
+__device__ void block_sum_write(double (&acc)[NV], double* dst) {
+    __shared__ double red[256];
+    for (int v = 0; v < NV; v++) {
+        red[threadIdx.x] = acc[v];
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) dst[v] = red[0];
+        __syncthreads();
+    }
+}
+
+template <int PASS>
+struct NlfPassWidth;
+template <>
+struct NlfPassWidth<0> { static constexpr int value = 6; };
+template <>
+struct NlfPassWidth<1> { static constexpr int value = 9; };
+template <>
+struct NlfPassWidth<2> { static constexpr int value = 3; };
+template <>
+struct NlfPassWidth<3> { static constexpr int value = 13; };
+
+// One statistics pass over every pixel; per-block partial sums.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_nlf_pass(View<const float4> stats, const NlfFitState* st,
+                                                  double* partials) {
+    constexpr int NV = NlfPassWidth<PASS>::value;
+    double acc[NV];
+    for (int v = 0; v < NV; v++) acc[v] = 0.0;
+    double vmax[3], A[3], B[3], e2[3];
+    for (int c = 0; c < 3; c++) {
+        vmax[c] = st->varianceMax[c];
+        A[c] = st->A[c];
+        B[c] = st->B[c];
+        e2[c] = st->err2[c];
+    }
+    const size_t npx = (size_t)stats.w * stats.h;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < npx; k += (size_t)gridDim.x * 256) {
+        const int x = (int)(k % stats.w), y = (int)(k / stats.w);
+        const float4 ns = stats.at(x, y);
+        if (PASS == 0) {  // pyramid_processor.cpp:190-201
+            const double v0 = ns.y, v1 = ns.z, v2 = ns.w;
+            const bool valid = !isnan(v0) && !isnan(v1) && !isnan(v2) && v0 > 0.0 && v1 > 0.0 && v2 > 0.0;
+            if (valid) {
+                double mx = fmax(fmax(log10(v0), log10(v1)), log10(v2));
+                mx = fmin(fmax(mx, -6.0), -1.0);
+                const int entry = 6 + (int)mx;
+                for (int e = 0; e < 6; e++) acc[e] += (e == entry) ? 1.0 : 0.0;
+            }
+        } else {
+            double m, v[3];
+            if (!nlf_selected(ns, vmax, m, v)) continue;
+            if (PASS == 1) {  // :223-236
+                acc[0] += m;
+                acc[1] += m * m;
+                for (int c = 0; c < 3; c++) {
+                    acc[2 + c] += v[c];
+                    acc[5 + c] += m * v[c];
+                }
+                acc[8] += 1.0;
+            } else if (PASS == 2) {  // :243-256
+                for (int c = 0; c < 3; c++) {
+                    const double diff = A[c] + B[c] * m - v[c];
+                    acc[c] += diff * diff;
+                }
+            } else {  // :269-291
+                double dsq[3];
+                bool fits = true;
+                for (int c = 0; c < 3; c++) {
+                    const double diff = fabs(A[c] + B[c] * m - v[c]);
+                    dsq[c] = diff * diff;
+                    fits = fits && dsq[c] <= e2[c];
+                }
+                if (fits) {
+                    acc[0] += m;
+                    acc[1] += m * m;
+                    for (int c = 0; c < 3; c++) {
+                        acc[2 + c] += v[c];
+                        acc[5 + c] += m * v[c];
+                        acc[9 + c] += dsq[c];
+                    }
+                    acc[8] += 1.0;
+                } else {
+                    acc[12] += 1.0;
+                }
+            }
+        }
+    }
+    block_sum_write<NV>(acc, partials + (size_t)blockIdx.x * kNlfMaxValues);
+}
+
+// Sum the per-block partials (fixed order) and advance the fit state.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_nlf_finalize(const double* partials, int nblocks, NlfFitState* st,
+                                                      float exposure_multiplier, double npx, NlfDev* nlf_out) {
+    constexpr int NV = NlfPassWidth<PASS>::value;
+    __shared__ double tot[kNlfMaxValues];
+    __shared__ double red[256];
+    for (int v = 0; v < NV; v++) {
+        double a = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += 256) a += partials[(size_t)b * kNlfMaxValues + v];
+        red[threadIdx.x] = a;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) tot[v] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (PASS == 0) {  // :204-208
+        for (int e = 0; e < 6; e++) st->hist[e] = tot[e];
+        const double* h = tot;
+        const double vm = h[0] > 1e3 ? 1.0e-5 : h[1] > 1e3 ? 1.0e-4 : h[2] > 1e3 ? 1.0e-3
+                        : h[3] > 1e3 ? 1.0e-2 : h[4] > 1e3 ? 1.0e-1 : 1.0;
+        for (int c = 0; c < 3; c++) st->varianceMax[c] = vm;
+    } else if (PASS == 1) {  // :239-240
+        const double N = tot[8], s_x = tot[0], s_xx = tot[1];
+        for (int c = 0; c < 3; c++) {
+            const double B = floor_max((N * tot[5 + c] - s_x * tot[2 + c]) / (N * s_xx - s_x * s_x), 1e-8);
+            st->B[c] = B;
+            st->A[c] = floor_max((tot[2 + c] - B * s_x) / N, 1e-8);
+        }
+        st->n1 = N;
+    } else if (PASS == 2) {  // :256, :259
+        for (int c = 0; c < 3; c++) {
+            st->err2[c] = tot[c] / st->n1;
+            st->varianceMax[c] = st->B[c];
+        }
+    } else {  // :292-319
+        const double N = tot[8], s_x = tot[0], s_xx = tot[1];
+        bool better = true;
+        for (int c = 0; c < 3; c++) {
+            st->newErr2[c] = tot[9 + c] / N;
+            better = better && st->newErr2[c] <= st->err2[c];
+        }
+        st->n2 = N;
+        st->discarded = tot[12];
+        st->accepted = 0;
+        if (better && N / npx > 0.01) {
+            for (int c = 0; c < 3; c++) {
+                const double B = floor_max((N * tot[5 + c] - s_x * tot[2 + c]) / (N * s_xx - s_x * s_x), 1e-8);
+                st->B[c] = B;
+                st->A[c] = floor_max((tot[2 + c] - B * s_x) / N, 1e-8);
+            }
+            st->accepted = 1;
+        }
+        const double adj = (double)exposure_multiplier * (double)exposure_multiplier;
+        for (int c = 0; c < 3; c++) {
+            nlf_out->a[c] = (float)(st->A[c] * adj);
+            nlf_out->b[c] = (float)(st->B[c] * adj);
+        }
+    }
+}
+
+__global__ void k_nlf_init(NlfFitState* st) {
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < 3; c++) st->varianceMax[c] = st->A[c] = st->B[c] = st->err2[c] = 0.0;
+    }
+}
+
+struct NlfLevels {
+    NlfDev lv[GCAM_MAX_LEVELS];
+};
+
+__global__ void k_set_nlf(NlfLevels v, int levels, NlfDev* dev) {
+    const int i = threadIdx.x;
+    if (i < levels) dev[i] = v.lv[i];
+}
+
+inline dim3 grid64x4(int w, int h) { return dim3((w + 63) / 64, (h + 3) / 4); }
+
+}  // namespace
+
+gcam_status launch_downsample4(View<const float4> in, View<float4> out, hipStream_t s) {
+    k_downsample4<<<grid64x4(out.w, out.h), dim3(64, 4), 0, s>>>(in, out);
+    GCAM_LAUNCH_CHECK("downsampleImageXYZ");
+    return GCAM_OK;
+}
+
+gcam_status launch_downsample2(View<const float2> in, View<float2> out, hipStream_t s) {
+    k_downsample2<<<grid64x4(out.w, out.h), dim3(64, 4), 0, s>>>(in, out);
+    GCAM_LAUNCH_CHECK("downsampleImageXY");
+    return GCAM_OK;
+}
+
+gcam_status launch_subtract_noise(View<const float4> in, View<const float4> in1, View<const float4> den1,
+                                  View<const float2> grad, float luma_weight, float sharpening, float nlf_x,
+                                  float nlf_y, const NlfDev* nlf_level, float thr_y, View<float4> out,
+                                  hipStream_t s) {
+    k_subtract_noise<<<grid64x4(out.w, out.h), dim3(64, 4), 0, s>>>(in, in1, den1, grad, luma_weight, sharpening,
+                                                                    nlf_x, nlf_y, nlf_level, thr_y, out);
+    GCAM_LAUNCH_CHECK("subtractNoiseImage");
+    return GCAM_OK;
+}
+
+gcam_status launch_noise_stats(View<const float4> in, View<float4> out, hipStream_t s) {
+    k_noise_stats<<<grid64x4(out.w, out.h), dim3(64, 4), 0, s>>>(in, out);
+    GCAM_LAUNCH_CHECK("basicNoiseStatistics");
+    return GCAM_OK;
+}
+
+// partials: kNlfBlocks * 13 doubles of scratch.
+gcam_status launch_nlf_fit(View<const float4> stats, float exposure_multiplier, double* partials,
+                           NlfFitState* state, NlfDev* nlf_out, hipStream_t s) {
+    const double npx = (double)stats.w * stats.h;
+    k_nlf_init<<<1, 64, 0, s>>>(state);
+    k_nlf_pass<0><<<kNlfBlocks, 256, 0, s>>>(stats, state, partials);
+    k_nlf_finalize<0><<<1, 256, 0, s>>>(partials, kNlfBlocks, state, exposure_multiplier, npx, nlf_out);
+    k_nlf_pass<1><<<kNlfBlocks, 256, 0, s>>>(stats, state, partials);
+    k_nlf_finalize<1><<<1, 256, 0, s>>>(partials, kNlfBlocks, state, exposure_multiplier, npx, nlf_out);
+    k_nlf_pass<2><<<kNlfBlocks, 256, 0, s>>>(stats, state, partials);
+    k_nlf_finalize<2><<<1, 256, 0, s>>>(partials, kNlfBlocks, state, exposure_multiplier, npx, nlf_out);
+    k_nlf_pass<3><<<kNlfBlocks, 256, 0, s>>>(stats, state, partials);
+    k_nlf_finalize<3><<<1, 256, 0, s>>>(partials, kNlfBlocks, state, exposure_multiplier, npx, nlf_out);
+    GCAM_LAUNCH_CHECK("MeasureYCbCrNLF");
+    return GCAM_OK;
+}
+
+gcam_status launch_set_nlf(const gcam_ycbcr_nlf* host_levels, int levels, NlfDev* dev, hipStream_t s) {
+    NlfLevels v{};
+    for (int i = 0; i < levels && i < GCAM_MAX_LEVELS; i++)
+        for (int c = 0; c < 3; c++) {
+            v.lv[i].a[c] = host_levels[i].a[c];
+            v.lv[i].b[c] = host_levels[i].b[c];
+        }
+    k_set_nlf<<<1, 64, 0, s>>>(v, levels, dev);
+    GCAM_LAUNCH_CHECK("set noise model");
+    return GCAM_OK;
+}
+
+}  // namespace gcam

@@ -1,0 +1,258 @@
+"""ctypes wrapper of the CPU oracle (``libgcam_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: the oracle restates the reference's denoise path
+(MetalPipeline/demosaic.metal, MetalPipeline/PCA.cpp,
+MetalPipeline/pyramid_processor.cpp) on the CPU so that the HIP product can be
+checked against it.  Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU
+baseline leg of ``bench.py`` import this module; the product package never does.
+
+Images are numpy arrays: YCbCr images are (H, W, 4), gradients (H, W, 2),
+features (H, W, 8), all C-contiguous float32 or float64.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libgcam_oracle.so")
+_lib = None
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile; returns the library path."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "libgcam_oracle.so"], check=True)
+    return _LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_fp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_d = ctypes.c_double
+
+
+class DenoiseParamsC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("luma", "chroma", "chromaBoost", "gradientBoost", "gradientThreshold", "sharpening")]
+
+
+def _declare(L):
+    for sfx, R in (("f32", _f), ("f64", _d)):
+        getattr(L, f"oracle_downsample_{sfx}").argtypes = [_fp, _i, _i, _i, _fp, _i, _i]
+        getattr(L, f"oracle_subtract_noise_{sfx}").argtypes = [_fp, _i, _i, _fp, _fp, _i, _i, _fp, R, R, R, R, _fp]
+        getattr(L, f"oracle_collect_patches_{sfx}").argtypes = [_fp, _i, _i, _fp]
+        getattr(L, f"oracle_collect_patches_{sfx}").restype = _i
+        getattr(L, f"oracle_patch_covariance_{sfx}").argtypes = [_fp, _i, _fp]
+        getattr(L, f"oracle_build_pca_space_{sfx}").argtypes = [_fp, _i, _fp, _fp]
+        getattr(L, f"oracle_pca_project_{sfx}").argtypes = [_fp, _i, _i, _fp, _fp]
+        getattr(L, f"oracle_block_match_{sfx}").argtypes = [_fp, _fp, _fp, _i, _i, _fp, _fp, _fp, R, R, R, R, _i, _fp]
+        getattr(L, f"oracle_noise_stats_{sfx}").argtypes = [_fp, _i, _i, _fp]
+        getattr(L, f"oracle_pyramid_denoise_{sfx}").argtypes = [_fp, _fp, _i, _i, _i, _i, _fp, _fp, R, R, _i,
+                                                                 _fp, _fp, _fp, _fp]
+        getattr(L, f"oracle_pyramid_denoise_{sfx}").restype = _i
+    L.oracle_eigh_jacobi.argtypes = [_fp, _i, _fp, _fp]
+    L.oracle_fit_ycbcr_nlf.argtypes = [_fp, _i, _i, _d, _fp, _fp, _fp]
+    L.oracle_fit_ycbcr_nlf.restype = _i
+    L.oracle_set_threads.argtypes = [_i]
+    L.oracle_get_threads.restype = _i
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _sfx(dtype):
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return "f32"
+    if dt == np.float64:
+        return "f64"
+    raise TypeError(f"oracle works in float32 or float64, got {dt}")
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def set_threads(n: int) -> None:
+    lib().oracle_set_threads(int(n))
+
+
+def level_sizes(w: int, h: int, levels: int):
+    """Pyramid level sizes, width / 2^i (MetalPipeline/pyramid_processor.cpp:38-47)."""
+    return [(w >> i, h >> i) for i in range(levels)]
+
+
+def downsample(img, out_w=None, out_h=None):
+    h, w, ch = img.shape
+    out_w = w // 2 if out_w is None else out_w
+    out_h = h // 2 if out_h is None else out_h
+    img = _c(img, img.dtype)
+    out = np.zeros((out_h, out_w, ch), img.dtype)
+    getattr(lib(), f"oracle_downsample_{_sfx(img.dtype)}")(_p(img), w, h, ch, _p(out), out_w, out_h)
+    return out
+
+
+def subtract_noise(img, img1, den1, grad=None, luma_weight=1.0, sharpening=1.0, nlf=(0.0, 0.0)):
+    dt = img.dtype
+    h, w, _ = img.shape
+    h1, w1, _ = img1.shape
+    img, img1, den1 = _c(img, dt), _c(img1, dt), _c(den1, dt)
+    grad = None if grad is None else _c(grad, dt)
+    out = np.zeros_like(img)
+    getattr(lib(), f"oracle_subtract_noise_{_sfx(dt)}")(_p(img), w, h, _p(img1), _p(den1), w1, h1, _p(grad),
+                                                        luma_weight, sharpening, nlf[0], nlf[1], _p(out))
+    return out
+
+
+def collect_patches(img):
+    dt = img.dtype
+    h, w, _ = img.shape
+    img = _c(img, dt)
+    n = (w * h) // 64
+    out = np.zeros((max(n, 1), 25), dt)
+    n2 = getattr(lib(), f"oracle_collect_patches_{_sfx(dt)}")(_p(img), w, h, _p(out))
+    assert n2 == n
+    return out[:n]
+
+
+def patch_covariance(patches):
+    patches = np.ascontiguousarray(patches)
+    cov = np.zeros((25, 25), np.float64)
+    getattr(lib(), f"oracle_patch_covariance_{_sfx(patches.dtype)}")(_p(patches), patches.shape[0], _p(cov))
+    return cov
+
+
+def eigh_jacobi(mat):
+    mat = _c(mat, np.float64)
+    n = mat.shape[0]
+    vals = np.zeros(n, np.float64)
+    vecs = np.zeros((n, n), np.float64)
+    lib().oracle_eigh_jacobi(_p(mat), n, _p(vals), _p(vecs))
+    return vals, vecs
+
+
+def build_pca_space(patches):
+    """FP64 basis (25, 8) and the 25 eigenvalues, descending (MetalPipeline/PCA.cpp:23-56)."""
+    patches = np.ascontiguousarray(patches)
+    basis = np.zeros((25, 8), np.float64)
+    ev = np.zeros(25, np.float64)
+    getattr(lib(), f"oracle_build_pca_space_{_sfx(patches.dtype)}")(_p(patches), patches.shape[0], _p(basis),
+                                                                     _p(ev))
+    return basis, ev
+
+
+def pca_project(img, basis):
+    dt = img.dtype
+    h, w, _ = img.shape
+    img = _c(img, dt)
+    basis = _c(basis, dt)
+    feat = np.zeros((h, w, 8), dt)
+    getattr(lib(), f"oracle_pca_project_{_sfx(dt)}")(_p(img), w, h, _p(basis), _p(feat))
+    return feat
+
+
+def block_match(img, feat, var_a, var_b, thr, chroma_boost=1.0, gradient_boost=0.0, gradient_threshold=1.0,
+                lens_shading=0.0, radius=10, grad=None):
+    dt = img.dtype
+    h, w, _ = img.shape
+    img, feat = _c(img, dt), _c(feat, dt)
+    grad = None if grad is None else _c(grad, dt)
+    va, vb, th = (_c(np.asarray(v).reshape(3), dt) for v in (var_a, var_b, thr))
+    out = np.zeros_like(img)
+    getattr(lib(), f"oracle_block_match_{_sfx(dt)}")(_p(img), _p(grad), _p(feat), w, h, _p(va), _p(vb), _p(th),
+                                                     chroma_boost, gradient_boost, gradient_threshold, lens_shading,
+                                                     radius, _p(out))
+    return out
+
+
+def noise_stats(img):
+    dt = img.dtype
+    h, w, _ = img.shape
+    img = _c(img, dt)
+    out = np.zeros_like(img)
+    getattr(lib(), f"oracle_noise_stats_{_sfx(dt)}")(_p(img), w, h, _p(out))
+    return out
+
+
+@dataclass
+class NlfFit:
+    A: np.ndarray
+    B: np.ndarray
+    accepted: bool
+    histogram: np.ndarray
+    n_first: float
+    n_second: float
+    err2: np.ndarray
+    new_err2: np.ndarray
+    discarded: int
+    variance_max: float
+
+
+def fit_ycbcr_nlf(stats, exposure_multiplier=1.0) -> NlfFit:
+    stats = _c(stats, np.float32)
+    h, w, _ = stats.shape
+    A = np.zeros(3)
+    B = np.zeros(3)
+    diag = np.zeros(16)
+    acc = lib().oracle_fit_ycbcr_nlf(_p(stats), w, h, exposure_multiplier, _p(A), _p(B), _p(diag))
+    return NlfFit(A, B, bool(acc), diag[:6].copy(), diag[6], diag[7], diag[8:11].copy(), diag[11:14].copy(),
+                  int(diag[14]), diag[15])
+
+
+@dataclass
+class PyramidResult:
+    denoised: list
+    subtracted: list
+    bases: np.ndarray
+    eigvals: np.ndarray
+    nlf: np.ndarray
+
+
+def pyramid_denoise(image, params, nlf, levels=5, radius=10, gradient=None, exposure_multiplier=1.0,
+                    lens_shading=0.0, calibrate=False, dtype=np.float32) -> PyramidResult:
+    """Whole multi-level denoise (MetalPipeline/pyramid_processor.cpp:96-172).
+
+    params: sequence of ``levels`` dicts/objects with the DenoiseParameters fields.
+    nlf: (levels, 2, 3) array of (A, B) per level; returned (updated if calibrating).
+    """
+    image = _c(image, dtype)
+    h, w, _ = image.shape
+    grad = None if gradient is None else _c(gradient, dtype)
+    P = (DenoiseParamsC * levels)()
+    for i, p in enumerate(params):
+        for k in ("luma", "chroma", "chromaBoost", "gradientBoost", "gradientThreshold", "sharpening"):
+            setattr(P[i], k, float(p[k] if isinstance(p, dict) else getattr(p, k)))
+    nlf_arr = np.ascontiguousarray(np.asarray(nlf, np.float64).reshape(levels, 6).copy())
+    sizes = level_sizes(w, h, levels)
+    total = sum(a * b for a, b in sizes)
+    den = np.zeros(total * 4, dtype)
+    sub = np.zeros(total * 4, dtype)
+    bases = np.zeros((levels, 25, 8))
+    ev = np.zeros((levels, 25))
+    rc = getattr(lib(), f"oracle_pyramid_denoise_{_sfx(dtype)}")(
+        _p(image), _p(grad), w, h, levels, radius, ctypes.cast(P, ctypes.c_void_p), _p(nlf_arr),
+        exposure_multiplier, lens_shading, int(bool(calibrate)), _p(den), _p(sub), _p(bases), _p(ev))
+    if rc != 0:
+        raise ValueError("pyramid level too small for PCA patch sampling")
+    dl, sl, o = [], [], 0
+    for (lw, lh) in sizes:
+        n = lw * lh * 4
+        dl.append(den[o:o + n].reshape(lh, lw, 4))
+        sl.append(sub[o:o + n].reshape(lh, lw, 4))
+        o += n
+    return PyramidResult(dl, sl, bases, ev, nlf_arr.reshape(levels, 2, 3))
