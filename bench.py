@@ -50,7 +50,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="12mp", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp32", choices=["fp32"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"])
+    ap.add_argument("--bm-rows", type=int, default=0, help="block-matching output rows per thread (0 = default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: min(cores, 16))")
     args = ap.parse_args()
@@ -71,7 +72,8 @@ def main() -> int:
     stream = torch.cuda.current_stream()
 
     ycc = gcam.synth_frame(w, h, a, b, scene_seed=1234, noise_seed=5678, device=dev)
-    proc = gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius, precision=args.precision, device=dev)
+    proc = gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius, precision=args.precision, device=dev,
+                                 bm_rows_per_thread=args.bm_rows)
     params = [gcam.DenoiseParameters() for _ in range(levels)]
     nlf = [((1e-8,) * 3, (1e-8,) * 3) for _ in range(levels)]
 
@@ -156,7 +158,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}[args.precision],
         "data": "synthetic",
         "config": {
             "workload": f"{w}x{h} RGGB 12-bit simplex scene, Poisson-Gaussian a={a:g} b={b:g}, bilinear demosaic "
@@ -167,7 +169,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_block_match<{radius}>",
+            "kernel": f"k_block_match{'' if args.precision == 'fp32' else '_lp'}<{radius}>",
             "achieved": round(bm_gbs, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -24,7 +24,7 @@ gcam_status set_error(gcam_status st, const std::string& msg) {
 using namespace gcam;
 
 struct gcam_pyramid {
-    int width = 0, height = 0, levels = 0, radius = 10, precision = GCAM_PRECISION_FP32, device = 0;
+    int width = 0, height = 0, levels = 0, radius = 10, precision = GCAM_PRECISION_FP32, bm_rows = 0, device = 0;
     int W[GCAM_MAX_LEVELS] = {}, H[GCAM_MAX_LEVELS] = {};
     char* arena = nullptr;        // one allocation for every fixed-size buffer
     float2* grad_arena = nullptr; // gradient pyramid, allocated on first use
@@ -111,13 +111,17 @@ gcam_status gcam_pyramid_create(int32_t width, int32_t height, int32_t levels, c
         return set_error(GCAM_ERR_INVALID_ARGUMENT, "levels must be 1.." + std::to_string(GCAM_MAX_LEVELS));
     if (width < 1 || height < 1 || width > kMaxDim || height > kMaxDim)
         return set_error(GCAM_ERR_INVALID_ARGUMENT, "image size out of range 1.." + std::to_string(kMaxDim));
-    gcam_config cfg{10, GCAM_PRECISION_FP32, {0, 0, 0, 0, 0, 0}};
+    gcam_config cfg{10, GCAM_PRECISION_FP32, 0, {0, 0, 0, 0, 0}};
     if (config) cfg = *config;
     if (cfg.search_radius < 1 || cfg.search_radius > GCAM_MAX_SEARCH_RADIUS)
         return set_error(GCAM_ERR_INVALID_ARGUMENT, "search_radius must be 1..10");
-    if (cfg.precision != GCAM_PRECISION_FP32)
-        return set_error(GCAM_ERR_UNSUPPORTED, "precision " + std::to_string(cfg.precision) + " is not available");
-    for (int i = 0; i < 6; i++)
+    if (cfg.precision != GCAM_PRECISION_FP32 && cfg.precision != GCAM_PRECISION_FP16 &&
+        cfg.precision != GCAM_PRECISION_BF16)
+        return set_error(GCAM_ERR_INVALID_ARGUMENT, "unknown precision " + std::to_string(cfg.precision));
+    if (cfg.bm_rows_per_thread != 0 && cfg.bm_rows_per_thread != 1 && cfg.bm_rows_per_thread != 2 &&
+        cfg.bm_rows_per_thread != 4)
+        return set_error(GCAM_ERR_INVALID_ARGUMENT, "bm_rows_per_thread must be 0, 1, 2 or 4");
+    for (int i = 0; i < 5; i++)
         if (cfg.reserved[i] != 0) return set_error(GCAM_ERR_INVALID_ARGUMENT, "gcam_config.reserved must be zero");
     for (int i = 0; i < levels; i++) {
         const long w = width >> i, h = height >> i;
@@ -135,6 +139,7 @@ gcam_status gcam_pyramid_create(int32_t width, int32_t height, int32_t levels, c
     p->levels = levels;
     p->radius = cfg.search_radius;
     p->precision = cfg.precision;
+    p->bm_rows = cfg.bm_rows_per_thread;
     p->device = dev;
     for (int i = 0; i < levels; i++) {
         p->W[i] = width >> i;
@@ -284,8 +289,10 @@ gcam_status gcam_pyramid_denoise(gcam_pyramid* p, const gcam_denoise_params* par
         st = launch_pca_solve(p->moment_partials, n, p->basis + 200 * i, p->eigvals + 25 * i, s);
         if (st != GCAM_OK) return st;
         const View<float> fv = view<float>(p->feat, p->W[i], p->H[i]);
-        st = launch_pca_project(layer, p->basis + 200 * i, fv, s);
-        if (st != GCAM_OK) return st;
+        if (p->precision == GCAM_PRECISION_FP32) {
+            st = launch_pca_project(layer, p->basis + 200 * i, fv, s);
+            if (st != GCAM_OK) return st;
+        }
         GCAM_MARK(i, GCAM_STAGE_PCA, 1);
         BmParams bp;
         bp.thr[0] = params[i].luma;  // nflMultiplier (:52-56)
@@ -296,13 +303,17 @@ gcam_status gcam_pyramid_denoise(gcam_pyramid* p, const gcam_denoise_params* par
         bp.gradientThreshold = params[i].gradientThreshold;
         bp.lensShading = lens_shading_correction;
         GCAM_MARK(i, GCAM_STAGE_BLOCK_MATCH, 0);
-        st = launch_block_match(layer, gradv(i), View<const float>{fv.ptr, fv.w, fv.h, fv.stride}, p->nlf_dev + i, bp,
-                                p->radius, view<float4>(p->den[i], p->W[i], p->H[i]), s);
+        if (p->precision == GCAM_PRECISION_FP32)
+            st = launch_block_match(layer, gradv(i), View<const float>{fv.ptr, fv.w, fv.h, fv.stride}, p->nlf_dev + i,
+                                    bp, p->radius, p->bm_rows, view<float4>(p->den[i], p->W[i], p->H[i]), s);
+        else
+            st = launch_block_match_lp(layer, gradv(i), p->basis + 200 * i, p->nlf_dev + i, bp, p->radius, p->bm_rows,
+                                       p->precision, view<float4>(p->den[i], p->W[i], p->H[i]), s);
         if (st != GCAM_OK) return st;
         GCAM_MARK(i, GCAM_STAGE_BLOCK_MATCH, 1);
     }
 #undef GCAM_MARK
-    p->feature_level = 0;
+    p->feature_level = p->precision == GCAM_PRECISION_FP32 ? 0 : -1;
 
     if (calibrate_from_image) {
         NlfDev host[GCAM_MAX_LEVELS];
@@ -485,7 +496,46 @@ gcam_status gcam_block_matching_denoise_image(const gcam_image* input, const gca
                                 gradient ? view<const float2>(*gradient) : View<const float2>{nullptr, 0, 0, 0},
                                 View<const float>{reinterpret_cast<const float*>(features->data), features->width,
                                                   features->height, features->stride},
-                                nd, bp, search_radius, view<float4>(*output), s);
+                                nd, bp, search_radius, 0, view<float4>(*output), s);
+    GCAM_HIP_TRY(hipFreeAsync(nd, s));
+    return st;
+}
+
+gcam_status gcam_block_matching_denoise_image_lp(const gcam_image* input, const gcam_image* gradient,
+                                                 const float* basis, const float var_a[3], const float var_b[3],
+                                                 const float threshold_multipliers[3], float chroma_boost,
+                                                 float gradient_boost, float gradient_threshold,
+                                                 float lens_shading_correction, int32_t search_radius,
+                                                 int32_t precision, gcam_image* output, void* stream) {
+    gcam_status st = check_image(input, GCAM_PIXEL_RGBA32F, "input");
+    if (st == GCAM_OK) st = check_image(output, GCAM_PIXEL_RGBA32F, "output", input->width, input->height);
+    if (st == GCAM_OK && gradient) st = check_image(gradient, GCAM_PIXEL_RG32F, "gradient", input->width, input->height);
+    if (st != GCAM_OK) return st;
+    if (!basis || !var_a || !var_b || !threshold_multipliers)
+        return set_error(GCAM_ERR_INVALID_ARGUMENT, "NULL parameter");
+    if (search_radius < 1 || search_radius > GCAM_MAX_SEARCH_RADIUS)
+        return set_error(GCAM_ERR_INVALID_ARGUMENT, "search radius must be 1..10");
+    if (precision != GCAM_PRECISION_FP16 && precision != GCAM_PRECISION_BF16)
+        return set_error(GCAM_ERR_INVALID_ARGUMENT, "precision must be GCAM_PRECISION_FP16 or GCAM_PRECISION_BF16");
+    hipStream_t s = as_stream(stream);
+    NlfDev* nd = nullptr;
+    GCAM_HIP_TRY(hipMallocAsync((void**)&nd, sizeof(NlfDev), s));
+    gcam_ycbcr_nlf hn;
+    for (int c = 0; c < 3; c++) {
+        hn.a[c] = var_a[c];
+        hn.b[c] = var_b[c];
+    }
+    st = launch_set_nlf(&hn, 1, nd, s);
+    BmParams bp;
+    for (int c = 0; c < 3; c++) bp.thr[c] = threshold_multipliers[c];
+    bp.chromaBoost = chroma_boost;
+    bp.gradientBoost = gradient_boost;
+    bp.gradientThreshold = gradient_threshold;
+    bp.lensShading = lens_shading_correction;
+    if (st == GCAM_OK)
+        st = launch_block_match_lp(view<const float4>(*input),
+                                   gradient ? view<const float2>(*gradient) : View<const float2>{nullptr, 0, 0, 0},
+                                   basis, nd, bp, search_radius, 0, precision, view<float4>(*output), s);
     GCAM_HIP_TRY(hipFreeAsync(nd, s));
     return st;
 }

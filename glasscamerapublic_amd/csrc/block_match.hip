@@ -34,34 +34,37 @@ __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
     return t * t * (3.0f - 2.0f * t);
 }
 
-// directionWeight per squared offset |q - p|^2 (0 .. 2 R^2).
+// directionWeight per squared offset |q - p|^2 (0 .. 2 R^2), and its log2.
 struct DirTable {
     float w[2 * GCAM_MAX_SEARCH_RADIUS * GCAM_MAX_SEARCH_RADIUS + 1];
+    float lw[2 * GCAM_MAX_SEARCH_RADIUS * GCAM_MAX_SEARCH_RADIUS + 1];
 };
 
 constexpr float kLog2e = 1.4426950408889634f;
 
-template <int R>
+// Tile of one workgroup: TW x TH threads, each owning PV vertically adjacent
+// output pixels, so the tile is TW x (TH * PV) pixels plus an R-pixel apron.
+template <int R, int PV, int TW, int TH>
 struct BmTile {
-    static constexpr int TW = 32;
-    static constexpr int TH = R > 6 ? 4 : 8;
     static constexpr int LW = TW + 2 * R;
-    static constexpr int LH = TH + 2 * R;
+    static constexpr int LH = TH * PV + 2 * R;
     static constexpr int LN = LW * LH;
+    static constexpr int THREADS = TW * TH;
+    static_assert(LN * 48 <= 64 * 1024, "LDS tile too large");
 };
 
-template <int R>
-__global__ __launch_bounds__(256) void k_block_match(View<const float4> in, View<const float2> grad,
-                                                     const float* __restrict__ feat, int feat_stride,
-                                                     const NlfDev* __restrict__ nlf, BmParams p, DirTable dir,
-                                                     View<float4> out) {
-    using T = BmTile<R>;
+template <int R, int PV, int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void k_block_match(View<const float4> in, View<const float2> grad,
+                                                        const float* __restrict__ feat, int feat_stride,
+                                                        const NlfDev* __restrict__ nlf, BmParams p, DirTable dir,
+                                                        View<float4> out) {
+    using T = BmTile<R, PV, TW, TH>;
     __shared__ float4 sY[T::LN];   // Y, Cb, Cr, (w)
     __shared__ float4 sF0[T::LN];  // features 0..3
     __shared__ float4 sF1[T::LN];  // features 4..7
-    const int x0 = blockIdx.x * T::TW, y0 = blockIdx.y * T::TH;
-    const int tid = threadIdx.y * T::TW + threadIdx.x;
-    for (int e = tid; e < T::LN; e += T::TW * T::TH) {
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH * PV;
+    const int tid = threadIdx.y * TW + threadIdx.x;
+    for (int e = tid; e < T::LN; e += T::THREADS) {
         const int ly = e / T::LW, lx = e % T::LW;
         const int gx = clampi(x0 + lx - R, 0, in.w - 1), gy = clampi(y0 + ly - R, 0, in.h - 1);
         sY[e] = in.at(gx, gy);
@@ -70,75 +73,116 @@ __global__ __launch_bounds__(256) void k_block_match(View<const float4> in, View
         sF1[e] = f[1];
     }
     __syncthreads();
-    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
-    if (x >= out.w || y >= out.h) return;
+    const int x = x0 + threadIdx.x, yb = y0 + threadIdx.y * PV;
+    if (x >= out.w || yb >= out.h) return;
 
-    const int c = (threadIdx.y + R) * T::LW + threadIdx.x + R;
-    const float4 ycc = sY[c];
-    const float4 fa = sF0[c], fb = sF1[c];
-
-    float lens = 1.0f;
-    if (p.lensShading > 0.0f) {
-        const float cx = (float)(in.w / 2), cy = (float)(in.h / 2);
-        const float dx = (float)x - cx, dy = (float)y - cy;
-        const float d = sqrtf(dx * dx + dy * dy) / sqrtf(cx * cx + cy * cy);
-        lens = 1.0f + p.lensShading * d * d;
-    }
-    const float s0 = sqrtf(nlf->a[0] + nlf->b[0] * lens * ycc.x);
-    const float s1 = sqrtf(nlf->a[1] + nlf->b[1] * lens * ycc.x);
-    const float s2 = sqrtf(nlf->a[2] + nlf->b[2] * lens * ycc.x);
-    const float dm0 = 1.0f / (p.thr[0] * s0), dm1 = 1.0f / (p.thr[1] * s1), dm2 = 1.0f / (p.thr[2] * s2);
-    float mag = 0.0f;
-    if (grad.ptr) {
-        const float2 g = grad.at(x, y);
-        mag = sqrtf(g.x * g.x + g.y * g.y);
-    }
-    const float edge = smoothstepf(2.0f, 16.0f, p.gradientThreshold * mag / s0);
-    const float lm = dm0 / (1.0f + p.gradientBoost * edge);
-    const float kl = -2.0f * kLog2e * lm * lm;  // wl = 2^(kl d2)
-    const float cb1 = dm1 / p.chromaBoost, cb2 = dm2 / p.chromaBoost;
-    const float kb = -2.0f * kLog2e * cb1 * cb1, kr = -2.0f * kLog2e * cb2 * cb2, kw = -2.0f * kLog2e;
-
-    float fY = 0.0f, fCb = 0.0f, fCr = 0.0f, nY = 0.0f, nC = 0.0f;
+    // Per-pixel coefficients (demosaic.metal:820-838), log2(e) folded in:
+    //   wl = 2^(kl d2),  wc = 2^(-(sb dCb)^2 - (sr dCr)^2 + kw wl^2 + log2 wd)
+    float4 fa[PV], fb[PV];
+    float kl[PV], sb[PV], sr[PV], sbc[PV], src[PV];
+    float fY[PV], fCb[PV], fCr[PV], nY[PV], nC[PV];
+    const float kw = -2.0f * kLog2e, kroot = sqrtf(2.0f * kLog2e);
 #pragma unroll
-    for (int dy = -R; dy <= R; dy++) {
+    for (int k = 0; k < PV; k++) {
+        const int y = min(yb + k, out.h - 1);
+        const int c = (threadIdx.y * PV + k + R) * T::LW + threadIdx.x + R;
+        const float4 ycc = sY[c];
+        fa[k] = sF0[c];
+        fb[k] = sF1[c];
+        float lens = 1.0f;
+        if (p.lensShading > 0.0f) {
+            const float cx = (float)(in.w / 2), cy = (float)(in.h / 2);
+            const float dx = (float)x - cx, dy = (float)y - cy;
+            const float d = sqrtf(dx * dx + dy * dy) / sqrtf(cx * cx + cy * cy);
+            lens = 1.0f + p.lensShading * d * d;
+        }
+        const float s0 = sqrtf(nlf->a[0] + nlf->b[0] * lens * ycc.x);
+        const float s1 = sqrtf(nlf->a[1] + nlf->b[1] * lens * ycc.x);
+        const float s2 = sqrtf(nlf->a[2] + nlf->b[2] * lens * ycc.x);
+        const float dm0 = 1.0f / (p.thr[0] * s0), dm1 = 1.0f / (p.thr[1] * s1), dm2 = 1.0f / (p.thr[2] * s2);
+        float mag = 0.0f;
+        if (grad.ptr) {
+            const float2 g = grad.at(x, y);
+            mag = sqrtf(g.x * g.x + g.y * g.y);
+        }
+        const float edge = smoothstepf(2.0f, 16.0f, p.gradientThreshold * mag / s0);
+        const float lm = dm0 / (1.0f + p.gradientBoost * edge);
+        kl[k] = -2.0f * kLog2e * lm * lm;
+        sb[k] = kroot * (dm1 / p.chromaBoost);
+        sr[k] = kroot * (dm2 / p.chromaBoost);
+        sbc[k] = sb[k] * ycc.y;
+        src[k] = sr[k] * ycc.z;
+        fY[k] = fCb[k] = fCr[k] = nY[k] = nC[k] = 0.0f;
+    }
+
+    // Neighbour rows dy = -R .. R+PV-1 relative to the first owned row: each
+    // staged neighbour is loaded once and used by every owned pixel whose
+    // window contains it.  Summation order per pixel: rows ascending, columns
+    // ascending, as the reference's loops (demosaic.metal:858-859).
+#pragma unroll 1
+    for (int dy = -R; dy <= R + PV - 1; dy++) {
+        const int rowq = (threadIdx.y * PV + R + dy) * T::LW + threadIdx.x + R;
 #pragma unroll
         for (int dx = -R; dx <= R; dx++) {
-            const int q = c + dy * T::LW + dx;
+            const int q = rowq + dx;
             const float4 qa = sF0[q], qb = sF1[q], qy = sY[q];
-            float t, d2;
-            t = qa.x - fa.x; d2 = t * t;
-            t = qa.y - fa.y; d2 = fmaf(t, t, d2);
-            t = qa.z - fa.z; d2 = fmaf(t, t, d2);
-            t = qa.w - fa.w; d2 = fmaf(t, t, d2);
-            t = qb.x - fb.x; d2 = fmaf(t, t, d2);
-            t = qb.y - fb.y; d2 = fmaf(t, t, d2);
-            t = qb.z - fb.z; d2 = fmaf(t, t, d2);
-            t = qb.w - fb.w; d2 = fmaf(t, t, d2);
-            const float wl = __builtin_amdgcn_exp2f(kl * d2);
-            const float db = qy.y - ycc.y, dr = qy.z - ycc.z;
-            const float e = fmaf(db * kb, db, fmaf(dr * kr, dr, kw * wl * wl));
-            const float wc = __builtin_amdgcn_exp2f(e);
-            const float wd = dir.w[dx * dx + dy * dy];
-            const float wY = wd * wl, wC = wd * wc;
-            fY = fmaf(wY, qy.x, fY);
-            fCb = fmaf(wC, qy.y, fCb);
-            fCr = fmaf(wC, qy.z, fCr);
-            nY += wY;
-            nC += wC;
+#pragma unroll
+            for (int k = 0; k < PV; k++) {
+                const int ddy = dy - k;
+                if (ddy < -R || ddy > R) continue;  // wave-uniform
+                const int o2 = dx * dx + ddy * ddy;
+                const float wd = dir.w[o2], lwd = dir.lw[o2];
+                float t, d2;
+                t = qa.x - fa[k].x; d2 = t * t;
+                t = qa.y - fa[k].y; d2 = fmaf(t, t, d2);
+                t = qa.z - fa[k].z; d2 = fmaf(t, t, d2);
+                t = qa.w - fa[k].w; d2 = fmaf(t, t, d2);
+                t = qb.x - fb[k].x; d2 = fmaf(t, t, d2);
+                t = qb.y - fb[k].y; d2 = fmaf(t, t, d2);
+                t = qb.z - fb[k].z; d2 = fmaf(t, t, d2);
+                t = qb.w - fb[k].w; d2 = fmaf(t, t, d2);
+                const float wl = __builtin_amdgcn_exp2f(kl[k] * d2);
+                const float wY = wd * wl;
+                const float tb = fmaf(sb[k], qy.y, -sbc[k]), tr = fmaf(sr[k], qy.z, -src[k]);
+                float e = fmaf(-tb, tb, lwd);
+                e = fmaf(-tr, tr, e);
+                e = fmaf(kw * wl, wl, e);
+                const float wC = __builtin_amdgcn_exp2f(e);
+                fY[k] = fmaf(wY, qy.x, fY[k]);
+                nY[k] += wY;
+                fCb[k] = fmaf(wC, qy.y, fCb[k]);
+                fCr[k] = fmaf(wC, qy.z, fCr[k]);
+                nC[k] += wC;
+            }
         }
     }
-    out.at(x, y) = make_float4(fY / nY, fCb / nC, fCr / nC, nY);
+#pragma unroll
+    for (int k = 0; k < PV; k++) {
+        if (yb + k < out.h) out.at(x, yb + k) = make_float4(fY[k] / nY[k], fCb[k] / nC[k], fCr[k] / nC[k], nY[k]);
+    }
 }
 
-template <int R>
-gcam_status launch_r(View<const float4> in, View<const float2> grad, View<const float> feat8, const NlfDev* nlf,
-                     const BmParams& p, const DirTable& dir, View<float4> out, hipStream_t s) {
-    using T = BmTile<R>;
-    dim3 grid((out.w + T::TW - 1) / T::TW, (out.h + T::TH - 1) / T::TH);
-    k_block_match<R><<<grid, dim3(T::TW, T::TH), 0, s>>>(in, grad, feat8.ptr, feat8.stride, nlf, p, dir, out);
+template <int R, int PV, int TW, int TH>
+gcam_status launch_shape(View<const float4> in, View<const float2> grad, View<const float> feat8, const NlfDev* nlf,
+                         const BmParams& p, const DirTable& dir, View<float4> out, hipStream_t s) {
+    dim3 grid((out.w + TW - 1) / TW, (out.h + TH * PV - 1) / (TH * PV));
+    k_block_match<R, PV, TW, TH><<<grid, dim3(TW, TH), 0, s>>>(in, grad, feat8.ptr, feat8.stride, nlf, p, dir, out);
     GCAM_LAUNCH_CHECK("blockMatchingDenoiseImage");
     return GCAM_OK;
+}
+
+// Tile shapes: 1 row per thread uses a 32 x 8 (R <= 6) or 32 x 4 (R > 6)
+// workgroup; 2 and 4 rows per thread (R <= 6) use 32 x 8 and 32 x 4.
+template <int R>
+gcam_status launch_r(View<const float4> in, View<const float2> grad, View<const float> feat8, const NlfDev* nlf,
+                     const BmParams& p, const DirTable& dir, int rows, View<float4> out, hipStream_t s) {
+    if constexpr (R <= 6) {
+        if (rows == 4) return launch_shape<R, 4, 32, 4>(in, grad, feat8, nlf, p, dir, out, s);
+        if (rows == 2) return launch_shape<R, 2, 32, 8>(in, grad, feat8, nlf, p, dir, out, s);
+        return launch_shape<R, 1, 32, 8>(in, grad, feat8, nlf, p, dir, out, s);
+    } else {
+        return launch_shape<R, 1, 32, 4>(in, grad, feat8, nlf, p, dir, out, s);
+    }
 }
 
 DirTable make_dir_table() {
@@ -147,6 +191,7 @@ DirTable make_dir_table() {
     for (int d2 = 0; d2 < (int)(sizeof(t.w) / sizeof(float)); d2++) {
         const float a = 0.1f * std::sqrt((float)d2);
         t.w[d2] = std::exp(-2.0f * a * a);
+        t.lw[d2] = std::log2(t.w[d2]);
     }
     return t;
 }
@@ -154,20 +199,21 @@ DirTable make_dir_table() {
 }  // namespace
 
 gcam_status launch_block_match(View<const float4> in, View<const float2> grad, View<const float> feat8,
-                               const NlfDev* nlf_level, const BmParams& p, int radius, View<float4> out,
-                               hipStream_t s) {
+                               const NlfDev* nlf_level, const BmParams& p, int radius, int rows_per_thread,
+                               View<float4> out, hipStream_t s) {
     static const DirTable dir = make_dir_table();
+    const int rows = rows_per_thread == 0 ? (radius <= 6 ? 2 : 1) : rows_per_thread;
     switch (radius) {
-        case 1: return launch_r<1>(in, grad, feat8, nlf_level, p, dir, out, s);
-        case 2: return launch_r<2>(in, grad, feat8, nlf_level, p, dir, out, s);
-        case 3: return launch_r<3>(in, grad, feat8, nlf_level, p, dir, out, s);
-        case 4: return launch_r<4>(in, grad, feat8, nlf_level, p, dir, out, s);
-        case 5: return launch_r<5>(in, grad, feat8, nlf_level, p, dir, out, s);
-        case 6: return launch_r<6>(in, grad, feat8, nlf_level, p, dir, out, s);
-        case 7: return launch_r<7>(in, grad, feat8, nlf_level, p, dir, out, s);
-        case 8: return launch_r<8>(in, grad, feat8, nlf_level, p, dir, out, s);
-        case 9: return launch_r<9>(in, grad, feat8, nlf_level, p, dir, out, s);
-        case 10: return launch_r<10>(in, grad, feat8, nlf_level, p, dir, out, s);
+        case 1: return launch_r<1>(in, grad, feat8, nlf_level, p, dir, rows, out, s);
+        case 2: return launch_r<2>(in, grad, feat8, nlf_level, p, dir, rows, out, s);
+        case 3: return launch_r<3>(in, grad, feat8, nlf_level, p, dir, rows, out, s);
+        case 4: return launch_r<4>(in, grad, feat8, nlf_level, p, dir, rows, out, s);
+        case 5: return launch_r<5>(in, grad, feat8, nlf_level, p, dir, rows, out, s);
+        case 6: return launch_r<6>(in, grad, feat8, nlf_level, p, dir, rows, out, s);
+        case 7: return launch_r<7>(in, grad, feat8, nlf_level, p, dir, rows, out, s);
+        case 8: return launch_r<8>(in, grad, feat8, nlf_level, p, dir, rows, out, s);
+        case 9: return launch_r<9>(in, grad, feat8, nlf_level, p, dir, rows, out, s);
+        case 10: return launch_r<10>(in, grad, feat8, nlf_level, p, dir, rows, out, s);
         default: return set_error(GCAM_ERR_INVALID_ARGUMENT, "search radius must be 1..10");
     }
 }

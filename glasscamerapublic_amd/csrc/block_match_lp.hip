@@ -1,0 +1,308 @@
+// Block-matching denoise, precision form (FP16 / BF16 contractions, FP32
+// accumulation) for gfx950.
+//
+//   pcaProjection              MetalPipeline/demosaic.metal:774-796
+//   blockMatchingDenoiseImage  MetalPipeline/demosaic.metal:806-884
+//
+// One kernel per level computes the PCA features of its tile in-workgroup
+// (they never touch HBM) and then the weighted window average:
+//
+// 1. Luma of the tile plus an (R+2)-pixel apron is staged in LDS, centred on
+//    the tile's first pixel and scaled by the level scale s (~1/sigma of the
+//    level), then rounded to FP16/BF16.  Feature distances are invariant to the
+//    centring; scaling divides the luma coefficient by s^2.  Centring keeps
+//    the rounded values small so their absolute rounding error is small.
+// 2. Projection F = P B on MFMA: v_mfma_f32_16x16x32_{f16,bf16}, 16 pixels x
+//    (25 patch values padded to K = 32) x (8 components padded to N = 16),
+//    FP32 accumulation.  The features are rounded to FP16/BF16 in LDS, and
+//    their squared norms |f|^2 are summed in FP32 from the rounded values.
+// 3. Distances as a Gram form: d^2 = |f_p|^2 + |f_q|^2 - 2 f_p.f_q.  The dot
+//    product runs as 4 packed dot2 (v_dot2c_f32_{f16,bf16}: FP16/BF16
+//    products, FP32 accumulation).  On the rounded features this is exactly
+//    the squared distance up to FP32 rounding.  The luma/chroma weights and
+//    the accumulation are FP32, as in the FP32 form.
+#include "gcam_internal.hpp"
+
+#include <cmath>
+
+namespace gcam {
+
+namespace {
+
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <class H>
+struct Prec;
+
+template <>
+struct Prec<_Float16> {
+    using h2 = f16x2;
+    using h8 = f16x8;
+    static __device__ __forceinline__ f32x4 mfma(h8 a, h8 b) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ float dot2(h2 a, h2 b, float c) { return __builtin_amdgcn_fdot2(a, b, c, false); }
+};
+
+template <>
+struct Prec<__bf16> {
+    using h2 = bf16x2;
+    using h8 = bf16x8;
+    static __device__ __forceinline__ f32x4 mfma(h8 a, h8 b) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ float dot2(h2 a, h2 b, float c) {
+        return __builtin_amdgcn_fdot2_f32_bf16(a, b, c, false);
+    }
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
+    float t = (x - e0) / (e1 - e0);
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+struct DirTable {
+    float w[2 * GCAM_MAX_SEARCH_RADIUS * GCAM_MAX_SEARCH_RADIUS + 1];
+    float lw[2 * GCAM_MAX_SEARCH_RADIUS * GCAM_MAX_SEARCH_RADIUS + 1];
+};
+
+constexpr float kLog2e = 1.4426950408889634f;
+
+// Tile TW x (TH*PV) outputs; feature region with an R apron; luma region
+// with a further 2-pixel apron (the 5x5 patch).
+template <int R, int PV, int TW, int TH>
+struct LpTile {
+    static constexpr int FW = TW + 2 * R, FH = TH * PV + 2 * R, FN = FW * FH;
+    static constexpr int LW = FW + 4, LH = FH + 4, LN = LW * LH;
+    static constexpr int THREADS = TW * TH, WAVES = THREADS / 64, GROUPS = (FN + 15) / 16;
+    static_assert(THREADS % 64 == 0, "whole wavefronts");
+    static_assert(LN * 2 + FN * 32 <= 64 * 1024, "LDS tile too large");
+};
+
+template <int R, int PV, int TW, int TH, class H>
+__global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in, View<const float2> grad,
+                                                           const float* __restrict__ basis,
+                                                           const NlfDev* __restrict__ nlf, BmParams p,
+                                                           DirTable dir, View<float4> out) {
+    using T = LpTile<R, PV, TW, TH>;
+    using P = Prec<H>;
+    using h2 = typename P::h2;
+    using h8 = typename P::h8;
+    __shared__ H lum[T::LN];
+    __shared__ float4 ycn[T::FN];  // Y, Cb, Cr, |f|^2 (scaled, rounded features)
+    __shared__ h8 fh[T::FN];       // 8 rounded features per pixel
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH * PV;
+    const int tid = threadIdx.y * TW + threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+
+    // Level scale ~ 1/sigma of mid-grey luma; centre = the tile's first pixel.
+    const float sref = sqrtf(nlf->a[0] + nlf->b[0] * 0.5f);
+    const float scale = sref > 0.0f ? 1.0f / sref : 1.0f;
+    const float mu = in.at(clampi(x0, 0, in.w - 1), clampi(y0, 0, in.h - 1)).x;
+
+    for (int e = tid; e < T::LN; e += T::THREADS) {
+        const int ly = e / T::LW, lx = e % T::LW;
+        const float yv = in.at(clampi(x0 - R - 2 + lx, 0, in.w - 1), clampi(y0 - R - 2 + ly, 0, in.h - 1)).x;
+        lum[e] = (H)((yv - mu) * scale);
+    }
+    for (int e = tid; e < T::FN; e += T::THREADS) {
+        const int fy = e / T::FW, fx = e % T::FW;
+        const float4 v = in.at(clampi(x0 - R + fx, 0, in.w - 1), clampi(y0 - R + fy, 0, in.h - 1));
+        ycn[e] = make_float4(v.x, v.y, v.z, 0.0f);
+    }
+    // B operand of the projection (constant): lane holds basis[k][j] for
+    // j = lane % 16, k = 8 (lane / 16) + v; zero outside 25 x 8.
+    h8 bop;
+    {
+        const int j = lane & 15, kg = lane >> 4;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            const int k = 8 * kg + v;
+            bop[v] = (H)((k < kPatch && j < kComponents) ? basis[k * kComponents + j] : 0.0f);
+        }
+    }
+    __syncthreads();
+
+    // Projection, 16 feature-region pixels per MFMA.  A operand: lane holds
+    // patch values k = 8 (lane / 16) + v of pixel lane % 16 of the group.
+    for (int g = wave; g < T::GROUPS; g += T::WAVES) {
+        const int i = lane & 15, kg = lane >> 4;
+        const int e = min(g * 16 + i, T::FN - 1);
+        const int fy = e / T::FW, fx = e % T::FW;
+        h8 aop;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            const int k = 8 * kg + v;
+            aop[v] = k < kPatch ? lum[(fy + k / 5) * T::LW + fx + k % 5] : (H)0.0f;
+        }
+        const f32x4 c = P::mfma(aop, bop);
+        // C: lane holds pixel 4 (lane / 16) + r of the group, component lane % 16.
+        const int j = lane & 15;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const H hq = (H)c[r];
+            const float fq = (float)hq;
+            float n = j < kComponents ? fq * fq : 0.0f;
+            n += __shfl_xor(n, 1);
+            n += __shfl_xor(n, 2);
+            n += __shfl_xor(n, 4);
+            const int er = g * 16 + 4 * kg + r;
+            if (er < T::FN) {
+                if (j < kComponents) reinterpret_cast<H*>(&fh[er])[j] = hq;
+                if (j == 0) ycn[er].w = n;
+            }
+        }
+    }
+    __syncthreads();
+
+    const int x = x0 + threadIdx.x, yb = y0 + threadIdx.y * PV;
+    if (x >= out.w || yb >= out.h) return;
+
+    h8 fp[PV];
+    float kl[PV], k2[PV], cp[PV], sb[PV], sr[PV], sbc[PV], src[PV];
+    float fY[PV], fCb[PV], fCr[PV], nY[PV], nC[PV];
+    const float kw = -2.0f * kLog2e, kroot = sqrtf(2.0f * kLog2e);
+#pragma unroll
+    for (int k = 0; k < PV; k++) {
+        const int y = min(yb + k, out.h - 1);
+        const int c = (threadIdx.y * PV + k + R) * T::FW + threadIdx.x + R;
+        const float4 v = ycn[c];
+        fp[k] = fh[c];
+        float lens = 1.0f;
+        if (p.lensShading > 0.0f) {
+            const float cx = (float)(in.w / 2), cy = (float)(in.h / 2);
+            const float dx = (float)x - cx, dy = (float)y - cy;
+            const float d = sqrtf(dx * dx + dy * dy) / sqrtf(cx * cx + cy * cy);
+            lens = 1.0f + p.lensShading * d * d;
+        }
+        const float s0 = sqrtf(nlf->a[0] + nlf->b[0] * lens * v.x);
+        const float s1 = sqrtf(nlf->a[1] + nlf->b[1] * lens * v.x);
+        const float s2 = sqrtf(nlf->a[2] + nlf->b[2] * lens * v.x);
+        const float dm0 = 1.0f / (p.thr[0] * s0), dm1 = 1.0f / (p.thr[1] * s1), dm2 = 1.0f / (p.thr[2] * s2);
+        float mag = 0.0f;
+        if (grad.ptr) {
+            const float2 gg = grad.at(x, y);
+            mag = sqrtf(gg.x * gg.x + gg.y * gg.y);
+        }
+        const float edge = smoothstepf(2.0f, 16.0f, p.gradientThreshold * mag / s0);
+        const float lm = dm0 / ((1.0f + p.gradientBoost * edge) * scale);
+        kl[k] = -2.0f * kLog2e * lm * lm;  // per unit of scaled squared distance
+        k2[k] = -2.0f * kl[k];
+        cp[k] = kl[k] * v.w;
+        sb[k] = kroot * (dm1 / p.chromaBoost);
+        sr[k] = kroot * (dm2 / p.chromaBoost);
+        sbc[k] = sb[k] * v.y;
+        src[k] = sr[k] * v.z;
+        fY[k] = fCb[k] = fCr[k] = nY[k] = nC[k] = 0.0f;
+    }
+
+#pragma unroll 1
+    for (int dy = -R; dy <= R + PV - 1; dy++) {
+        const int rowq = (threadIdx.y * PV + R + dy) * T::FW + threadIdx.x + R;
+#pragma unroll
+        for (int dx = -R; dx <= R; dx++) {
+            const int q = rowq + dx;
+            const h8 fq = fh[q];
+            const float4 qy = ycn[q];
+#pragma unroll
+            for (int k = 0; k < PV; k++) {
+                const int ddy = dy - k;
+                if (ddy < -R || ddy > R) continue;  // wave-uniform
+                const int o2 = dx * dx + ddy * ddy;
+                const float wd = dir.w[o2], lwd = dir.lw[o2];
+                float G = P::dot2(h2{fq[0], fq[1]}, h2{fp[k][0], fp[k][1]}, 0.0f);
+                G = P::dot2(h2{fq[2], fq[3]}, h2{fp[k][2], fp[k][3]}, G);
+                G = P::dot2(h2{fq[4], fq[5]}, h2{fp[k][4], fp[k][5]}, G);
+                G = P::dot2(h2{fq[6], fq[7]}, h2{fp[k][6], fp[k][7]}, G);
+                const float wl = __builtin_amdgcn_exp2f(fmaf(k2[k], G, fmaf(kl[k], qy.w, cp[k])));
+                const float wY = wd * wl;
+                const float tb = fmaf(sb[k], qy.y, -sbc[k]), tr = fmaf(sr[k], qy.z, -src[k]);
+                float e = fmaf(-tb, tb, lwd);
+                e = fmaf(-tr, tr, e);
+                e = fmaf(kw * wl, wl, e);
+                const float wC = __builtin_amdgcn_exp2f(e);
+                fY[k] = fmaf(wY, qy.x, fY[k]);
+                nY[k] += wY;
+                fCb[k] = fmaf(wC, qy.y, fCb[k]);
+                fCr[k] = fmaf(wC, qy.z, fCr[k]);
+                nC[k] += wC;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PV; k++) {
+        if (yb + k < out.h) out.at(x, yb + k) = make_float4(fY[k] / nY[k], fCb[k] / nC[k], fCr[k] / nC[k], nY[k]);
+    }
+}
+
+template <int R, int PV, int TW, int TH, class H>
+gcam_status launch_lp_shape(View<const float4> in, View<const float2> grad, const float* basis, const NlfDev* nlf,
+                            const BmParams& p, const DirTable& dir, View<float4> out, hipStream_t s) {
+    dim3 grid((out.w + TW - 1) / TW, (out.h + TH * PV - 1) / (TH * PV));
+    k_block_match_lp<R, PV, TW, TH, H><<<grid, dim3(TW, TH), 0, s>>>(in, grad, basis, nlf, p, dir, out);
+    GCAM_LAUNCH_CHECK("blockMatchingDenoiseImage (low precision)");
+    return GCAM_OK;
+}
+
+template <int R, class H>
+gcam_status launch_lp_r(View<const float4> in, View<const float2> grad, const float* basis, const NlfDev* nlf,
+                        const BmParams& p, const DirTable& dir, int rows, View<float4> out, hipStream_t s) {
+    if constexpr (R <= 6) {
+        if (rows == 4) return launch_lp_shape<R, 4, 32, 4, H>(in, grad, basis, nlf, p, dir, out, s);
+        if (rows == 2) return launch_lp_shape<R, 2, 32, 8, H>(in, grad, basis, nlf, p, dir, out, s);
+        return launch_lp_shape<R, 1, 32, 8, H>(in, grad, basis, nlf, p, dir, out, s);
+    } else {
+        return launch_lp_shape<R, 1, 32, 4, H>(in, grad, basis, nlf, p, dir, out, s);
+    }
+}
+
+DirTable make_dir_table() {
+    DirTable t{};
+    for (int d2 = 0; d2 < (int)(sizeof(t.w) / sizeof(float)); d2++) {
+        const float a = 0.1f * std::sqrt((float)d2);
+        t.w[d2] = std::exp(-2.0f * a * a);
+        t.lw[d2] = std::log2(t.w[d2]);
+    }
+    return t;
+}
+
+template <class H>
+gcam_status dispatch(View<const float4> in, View<const float2> grad, const float* basis, const NlfDev* nlf,
+                     const BmParams& p, int radius, int rows, View<float4> out, hipStream_t s) {
+    static const DirTable dir = make_dir_table();
+    switch (radius) {
+        case 1: return launch_lp_r<1, H>(in, grad, basis, nlf, p, dir, rows, out, s);
+        case 2: return launch_lp_r<2, H>(in, grad, basis, nlf, p, dir, rows, out, s);
+        case 3: return launch_lp_r<3, H>(in, grad, basis, nlf, p, dir, rows, out, s);
+        case 4: return launch_lp_r<4, H>(in, grad, basis, nlf, p, dir, rows, out, s);
+        case 5: return launch_lp_r<5, H>(in, grad, basis, nlf, p, dir, rows, out, s);
+        case 6: return launch_lp_r<6, H>(in, grad, basis, nlf, p, dir, rows, out, s);
+        case 7: return launch_lp_r<7, H>(in, grad, basis, nlf, p, dir, rows, out, s);
+        case 8: return launch_lp_r<8, H>(in, grad, basis, nlf, p, dir, rows, out, s);
+        case 9: return launch_lp_r<9, H>(in, grad, basis, nlf, p, dir, rows, out, s);
+        case 10: return launch_lp_r<10, H>(in, grad, basis, nlf, p, dir, rows, out, s);
+        default: return set_error(GCAM_ERR_INVALID_ARGUMENT, "search radius must be 1..10");
+    }
+}
+
+}  // namespace
+
+gcam_status launch_block_match_lp(View<const float4> in, View<const float2> grad, const float* basis,
+                                  const NlfDev* nlf_level, const BmParams& p, int radius, int rows_per_thread,
+                                  int precision, View<float4> out, hipStream_t s) {
+    const int rows = rows_per_thread == 0 ? (radius <= 6 ? 2 : 1) : rows_per_thread;
+    if (precision == GCAM_PRECISION_FP16) return dispatch<_Float16>(in, grad, basis, nlf_level, p, radius, rows, out, s);
+    if (precision == GCAM_PRECISION_BF16) return dispatch<__bf16>(in, grad, basis, nlf_level, p, radius, rows, out, s);
+    return set_error(GCAM_ERR_INVALID_ARGUMENT, "low-precision block matching needs FP16 or BF16");
+}
+
+}  // namespace gcam

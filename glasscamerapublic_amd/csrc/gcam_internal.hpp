@@ -97,8 +97,13 @@ gcam_status launch_pca_solve(const double* partials, int64_t n, float* basis, do
 gcam_status launch_pca_project(View<const float4> in, const float* basis, View<float> feat8, hipStream_t s);
 
 gcam_status launch_block_match(View<const float4> in, View<const float2> grad, View<const float> feat8,
-                               const NlfDev* nlf_level, const BmParams& p, int radius, View<float4> out,
-                               hipStream_t s);
+                               const NlfDev* nlf_level, const BmParams& p, int radius, int rows_per_thread,
+                               View<float4> out, hipStream_t s);
+
+// Precision form: projection fused in (features never materialized), basis = 25 x 8 f32 on the device.
+gcam_status launch_block_match_lp(View<const float4> in, View<const float2> grad, const float* basis,
+                                  const NlfDev* nlf_level, const BmParams& p, int radius, int rows_per_thread,
+                                  int precision, View<float4> out, hipStream_t s);
 
 gcam_status launch_synth(int w, int h, const gcam_synth_params& p, uint16_t* raw, View<float4> ycc,
                          View<float4> clean, hipStream_t s);

@@ -88,12 +88,12 @@ class PyramidProcessor:
     """
 
     def __init__(self, width: int, height: int, levels: int = 5, search_radius: int = 10,
-                 precision: str = "fp32", device: Optional[torch.device] = None):
+                 precision: str = "fp32", device: Optional[torch.device] = None, bm_rows_per_thread: int = 0):
         self.width, self.height, self.levels = int(width), int(height), int(levels)
         self.search_radius = int(search_radius)
         self.precision = precision
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
-        cfg = ConfigC(self.search_radius, _lib.GCAM_PRECISION[precision])
+        cfg = ConfigC(self.search_radius, _lib.GCAM_PRECISION[precision], int(bm_rows_per_thread))
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             check(lib().gcam_pyramid_create(self.width, self.height, self.levels, ctypes.byref(cfg),
@@ -264,6 +264,24 @@ def block_matching_denoise_image(image, features, var_a, var_b, thresholdMultipl
         ctypes.byref(as_image(image)), ctypes.byref(g) if g is not None else None,
         ctypes.byref(as_image(features, GCAM_PIXEL_PCA8_F32)), va, vb, th, float(chromaBoost), float(gradientBoost),
         float(gradientThreshold), float(lensShadingCorrection), int(search_radius), ctypes.byref(as_image(out)),
+        ctypes.c_void_p(_stream(stream))))
+    return out
+
+
+def block_matching_denoise_image_lp(image, basis, var_a, var_b, thresholdMultipliers, chromaBoost=1.0,
+                                    gradientBoost=0.0, gradientThreshold=1.0, lensShadingCorrection=0.0,
+                                    search_radius=10, gradient=None, precision="fp16", stream=None) -> torch.Tensor:
+    """Precision form of pcaProjection + blockMatchingDenoiseImage (features projected in-kernel)."""
+    out = torch.empty_like(image)
+    basis = basis.to(device=image.device, dtype=torch.float32).contiguous()
+    va = (ctypes.c_float * 3)(*[float(v) for v in var_a])
+    vb = (ctypes.c_float * 3)(*[float(v) for v in var_b])
+    th = (ctypes.c_float * 3)(*[float(v) for v in thresholdMultipliers])
+    g = as_image(gradient)
+    check(lib().gcam_block_matching_denoise_image_lp(
+        ctypes.byref(as_image(image)), ctypes.byref(g) if g is not None else None, ctypes.c_void_p(basis.data_ptr()),
+        va, vb, th, float(chromaBoost), float(gradientBoost), float(gradientThreshold), float(lensShadingCorrection),
+        int(search_radius), _lib.GCAM_PRECISION[precision], ctypes.byref(as_image(out)),
         ctypes.c_void_p(_stream(stream))))
     return out
 

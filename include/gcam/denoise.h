@@ -83,7 +83,9 @@ typedef enum gcam_precision {
 typedef struct gcam_config {
     int32_t search_radius; /* 1..GCAM_MAX_SEARCH_RADIUS; window (2r+1)^2      */
     int32_t precision;     /* gcam_precision                                   */
-    int32_t reserved[6];   /* must be zero                                     */
+    int32_t bm_rows_per_thread; /* block-matching tile shape: output rows per
+                                   thread, 1, 2 or 4; 0 = tuned default         */
+    int32_t reserved[5];   /* must be zero                                     */
 } gcam_config;
 
 typedef struct gcam_pyramid gcam_pyramid;
@@ -190,6 +192,19 @@ gcam_status gcam_block_matching_denoise_image(const gcam_image* input, const gca
                                               float chroma_boost, float gradient_boost, float gradient_threshold,
                                               float lens_shading_correction, int32_t search_radius,
                                               gcam_image* output, void* stream);
+
+/* Precision form of pcaProjection + blockMatchingDenoiseImage (demosaic.metal:774-796,
+ * 806-884): the features of each tile are projected in-kernel on MFMA in
+ * FP16 or BF16 (precision = GCAM_PRECISION_FP16 / _BF16) with FP32
+ * accumulation and never written to memory; patch distances use FP16/BF16
+ * products with FP32 accumulation; weights and sums are FP32.
+ * basis: device 25 x 8 floats. */
+gcam_status gcam_block_matching_denoise_image_lp(const gcam_image* input, const gcam_image* gradient,
+                                                 const float* basis, const float var_a[3], const float var_b[3],
+                                                 const float threshold_multipliers[3], float chroma_boost,
+                                                 float gradient_boost, float gradient_threshold,
+                                                 float lens_shading_correction, int32_t search_radius,
+                                                 int32_t precision, gcam_image* output, void* stream);
 
 /* basicNoiseStatisticsKernel (demosaic_kernels.hpp:438-450), demosaic.metal:1138-1159. */
 gcam_status gcam_basic_noise_statistics(const gcam_image* input, gcam_image* statistics, void* stream);

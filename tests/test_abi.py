@@ -82,3 +82,26 @@ def test_product_does_not_import_the_oracle():
             if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
                 text = open(os.path.join(root, f)).read()
                 assert "import oracle" not in text and "gcam_oracle" not in text, f
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    """gcam::PyramidProcessor<5> (the reference-shaped C++ surface) compiles and
+    links against the shared library with a plain host compiler."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = tmp_path / "facade.cpp"
+    src.write_text(
+        '#include "gcam/pyramid_processor.hpp"\n'
+        "int main() {\n"
+        "  std::array<gcam::DenoiseParameters, 5> p; p.fill(gcam::defaultDenoiseParameters);\n"
+        "  try { gcam::PyramidProcessor<5> proc(4032, 3024, 4); (void)proc; }\n"
+        "  catch (const gcam::Error& e) { return e.status == GCAM_ERR_NO_DEVICE ? 0 : 1; }\n"
+        "  return 0;\n"
+        "}\n")
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    exe = tmp_path / "facade"
+    subprocess.run(["g++", "-std=c++17", f"-I{repo}/include", "-D__HIP_PLATFORM_AMD__", str(src), "-o", str(exe),
+                    f"-L{libdir}", "-lgcam_denoise", f"-Wl,-rpath,{libdir}"], check=True)
