@@ -6,11 +6,13 @@ this package is the Python mirror of the reference's operator surface.
 """
 from ._lib import GcamError, lib  # noqa: F401
 from .pyramid import (DenoiseParameters, PyramidProcessor, basic_noise_statistics,  # noqa: F401
-                      block_matching_denoise_image, block_matching_denoise_image_lp, build_pca_space, collect_patches, downsample_image,
-                      measure_ycbcr_nlf, pca_projection, subtract_noise_image, synth_frame)
+                      block_matching_denoise_image, block_matching_denoise_image_lp, build_pca_space,
+                      collect_patches, denoise_image, downsample_image, measure_ycbcr_nlf, pca_projection,
+                      subtract_noise_image, synth_frame)
 
 __all__ = [
     "GcamError", "lib", "DenoiseParameters", "PyramidProcessor", "basic_noise_statistics",
-    "block_matching_denoise_image", "block_matching_denoise_image_lp", "build_pca_space", "collect_patches", "downsample_image", "measure_ycbcr_nlf",
-    "pca_projection", "subtract_noise_image", "synth_frame",
+    "block_matching_denoise_image", "block_matching_denoise_image_lp", "build_pca_space", "collect_patches",
+    "denoise_image", "downsample_image", "measure_ycbcr_nlf", "pca_projection", "subtract_noise_image",
+    "synth_frame",
 ]

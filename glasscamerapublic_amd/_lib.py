@@ -29,7 +29,7 @@ EXPORTED = (
     "gcam_pyramid_create", "gcam_pyramid_destroy", "gcam_pyramid_denoise", "gcam_pyramid_level",
     "gcam_pyramid_basis", "gcam_pyramid_set_profiling", "gcam_pyramid_stage_times", "gcam_downsample_image", "gcam_subtract_noise_image", "gcam_collect_patches",
     "gcam_build_pca_space", "gcam_pca_projection", "gcam_block_matching_denoise_image",
-    "gcam_block_matching_denoise_image_lp",
+    "gcam_block_matching_denoise_image_lp", "gcam_denoise_image",
     "gcam_basic_noise_statistics", "gcam_measure_ycbcr_nlf", "gcam_synth_frame",
 )
 
@@ -58,7 +58,8 @@ class NlfC(ctypes.Structure):
 
 class ConfigC(ctypes.Structure):
     _fields_ = [("search_radius", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("bm_rows_per_thread", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("bm_rows_per_thread", ctypes.c_int32), ("bilateral", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class SynthParamsC(ctypes.Structure):
@@ -96,6 +97,7 @@ def _declare(L: ctypes.CDLL) -> None:
     L.gcam_pca_projection.argtypes = [_IP, _P, _IP, _P]
     L.gcam_block_matching_denoise_image.argtypes = [_IP, _IP, _IP, _P, _P, _P, _F, _F, _F, _F, _I, _IP, _P]
     L.gcam_block_matching_denoise_image_lp.argtypes = [_IP, _IP, _P, _P, _P, _P, _F, _F, _F, _F, _I, _I, _IP, _P]
+    L.gcam_denoise_image.argtypes = [_IP, _IP, _P, _P, _P, _F, _F, _F, _IP, _P]
     L.gcam_basic_noise_statistics.argtypes = [_IP, _IP, _P]
     L.gcam_measure_ycbcr_nlf.argtypes = [_IP, _IP, _F, ctypes.POINTER(NlfC), _P]
     L.gcam_synth_frame.argtypes = [_I, _I, ctypes.POINTER(SynthParamsC), _P, _IP, _IP, _P]

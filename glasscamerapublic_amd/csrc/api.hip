@@ -25,6 +25,7 @@ using namespace gcam;
 
 struct gcam_pyramid {
     int width = 0, height = 0, levels = 0, radius = 10, precision = GCAM_PRECISION_FP32, bm_rows = 0, device = 0;
+    bool bilateral = false;  // usePatchSimiliarity == false
     int W[GCAM_MAX_LEVELS] = {}, H[GCAM_MAX_LEVELS] = {};
     char* arena = nullptr;        // one allocation for every fixed-size buffer
     float2* grad_arena = nullptr; // gradient pyramid, allocated on first use
@@ -111,8 +112,9 @@ gcam_status gcam_pyramid_create(int32_t width, int32_t height, int32_t levels, c
         return set_error(GCAM_ERR_INVALID_ARGUMENT, "levels must be 1.." + std::to_string(GCAM_MAX_LEVELS));
     if (width < 1 || height < 1 || width > kMaxDim || height > kMaxDim)
         return set_error(GCAM_ERR_INVALID_ARGUMENT, "image size out of range 1.." + std::to_string(kMaxDim));
-    gcam_config cfg{10, GCAM_PRECISION_FP32, 0, {0, 0, 0, 0, 0}};
+    gcam_config cfg{10, GCAM_PRECISION_FP32, 0, 0, {0, 0, 0, 0}};
     if (config) cfg = *config;
+    if (cfg.bilateral != 0 && cfg.bilateral != 1) return set_error(GCAM_ERR_INVALID_ARGUMENT, "bilateral must be 0 or 1");
     if (cfg.search_radius < 1 || cfg.search_radius > GCAM_MAX_SEARCH_RADIUS)
         return set_error(GCAM_ERR_INVALID_ARGUMENT, "search_radius must be 1..10");
     if (cfg.precision != GCAM_PRECISION_FP32 && cfg.precision != GCAM_PRECISION_FP16 &&
@@ -121,7 +123,7 @@ gcam_status gcam_pyramid_create(int32_t width, int32_t height, int32_t levels, c
     if (cfg.bm_rows_per_thread != 0 && cfg.bm_rows_per_thread != 1 && cfg.bm_rows_per_thread != 2 &&
         cfg.bm_rows_per_thread != 4)
         return set_error(GCAM_ERR_INVALID_ARGUMENT, "bm_rows_per_thread must be 0, 1, 2 or 4");
-    for (int i = 0; i < 5; i++)
+    for (int i = 0; i < 4; i++)
         if (cfg.reserved[i] != 0) return set_error(GCAM_ERR_INVALID_ARGUMENT, "gcam_config.reserved must be zero");
     for (int i = 0; i < levels; i++) {
         const long w = width >> i, h = height >> i;
@@ -140,6 +142,7 @@ gcam_status gcam_pyramid_create(int32_t width, int32_t height, int32_t levels, c
     p->radius = cfg.search_radius;
     p->precision = cfg.precision;
     p->bm_rows = cfg.bm_rows_per_thread;
+    p->bilateral = cfg.bilateral != 0;
     p->device = dev;
     for (int i = 0; i < levels; i++) {
         p->W[i] = width >> i;
@@ -282,6 +285,21 @@ gcam_status gcam_pyramid_denoise(gcam_pyramid* p, const gcam_denoise_params* par
             layer = View<const float4>{sv.ptr, sv.w, sv.h, sv.stride};
             GCAM_MARK(i, GCAM_STAGE_SUBTRACT, 1);
         }
+        BmParams bp;
+        bp.thr[0] = params[i].luma;  // nflMultiplier (:52-56)
+        bp.thr[1] = params[i].chroma;
+        bp.thr[2] = params[i].chroma;
+        bp.chromaBoost = params[i].chromaBoost;
+        bp.gradientBoost = params[i].gradientBoost;
+        bp.gradientThreshold = params[i].gradientThreshold;
+        bp.lensShading = lens_shading_correction;
+        if (p->bilateral) {  // usePatchSimiliarity == false (:162-168)
+            GCAM_MARK(i, GCAM_STAGE_BLOCK_MATCH, 0);
+            st = launch_denoise_image(layer, gradv(i), p->nlf_dev + i, bp, view<float4>(p->den[i], p->W[i], p->H[i]), s);
+            if (st != GCAM_OK) return st;
+            GCAM_MARK(i, GCAM_STAGE_BLOCK_MATCH, 1);
+            continue;
+        }
         GCAM_MARK(i, GCAM_STAGE_PCA, 0);
         const int64_t n = ((int64_t)p->W[i] * p->H[i]) / 64;  // :142
         st = launch_image_moments(layer, p->moment_partials, s);
@@ -294,14 +312,6 @@ gcam_status gcam_pyramid_denoise(gcam_pyramid* p, const gcam_denoise_params* par
             if (st != GCAM_OK) return st;
         }
         GCAM_MARK(i, GCAM_STAGE_PCA, 1);
-        BmParams bp;
-        bp.thr[0] = params[i].luma;  // nflMultiplier (:52-56)
-        bp.thr[1] = params[i].chroma;
-        bp.thr[2] = params[i].chroma;
-        bp.chromaBoost = params[i].chromaBoost;
-        bp.gradientBoost = params[i].gradientBoost;
-        bp.gradientThreshold = params[i].gradientThreshold;
-        bp.lensShading = lens_shading_correction;
         GCAM_MARK(i, GCAM_STAGE_BLOCK_MATCH, 0);
         if (p->precision == GCAM_PRECISION_FP32)
             st = launch_block_match(layer, gradv(i), View<const float>{fv.ptr, fv.w, fv.h, fv.stride}, p->nlf_dev + i,
@@ -313,7 +323,7 @@ gcam_status gcam_pyramid_denoise(gcam_pyramid* p, const gcam_denoise_params* par
         GCAM_MARK(i, GCAM_STAGE_BLOCK_MATCH, 1);
     }
 #undef GCAM_MARK
-    p->feature_level = p->precision == GCAM_PRECISION_FP32 ? 0 : -1;
+    p->feature_level = (p->precision == GCAM_PRECISION_FP32 && !p->bilateral) ? 0 : -1;
 
     if (calibrate_from_image) {
         NlfDev host[GCAM_MAX_LEVELS];
@@ -536,6 +546,37 @@ gcam_status gcam_block_matching_denoise_image_lp(const gcam_image* input, const 
         st = launch_block_match_lp(view<const float4>(*input),
                                    gradient ? view<const float2>(*gradient) : View<const float2>{nullptr, 0, 0, 0},
                                    basis, nd, bp, search_radius, 0, precision, view<float4>(*output), s);
+    GCAM_HIP_TRY(hipFreeAsync(nd, s));
+    return st;
+}
+
+gcam_status gcam_denoise_image(const gcam_image* input, const gcam_image* gradient, const float var_a[3],
+                               const float var_b[3], const float threshold_multipliers[3], float chroma_boost,
+                               float gradient_boost, float gradient_threshold, gcam_image* output, void* stream) {
+    gcam_status st = check_image(input, GCAM_PIXEL_RGBA32F, "input");
+    if (st == GCAM_OK) st = check_image(output, GCAM_PIXEL_RGBA32F, "output", input->width, input->height);
+    if (st == GCAM_OK && gradient) st = check_image(gradient, GCAM_PIXEL_RG32F, "gradient", input->width, input->height);
+    if (st != GCAM_OK) return st;
+    if (!var_a || !var_b || !threshold_multipliers) return set_error(GCAM_ERR_INVALID_ARGUMENT, "NULL parameter");
+    hipStream_t s = as_stream(stream);
+    NlfDev* nd = nullptr;
+    GCAM_HIP_TRY(hipMallocAsync((void**)&nd, sizeof(NlfDev), s));
+    gcam_ycbcr_nlf hn;
+    for (int c = 0; c < 3; c++) {
+        hn.a[c] = var_a[c];
+        hn.b[c] = var_b[c];
+    }
+    st = launch_set_nlf(&hn, 1, nd, s);
+    BmParams bp;
+    for (int c = 0; c < 3; c++) bp.thr[c] = threshold_multipliers[c];
+    bp.chromaBoost = chroma_boost;
+    bp.gradientBoost = gradient_boost;
+    bp.gradientThreshold = gradient_threshold;
+    bp.lensShading = 0.0f;
+    if (st == GCAM_OK)
+        st = launch_denoise_image(view<const float4>(*input),
+                                  gradient ? view<const float2>(*gradient) : View<const float2>{nullptr, 0, 0, 0}, nd,
+                                  bp, view<float4>(*output), s);
     GCAM_HIP_TRY(hipFreeAsync(nd, s));
     return st;
 }

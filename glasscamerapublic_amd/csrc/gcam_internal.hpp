@@ -105,6 +105,10 @@ gcam_status launch_block_match_lp(View<const float4> in, View<const float2> grad
                                   const NlfDev* nlf_level, const BmParams& p, int radius, int rows_per_thread,
                                   int precision, View<float4> out, hipStream_t s);
 
+// denoiseImage, the non-PCA bilateral (usePatchSimiliarity == false).
+gcam_status launch_denoise_image(View<const float4> in, View<const float2> grad, const NlfDev* nlf_level,
+                                 const BmParams& p, View<float4> out, hipStream_t s);
+
 gcam_status launch_synth(int w, int h, const gcam_synth_params& p, uint16_t* raw, View<float4> ycc,
                          View<float4> clean, hipStream_t s);
 

@@ -84,16 +84,21 @@ class PyramidProcessor:
 
     search_radius: half-width of the block-matching window; the reference
     hard-codes 10 (21 x 21, demosaic.metal:853).
-    precision: "fp32" (plain form).
+    precision: "fp32" (plain form), "fp16" or "bf16" (contractions in 16 bits,
+    FP32 accumulation).
+    usePatchSimiliarity: the reference's switch (pyramid_processor.hpp:27);
+    False selects the non-PCA denoiseImage bilateral.
     """
 
     def __init__(self, width: int, height: int, levels: int = 5, search_radius: int = 10,
-                 precision: str = "fp32", device: Optional[torch.device] = None, bm_rows_per_thread: int = 0):
+                 precision: str = "fp32", device: Optional[torch.device] = None, bm_rows_per_thread: int = 0,
+                 usePatchSimiliarity: bool = True):
         self.width, self.height, self.levels = int(width), int(height), int(levels)
         self.search_radius = int(search_radius)
         self.precision = precision
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
-        cfg = ConfigC(self.search_radius, _lib.GCAM_PRECISION[precision], int(bm_rows_per_thread))
+        cfg = ConfigC(self.search_radius, _lib.GCAM_PRECISION[precision], int(bm_rows_per_thread),
+                      0 if usePatchSimiliarity else 1)
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             check(lib().gcam_pyramid_create(self.width, self.height, self.levels, ctypes.byref(cfg),
@@ -283,6 +288,20 @@ def block_matching_denoise_image_lp(image, basis, var_a, var_b, thresholdMultipl
         va, vb, th, float(chromaBoost), float(gradientBoost), float(gradientThreshold), float(lensShadingCorrection),
         int(search_radius), _lib.GCAM_PRECISION[precision], ctypes.byref(as_image(out)),
         ctypes.c_void_p(_stream(stream))))
+    return out
+
+
+def denoise_image(image, var_a, var_b, thresholdMultipliers, chromaBoost=1.0, gradientBoost=0.0,
+                  gradientThreshold=1.0, gradient=None, stream=None) -> torch.Tensor:
+    """denoiseImage (demosaic.metal:664-719): the non-PCA bilateral."""
+    out = torch.empty_like(image)
+    va = (ctypes.c_float * 3)(*[float(v) for v in var_a])
+    vb = (ctypes.c_float * 3)(*[float(v) for v in var_b])
+    th = (ctypes.c_float * 3)(*[float(v) for v in thresholdMultipliers])
+    g = as_image(gradient)
+    check(lib().gcam_denoise_image(ctypes.byref(as_image(image)), ctypes.byref(g) if g is not None else None, va, vb,
+                                   th, float(chromaBoost), float(gradientBoost), float(gradientThreshold),
+                                   ctypes.byref(as_image(out)), ctypes.c_void_p(_stream(stream))))
     return out
 
 

@@ -85,7 +85,10 @@ typedef struct gcam_config {
     int32_t precision;     /* gcam_precision                                   */
     int32_t bm_rows_per_thread; /* block-matching tile shape: output rows per
                                    thread, 1, 2 or 4; 0 = tuned default         */
-    int32_t reserved[5];   /* must be zero                                     */
+    int32_t bilateral;     /* 0: PCA block matching (usePatchSimiliarity =
+                              true, pyramid_processor.hpp:27); 1: the non-PCA
+                              denoiseImage bilateral (demosaic.metal:669-719) */
+    int32_t reserved[4];   /* must be zero                                     */
 } gcam_config;
 
 typedef struct gcam_pyramid gcam_pyramid;
@@ -205,6 +208,13 @@ gcam_status gcam_block_matching_denoise_image_lp(const gcam_image* input, const 
                                                  float gradient_boost, float gradient_threshold,
                                                  float lens_shading_correction, int32_t search_radius,
                                                  int32_t precision, gcam_image* output, void* stream);
+
+/* denoiseImageKernel (demosaic_kernels.hpp:317-343), demosaic.metal:664-719:
+ * the non-PCA bilateral (5x5 window, 9x9 when gradient_boost > 0).
+ * gradient may be NULL.  Output .w = gradient magnitude, as the reference. */
+gcam_status gcam_denoise_image(const gcam_image* input, const gcam_image* gradient, const float var_a[3],
+                               const float var_b[3], const float threshold_multipliers[3], float chroma_boost,
+                               float gradient_boost, float gradient_threshold, gcam_image* output, void* stream);
 
 /* basicNoiseStatisticsKernel (demosaic_kernels.hpp:438-450), demosaic.metal:1138-1159. */
 gcam_status gcam_basic_noise_statistics(const gcam_image* input, gcam_image* statistics, void* stream);

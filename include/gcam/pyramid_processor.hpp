@@ -37,9 +37,9 @@ class PyramidProcessor {
     const int width, height;
 
     PyramidProcessor(int width_, int height_, int searchRadius = 10,
-                     gcam_precision precision = GCAM_PRECISION_FP32)
+                     gcam_precision precision = GCAM_PRECISION_FP32, bool usePatchSimiliarity = true)
         : width(width_), height(height_) {
-        gcam_config cfg{searchRadius, precision, 0, {0, 0, 0, 0, 0}};
+        gcam_config cfg{searchRadius, precision, 0, usePatchSimiliarity ? 0 : 1, {0, 0, 0, 0}};
         check(gcam_pyramid_create(width, height, (int32_t)levels, &cfg, &handle_));
     }
     ~PyramidProcessor() { gcam_pyramid_destroy(handle_); }

@@ -61,7 +61,8 @@ def _declare(L):
         getattr(L, f"oracle_pca_project_{sfx}").argtypes = [_fp, _i, _i, _fp, _fp]
         getattr(L, f"oracle_block_match_{sfx}").argtypes = [_fp, _fp, _fp, _i, _i, _fp, _fp, _fp, R, R, R, R, _i, _fp]
         getattr(L, f"oracle_noise_stats_{sfx}").argtypes = [_fp, _i, _i, _fp]
-        getattr(L, f"oracle_pyramid_denoise_{sfx}").argtypes = [_fp, _fp, _i, _i, _i, _i, _fp, _fp, R, R, _i,
+        getattr(L, f"oracle_denoise_image_{sfx}").argtypes = [_fp, _fp, _i, _i, _fp, _fp, _fp, R, R, R, _fp]
+        getattr(L, f"oracle_pyramid_denoise_{sfx}").argtypes = [_fp, _fp, _i, _i, _i, _i, _i, _fp, _fp, R, R, _i,
                                                                  _fp, _fp, _fp, _fp]
         getattr(L, f"oracle_pyramid_denoise_{sfx}").restype = _i
     L.oracle_eigh_jacobi.argtypes = [_fp, _i, _fp, _fp]
@@ -223,8 +224,22 @@ class PyramidResult:
     nlf: np.ndarray
 
 
+def denoise_image(img, var_a, var_b, thr, chroma_boost=1.0, gradient_boost=0.0, gradient_threshold=1.0,
+                  grad=None):
+    """denoiseImage (demosaic.metal:664-719): the non-PCA bilateral."""
+    dt = img.dtype
+    h, w, _ = img.shape
+    img = _c(img, dt)
+    grad = None if grad is None else _c(grad, dt)
+    va, vb, th = (_c(np.asarray(v).reshape(3), dt) for v in (var_a, var_b, thr))
+    out = np.zeros_like(img)
+    getattr(lib(), f"oracle_denoise_image_{_sfx(dt)}")(_p(img), _p(grad), w, h, _p(va), _p(vb), _p(th),
+                                                       chroma_boost, gradient_boost, gradient_threshold, _p(out))
+    return out
+
+
 def pyramid_denoise(image, params, nlf, levels=5, radius=10, gradient=None, exposure_multiplier=1.0,
-                    lens_shading=0.0, calibrate=False, dtype=np.float32) -> PyramidResult:
+                    lens_shading=0.0, calibrate=False, dtype=np.float32, patch_similarity=True) -> PyramidResult:
     """Whole multi-level denoise (MetalPipeline/pyramid_processor.cpp:96-172).
 
     params: sequence of ``levels`` dicts/objects with the DenoiseParameters fields.
@@ -245,7 +260,8 @@ def pyramid_denoise(image, params, nlf, levels=5, radius=10, gradient=None, expo
     bases = np.zeros((levels, 25, 8))
     ev = np.zeros((levels, 25))
     rc = getattr(lib(), f"oracle_pyramid_denoise_{_sfx(dtype)}")(
-        _p(image), _p(grad), w, h, levels, radius, ctypes.cast(P, ctypes.c_void_p), _p(nlf_arr),
+        _p(image), _p(grad), w, h, levels, radius, int(bool(patch_similarity)), ctypes.cast(P, ctypes.c_void_p),
+        _p(nlf_arr),
         exposure_multiplier, lens_shading, int(bool(calibrate)), _p(den), _p(sub), _p(bases), _p(ev))
     if rc != 0:
         raise ValueError("pyramid level too small for PCA patch sampling")

@@ -401,6 +401,65 @@ void block_match(const T* in, const T* grad, const T* feat, int w, int h, const 
     });
 }
 
+// demosaic.metal:664-719: tunnel() and denoiseImage.  Metal's step(e, x) is
+// x < e ? 0 : 1; length() sums x, y, z in order.
+template <class T>
+void denoise_image(const T* in, const T* grad, int w, int h, const T var_a[3], const T var_b[3], const T thr[3],
+                   T chromaBoost, T gradientBoost, T gradientThreshold, T* out) {
+    parallel_rows(h, [&](int ya, int yb) {
+        for (int py = ya; py < yb; py++) {
+            for (int px = 0; px < w; px++) {
+                const T* ip = in + ((size_t)py * w + px) * 4;
+                T sigma[3], dm[3];
+                for (int c = 0; c < 3; c++) {
+                    sigma[c] = std::sqrt(var_a[c] + var_b[c] * ip[0]);
+                    dm[c] = T(1) / (thr[c] * sigma[c]);
+                }
+                T gx = 0, gy = 0;
+                if (grad) {
+                    gx = grad[((size_t)py * w + px) * 2];
+                    gy = grad[((size_t)py * w + px) * 2 + 1];
+                }
+                const T angle = std::atan2(gy, gx);
+                const T magnitude = std::sqrt(gx * gx + gy * gy);
+                const T edge = smoothstep(T(4), T(16), gradientThreshold * magnitude / sigma[0]);
+                const T ca = std::cos(angle), sa = std::sin(angle);
+                const int size = gradientBoost > 0 ? 4 : 2;
+                T fil[3] = {0, 0, 0}, norm[3] = {0, 0, 0};
+                for (int y = -size; y <= size; y++) {
+                    for (int x = -size; x <= size; x++) {
+                        const int sx = clampi(px + x, 0, w - 1), sy = clampi(py + y, 0, h - 1);
+                        const T* sp = in + ((size_t)sy * w + sx) * 4;
+                        T sgx = 0, sgy = 0;
+                        if (grad) {
+                            sgx = grad[((size_t)sy * w + sx) * 2];
+                            sgy = grad[((size_t)sy * w + sx) * 2 + 1];
+                        }
+                        T d[3];
+                        for (int c = 0; c < 3; c++) d[c] = (sp[c] - ip[c]) * dm[c];
+                        const T gdx = (sgx - gx) / sigma[0], gdy = (sgy - gy) / sigma[0];
+                        const T a = T(x) * ca + T(y) * sa;
+                        const T tunnel = std::exp(-(a * a) / T(0.25));
+                        const T directionWeight = T(1) + (tunnel - T(1)) * edge;  // mix(1, tunnel, edge)
+                        const T gradientWeight = T(1) - smoothstep(T(2), T(8), std::sqrt(gdx * gdx + gdy * gdy));
+                        const T lumaWeight = std::fabs(d[0]) < T(1) + gradientBoost * edge ? T(1) : T(0);
+                        const T chromaWeight =
+                            std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) < chromaBoost ? T(1) : T(0);
+                        const T sw[3] = {directionWeight * gradientWeight * lumaWeight, chromaWeight, chromaWeight};
+                        for (int c = 0; c < 3; c++) {
+                            fil[c] += sw[c] * sp[c];
+                            norm[c] += sw[c];
+                        }
+                    }
+                }
+                T* o = out + ((size_t)py * w + px) * 4;
+                for (int c = 0; c < 3; c++) o[c] = fil[c] / norm[c];
+                o[3] = magnitude;
+            }
+        }
+    });
+}
+
 // demosaic.metal:1138-1159, clamp-to-edge reads.
 template <class T>
 void noise_stats(const T* in, int w, int h, T* stats) {
@@ -452,6 +511,16 @@ void oracle_block_match_f64(const double* in, const double* grad, const double* 
                             const double var_a[3], const double var_b[3], const double thr[3], double cB, double gB,
                             double gT, double ls, int radius, double* out) {
     block_match(in, grad, feat, w, h, var_a, var_b, thr, cB, gB, gT, ls, radius, out);
+}
+
+void oracle_denoise_image_f32(const float* in, const float* grad, int w, int h, const float var_a[3],
+                              const float var_b[3], const float thr[3], float cB, float gB, float gT, float* out) {
+    denoise_image(in, grad, w, h, var_a, var_b, thr, cB, gB, gT, out);
+}
+void oracle_denoise_image_f64(const double* in, const double* grad, int w, int h, const double var_a[3],
+                              const double var_b[3], const double thr[3], double cB, double gB, double gT,
+                              double* out) {
+    denoise_image(in, grad, w, h, var_a, var_b, thr, cB, gB, gT, out);
 }
 
 void oracle_noise_stats_f32(const float* in, int w, int h, float* stats) { noise_stats(in, w, h, stats); }
@@ -580,7 +649,7 @@ int oracle_fit_ycbcr_nlf(const float* stats, int w, int h, double exposure_multi
 namespace {
 
 template <class T>
-int pyramid_denoise(const T* image, const T* gradient, int w, int h, int levels, int radius,
+int pyramid_denoise(const T* image, const T* gradient, int w, int h, int levels, int radius, int patch_similarity,
                     const oracle_denoise_params* params, double* nlf, T exposure, T lensShading, int calibrate,
                     T* denoised, T* subtracted_out, double* bases, double* eigvals) {
     std::vector<int> W(levels), H(levels);
@@ -628,6 +697,16 @@ int pyramid_denoise(const T* image, const T* gradient, int w, int h, int levels,
             layerImg = L(i);
         }
         if (subtracted_out) std::memcpy(subtracted_out + off[i] * 4, layerImg, sizeof(T) * W[i] * H[i] * 4);
+        if (!patch_similarity) {  // :162-168
+            T va[3], vb[3];
+            for (int c = 0; c < 3; c++) {
+                va[c] = (T)nlf[i * 6 + c];
+                vb[c] = (T)nlf[i * 6 + 3 + c];
+            }
+            denoise_image(layerImg, G(i), W[i], H[i], va, vb, thr[i], (T)params[i].chromaBoost,
+                          (T)params[i].gradientBoost, (T)params[i].gradientThreshold, denoised + off[i] * 4);
+            continue;
+        }
         std::vector<T> patches((size_t)W[i] * H[i] / 64 * 25 + 25);
         int n = collect_patches(layerImg, W[i], H[i], patches.data());
         double basis_d[200], ev[25], cov[625];
@@ -656,15 +735,17 @@ int pyramid_denoise(const T* image, const T* gradient, int w, int h, int levels,
 extern "C" {
 
 int oracle_pyramid_denoise_f32(const float* image, const float* gradient, int w, int h, int levels, int radius,
+                               int patch_similarity,
                                const oracle_denoise_params* params, double* nlf, float exposure, float ls,
                                int calibrate, float* denoised, float* subtracted, double* bases, double* eigvals) {
-    return pyramid_denoise(image, gradient, w, h, levels, radius, params, nlf, exposure, ls, calibrate, denoised,
+    return pyramid_denoise(image, gradient, w, h, levels, radius, patch_similarity, params, nlf, exposure, ls, calibrate, denoised,
                            subtracted, bases, eigvals);
 }
 int oracle_pyramid_denoise_f64(const double* image, const double* gradient, int w, int h, int levels, int radius,
+                               int patch_similarity,
                                const oracle_denoise_params* params, double* nlf, double exposure, double ls,
                                int calibrate, double* denoised, double* subtracted, double* bases, double* eigvals) {
-    return pyramid_denoise(image, gradient, w, h, levels, radius, params, nlf, exposure, ls, calibrate, denoised,
+    return pyramid_denoise(image, gradient, w, h, levels, radius, patch_similarity, params, nlf, exposure, ls, calibrate, denoised,
                            subtracted, bases, eigvals);
 }
 

@@ -78,6 +78,16 @@ void oracle_block_match_f64(const double* in, const double* grad, const double* 
                             double gradientBoost, double gradientThreshold, double lensShading, int radius,
                             double* out);
 
+// MetalPipeline/demosaic.metal:664-719 (tunnel + denoiseImage), the non-PCA
+// bilateral used when usePatchSimiliarity is false
+// (MetalPipeline/pyramid_processor.hpp:27, .cpp:162-168).
+void oracle_denoise_image_f32(const float* in, const float* grad, int w, int h, const float var_a[3],
+                              const float var_b[3], const float thr[3], float chromaBoost, float gradientBoost,
+                              float gradientThreshold, float* out);
+void oracle_denoise_image_f64(const double* in, const double* grad, int w, int h, const double var_a[3],
+                              const double var_b[3], const double thr[3], double chromaBoost, double gradientBoost,
+                              double gradientThreshold, double* out);
+
 // MetalPipeline/demosaic.metal:1138-1159 (basicNoiseStatistics).
 void oracle_noise_stats_f32(const float* in, int w, int h, float* stats);
 void oracle_noise_stats_f64(const double* in, int w, int h, double* stats);
@@ -101,10 +111,12 @@ int oracle_fit_ycbcr_nlf(const float* stats, int w, int h, double exposure_multi
 //   eigvals (nullable): levels*25 doubles
 // Returns 0 on success, -1 if a level is too small to sample two patches.
 int oracle_pyramid_denoise_f32(const float* image, const float* gradient, int w, int h, int levels, int radius,
+                               int patch_similarity,
                                const oracle_denoise_params* params, double* nlf, float exposure_multiplier,
                                float lensShading, int calibrate, float* denoised, float* subtracted, double* bases,
                                double* eigvals);
 int oracle_pyramid_denoise_f64(const double* image, const double* gradient, int w, int h, int levels, int radius,
+                               int patch_similarity,
                                const oracle_denoise_params* params, double* nlf, double exposure_multiplier,
                                double lensShading, int calibrate, double* denoised, double* subtracted,
                                double* bases, double* eigvals);

@@ -165,3 +165,27 @@ def test_errors_are_raised_not_silent():
     with pytest.raises(gcam.GcamError, match="search radius"):
         gcam.block_matching_denoise_image(img, torch.zeros((32, 32, 8), device=DEV), [1] * 3, [1] * 3, [1] * 3,
                                           search_radius=11)
+
+
+@pytest.mark.parametrize("gradient_boost", [0.0, 1.5])
+def test_denoise_image_parity(gradient_boost):
+    img = rand_img(53, 71, 15)
+    img[..., 0] = np.clip(img[..., 0], 0.05, 1)
+    grad = np.random.default_rng(16).normal(0, 0.05, img.shape[:2] + (2,)).astype(np.float32)
+    args = ([1e-4, 2e-4, 2e-4], [1e-2, 5e-3, 5e-3], [1.5, 1.0, 1.2], 1.3, gradient_boost, 30.0)
+    got = to_host(gcam.denoise_image(to_dev(img), *args, gradient=to_dev(grad)))
+    ref = oracle.denoise_image(img, *args, grad=grad)
+    np.testing.assert_allclose(got[..., :3], ref[..., :3], atol=1e-5)
+    np.testing.assert_allclose(got[..., 3], ref[..., 3], atol=1e-6)
+
+
+def test_pyramid_bilateral_mode_parity(oracle_threads):
+    from tests.helpers import nlf_for
+    ycc = gcam.synth_frame(320, 240, 4e-3, 2e-3, scene_seed=17, noise_seed=18)
+    nlf = [(tuple(x[0]), tuple(x[1])) for x in nlf_for(3)]
+    params = [gcam.DenoiseParameters(luma=2.0, chromaBoost=1.5) for _ in range(3)]
+    proc = gcam.PyramidProcessor(320, 240, levels=3, usePatchSimiliarity=False)
+    proc.denoise(params, ycc, None, list(nlf))
+    ref = oracle.pyramid_denoise(to_host(ycc), params, np.array(nlf), levels=3, patch_similarity=False)
+    for i, lvl in enumerate(proc.denoisedImagePyramid):
+        np.testing.assert_allclose(to_host(lvl), ref.denoised[i], atol=1e-4, err_msg=f"level {i}")

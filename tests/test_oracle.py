@@ -408,3 +408,69 @@ def test_pyramid_golden_fixture():
     for i in range(2):
         np.testing.assert_allclose(res.denoised[i], g[f"denoised{i}"], atol=2e-6)
         np.testing.assert_allclose(res.bases[i] @ res.bases[i].T, g[f"projector{i}"], atol=1e-9)
+
+
+# ---------------- denoiseImage (demosaic.metal:664-719) ----------------
+
+def test_denoise_image_limits():
+    img = rand_img(12, 14, 40)
+    # constant image: fixed point; magnitude channel = 0 without a gradient
+    c = np.zeros((9, 9, 4))
+    c[..., :3] = [0.3, 0.1, -0.1]
+    out = oracle.denoise_image(c, [1e-4] * 3, [1e-3] * 3, [1, 1, 1])
+    np.testing.assert_allclose(out[..., :3], c[..., :3], atol=1e-15)
+    assert np.all(out[..., 3] == 0)
+    # huge thresholds: every sample passes both steps -> plain 5x5 (or 9x9 with gradientBoost) box mean
+    for gb, size in ((0.0, 2), (1.0, 4)):
+        out = oracle.denoise_image(img, [1e-4] * 3, [1e-3] * 3, [1e9] * 3, gradient_boost=gb)
+        pad = np.pad(img[..., :3], ((size, size), (size, size), (0, 0)), mode="edge")
+        win = np.lib.stride_tricks.sliding_window_view(pad, (2 * size + 1, 2 * size + 1), axis=(0, 1))
+        np.testing.assert_allclose(out[..., :3], win.mean(axis=(-1, -2)), atol=1e-12)
+    # tiny thresholds: only the centre passes
+    out = oracle.denoise_image(img, [1e-4] * 3, [1e-3] * 3, [1e-9] * 3)
+    np.testing.assert_allclose(out[..., :3], img[..., :3], atol=1e-15)
+
+
+def test_denoise_image_brute_force_with_gradient():
+    img = rand_img(8, 9, 41)
+    grad = np.random.default_rng(42).normal(0, 0.05, (8, 9, 2))
+    va, vb, th = np.array([1e-4, 2e-4, 2e-4]), np.array([1e-2, 5e-3, 5e-3]), np.array([1.5, 1.0, 1.2])
+    cB, gB, gT = 1.3, 0.7, 30.0
+    out = oracle.denoise_image(img, va, vb, th, cB, gB, gT, grad=grad)
+    h, w = 8, 9
+
+    def ss(e0, e1, x):
+        t = np.clip((x - e0) / (e1 - e0), 0, 1)
+        return t * t * (3 - 2 * t)
+    for py in range(h):
+        for px in range(w):
+            sig = np.sqrt(va + vb * img[py, px, 0])
+            dm = 1 / (th * sig)
+            g = grad[py, px]
+            ang, mag = np.arctan2(g[1], g[0]), np.hypot(*g)
+            edge = ss(4, 16, gT * mag / sig[0])
+            num, den = np.zeros(3), np.zeros(3)
+            for y in range(-4, 5):
+                for x in range(-4, 5):
+                    qy, qx = min(max(py + y, 0), h - 1), min(max(px + x, 0), w - 1)
+                    d = (img[qy, qx, :3] - img[py, px, :3]) * dm
+                    gd = (grad[qy, qx] - g) / sig[0]
+                    a = x * np.cos(ang) + y * np.sin(ang)
+                    dw = 1 + (np.exp(-a * a / 0.25) - 1) * edge
+                    gw = 1 - ss(2, 8, np.hypot(*gd))
+                    lw = 1.0 if abs(d[0]) < 1 + gB * edge else 0.0
+                    cw = 1.0 if np.linalg.norm(d) < cB else 0.0
+                    sw = np.array([dw * gw * lw, cw, cw])
+                    num += sw * img[qy, qx, :3]
+                    den += sw
+            np.testing.assert_allclose(out[py, px, :3], num / den, atol=1e-12)
+            assert out[py, px, 3] == pytest.approx(mag, rel=1e-12)
+
+
+def test_pyramid_bilateral_mode_runs_denoise_image():
+    noisy, _ = noisy_ycc(128, 96, seed=43)
+    res = oracle.pyramid_denoise(noisy, default_params(3), nlf_for(3), levels=3, patch_similarity=False)
+    # the coarsest level is denoiseImage of the downsampled input
+    lvl2_in = oracle.downsample(oracle.downsample(noisy))
+    ref = oracle.denoise_image(lvl2_in, nlf_for(3)[2, 0], nlf_for(3)[2, 1], [1, 1, 1])
+    np.testing.assert_allclose(res.denoised[2], ref, atol=1e-6)
