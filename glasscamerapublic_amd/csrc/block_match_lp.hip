@@ -9,18 +9,19 @@
 //
 // 1. Luma of the tile plus an (R+2)-pixel apron is staged in LDS, centred on
 //    the tile's first pixel and scaled by the level scale s (~1/sigma of the
-//    level), then rounded to FP16/BF16.  Feature distances are invariant to the
-//    centring; scaling divides the luma coefficient by s^2.  Centring keeps
-//    the rounded values small so their absolute rounding error is small.
+//    level).  Feature distances are invariant to the centring; the scale
+//    divides the luma coefficient by s^2.
 // 2. Projection F = P B on MFMA: v_mfma_f32_16x16x32_{f16,bf16}, 16 pixels x
 //    (25 patch values padded to K = 32) x (8 components padded to N = 16),
-//    FP32 accumulation.  The features are rounded to FP16/BF16 in LDS, and
-//    their squared norms |f|^2 are summed in FP32 from the rounded values.
-// 3. Distances as a Gram form: d^2 = |f_p|^2 + |f_q|^2 - 2 f_p.f_q.  The dot
-//    product runs as 4 packed dot2 (v_dot2c_f32_{f16,bf16}: FP16/BF16
-//    products, FP32 accumulation).  On the rounded features this is exactly
-//    the squared distance up to FP32 rounding.  The luma/chroma weights and
-//    the accumulation are FP32, as in the FP32 form.
+//    FP32 accumulation.  Each luma value enters as a 16-bit high part plus a
+//    16-bit remainder (two MFMAs), so the projection input keeps ~22 bits.
+// 3. Component 0 (the largest-variance one, in practice the patch mean) spans
+//    the widest range across a tile, so its rounding error would dominate the
+//    distances: it stays FP32.  Components 1..7 are rounded to FP16/BF16 and
+//    their distance is taken as a Gram form |f_p|^2 + |f_q|^2 - 2 f_p.f_q, the
+//    dot product as 4 packed dot2 (v_dot2c_f32_{f16,bf16}: 16-bit products,
+//    FP32 accumulation) -- exactly the squared distance of the rounded
+//    features up to FP32 rounding.  Weights and sums are FP32.
 #include "gcam_internal.hpp"
 
 #include <cmath>
@@ -42,8 +43,7 @@ template <>
 struct Prec<_Float16> {
     using h2 = f16x2;
     using h8 = f16x8;
-    static __device__ __forceinline__ f32x4 mfma(h8 a, h8 b) {
-        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+    static __device__ __forceinline__ f32x4 mfma(h8 a, h8 b, f32x4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     }
     static __device__ __forceinline__ float dot2(h2 a, h2 b, float c) { return __builtin_amdgcn_fdot2(a, b, c, false); }
@@ -53,8 +53,7 @@ template <>
 struct Prec<__bf16> {
     using h2 = bf16x2;
     using h8 = bf16x8;
-    static __device__ __forceinline__ f32x4 mfma(h8 a, h8 b) {
-        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+    static __device__ __forceinline__ f32x4 mfma(h8 a, h8 b, f32x4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
     static __device__ __forceinline__ float dot2(h2 a, h2 b, float c) {
@@ -85,7 +84,7 @@ struct LpTile {
     static constexpr int LW = FW + 4, LH = FH + 4, LN = LW * LH;
     static constexpr int THREADS = TW * TH, WAVES = THREADS / 64, GROUPS = (FN + 15) / 16;
     static_assert(THREADS % 64 == 0, "whole wavefronts");
-    static_assert(LN * 2 + FN * 32 <= 64 * 1024, "LDS tile too large");
+    static_assert(LN * 4 + FN * 36 <= 64 * 1024, "LDS tile too large");
 };
 
 template <int R, int PV, int TW, int TH, class H>
@@ -97,9 +96,10 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
     using P = Prec<H>;
     using h2 = typename P::h2;
     using h8 = typename P::h8;
-    __shared__ H lum[T::LN];
-    __shared__ float4 ycn[T::FN];  // Y, Cb, Cr, |f|^2 (scaled, rounded features)
-    __shared__ h8 fh[T::FN];       // 8 rounded features per pixel
+    __shared__ float lum[T::LN];   // centred, scaled luma
+    __shared__ float4 ycn[T::FN];  // Y, Cb, Cr, |f_1..7|^2 (scaled, rounded)
+    __shared__ float f0s[T::FN];   // component 0, FP32
+    __shared__ h8 fh[T::FN];       // components 1..7 rounded, 0
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH * PV;
     const int tid = threadIdx.y * TW + threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
     for (int e = tid; e < T::LN; e += T::THREADS) {
         const int ly = e / T::LW, lx = e % T::LW;
         const float yv = in.at(clampi(x0 - R - 2 + lx, 0, in.w - 1), clampi(y0 - R - 2 + ly, 0, in.h - 1)).x;
-        lum[e] = (H)((yv - mu) * scale);
+        lum[e] = (yv - mu) * scale;
     }
     for (int e = tid; e < T::FN; e += T::THREADS) {
         const int fy = e / T::FW, fx = e % T::FW;
@@ -138,27 +138,36 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
         const int i = lane & 15, kg = lane >> 4;
         const int e = min(g * 16 + i, T::FN - 1);
         const int fy = e / T::FW, fx = e % T::FW;
-        h8 aop;
+        h8 ahi, alo;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
             const int k = 8 * kg + v;
-            aop[v] = k < kPatch ? lum[(fy + k / 5) * T::LW + fx + k % 5] : (H)0.0f;
+            const float yv = k < kPatch ? lum[(fy + k / 5) * T::LW + fx + k % 5] : 0.0f;
+            const H hi = (H)yv;
+            ahi[v] = hi;
+            alo[v] = (H)(yv - (float)hi);
         }
-        const f32x4 c = P::mfma(aop, bop);
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        c = P::mfma(ahi, bop, c);
+        c = P::mfma(alo, bop, c);
         // C: lane holds pixel 4 (lane / 16) + r of the group, component lane % 16.
         const int j = lane & 15;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const H hq = (H)c[r];
             const float fq = (float)hq;
-            float n = j < kComponents ? fq * fq : 0.0f;
+            float n = (j >= 1 && j < kComponents) ? fq * fq : 0.0f;
             n += __shfl_xor(n, 1);
             n += __shfl_xor(n, 2);
             n += __shfl_xor(n, 4);
             const int er = g * 16 + 4 * kg + r;
             if (er < T::FN) {
-                if (j < kComponents) reinterpret_cast<H*>(&fh[er])[j] = hq;
-                if (j == 0) ycn[er].w = n;
+                if (j >= 1 && j < kComponents) reinterpret_cast<H*>(&fh[er])[j - 1] = hq;
+                if (j == kComponents) reinterpret_cast<H*>(&fh[er])[kComponents - 1] = (H)0.0f;
+                if (j == 0) {
+                    f0s[er] = c[r];
+                    ycn[er].w = n;
+                }
             }
         }
     }
@@ -168,7 +177,7 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
     if (x >= out.w || yb >= out.h) return;
 
     h8 fp[PV];
-    float kl[PV], k2[PV], cp[PV], sb[PV], sr[PV], sbc[PV], src[PV];
+    float f0p[PV], kl[PV], k2[PV], cp[PV], sb[PV], sr[PV], sbc[PV], src[PV];
     float fY[PV], fCb[PV], fCr[PV], nY[PV], nC[PV];
     const float kw = -2.0f * kLog2e, kroot = sqrtf(2.0f * kLog2e);
 #pragma unroll
@@ -177,6 +186,7 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
         const int c = (threadIdx.y * PV + k + R) * T::FW + threadIdx.x + R;
         const float4 v = ycn[c];
         fp[k] = fh[c];
+        f0p[k] = f0s[c];
         float lens = 1.0f;
         if (p.lensShading > 0.0f) {
             const float cx = (float)(in.w / 2), cy = (float)(in.h / 2);
@@ -213,6 +223,7 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
             const int q = rowq + dx;
             const h8 fq = fh[q];
             const float4 qy = ycn[q];
+            const float f0q = f0s[q];
 #pragma unroll
             for (int k = 0; k < PV; k++) {
                 const int ddy = dy - k;
@@ -223,7 +234,9 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
                 G = P::dot2(h2{fq[2], fq[3]}, h2{fp[k][2], fp[k][3]}, G);
                 G = P::dot2(h2{fq[4], fq[5]}, h2{fp[k][4], fp[k][5]}, G);
                 G = P::dot2(h2{fq[6], fq[7]}, h2{fp[k][6], fp[k][7]}, G);
-                const float wl = __builtin_amdgcn_exp2f(fmaf(k2[k], G, fmaf(kl[k], qy.w, cp[k])));
+                const float t0 = f0q - f0p[k];
+                const float wl =
+                    __builtin_amdgcn_exp2f(fmaf(kl[k] * t0, t0, fmaf(k2[k], G, fmaf(kl[k], qy.w, cp[k]))));
                 const float wY = wd * wl;
                 const float tb = fmaf(sb[k], qy.y, -sbc[k]), tr = fmaf(sr[k], qy.z, -src[k]);
                 float e = fmaf(-tb, tb, lwd);
