@@ -272,3 +272,26 @@ def pyramid_denoise(image, params, nlf, levels=5, radius=10, gradient=None, expo
         sl.append(sub[o:o + n].reshape(lh, lw, 4))
         o += n
     return PyramidResult(dl, sl, bases, ev, nlf_arr.reshape(levels, 2, 3))
+
+
+def pyramid_loss_grad(image, clean, thr, chroma_boost, nlf, levels=5, radius=4, want_grad=True):
+    """Differentiation form, FP64 (oracle_pyramid_loss_grad_f64): (loss, dL/dthr (levels, 3) or None, denoised0).
+
+    thr: (levels, 3) per-channel thresholds; chroma_boost: (levels,); nlf: (levels, 2, 3).
+    """
+    L = lib()
+    if not hasattr(L, "_loss_grad_declared"):
+        L.oracle_pyramid_loss_grad_f64.argtypes = [_fp, _fp, _i, _i, _i, _i, _fp, _fp, _fp, _fp, _fp]
+        L.oracle_pyramid_loss_grad_f64.restype = _d
+        L._loss_grad_declared = True
+    image = _c(image, np.float64)
+    clean = _c(clean, np.float64)
+    h, w, _ = image.shape
+    thr = _c(np.asarray(thr).reshape(levels, 3), np.float64)
+    cb = _c(np.asarray(chroma_boost).reshape(levels), np.float64)
+    nl = _c(np.asarray(nlf).reshape(levels, 6), np.float64)
+    g = np.zeros((levels, 3)) if want_grad else None
+    den = np.zeros((h, w, 4))
+    loss = L.oracle_pyramid_loss_grad_f64(_p(image), _p(clean), w, h, levels, radius, _p(thr), _p(cb), _p(nl),
+                                          _p(g), _p(den))
+    return loss, g, den

@@ -750,3 +750,254 @@ int oracle_pyramid_denoise_f64(const double* image, const double* gradient, int 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Differentiation form (FP64): forward + reverse-mode gradient w.r.t. the
+// per-level, per-channel thresholds.  Forward as pyramid_processor.cpp:96-172
+// with gradientBoost = 0, sharpening = 1, lens shading 0 and no gradient
+// image; every stage is differentiated by scatter (the device adjoint gathers,
+// so the two formulations differ).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct LevelState {
+    int w = 0, h = 0, n = 0;
+    std::vector<double> X, S, F, D;  // noisy, layer (subtracted), features, denoised
+    double V[625], lam[25], B[200], sgn[8], mu[25];
+};
+
+// Full eigen-decomposition of the patch covariance and the oriented basis.
+void pca_full(const double* patches, int n, double* V, double* lam, double* B, double* sgn, double* mu) {
+    double cov[625];
+    patch_covariance(patches, n, cov);
+    for (int r = 0; r < 25; r++) {
+        double m = 0;
+        for (int k = 0; k < n; k++) m += patches[(size_t)k * 25 + r];
+        mu[r] = m / n;
+    }
+    oracle_eigh_jacobi(cov, 25, lam, V);
+    for (int c = 0; c < 8; c++) {
+        double dot = 0;
+        for (int r = 0; r < 25; r++) dot += V[r * 25 + c] * sign_ref(r);
+        sgn[c] = dot < 0 ? -1.0 : 1.0;
+        for (int r = 0; r < 25; r++) B[r * 8 + c] = sgn[c] * V[r * 25 + c];
+    }
+}
+
+// Reverse of block_match (demosaic.metal:806-884) with gB = 0, no lens shading.
+void bm_backward(const LevelState& L, const double thr[3], double cB, const double va[3], const double vb[3],
+                 int R, const double* gD, double* gS, double* gF, double gthr[3]) {
+    const int w = L.w, h = L.h;
+    const double* S = L.S.data();
+    const double* F = L.F.data();
+    const double* D = L.D.data();
+    for (int py = 0; py < h; py++) {
+        for (int px = 0; px < w; px++) {
+            const size_t p = (size_t)py * w + px;
+            const double* ip = S + p * 4;
+            double sigma[3], dm[3];
+            for (int c = 0; c < 3; c++) {
+                sigma[c] = std::sqrt(va[c] + vb[c] * ip[0]);
+                dm[c] = 1.0 / (thr[c] * sigma[c]);
+            }
+            // denominators of this pixel (recomputed)
+            double DY = 0, DC = 0;
+            for (int y = -R; y <= R; y++)
+                for (int x = -R; x <= R; x++) {
+                    const size_t q = (size_t)clampi(py + y, 0, h - 1) * w + clampi(px + x, 0, w - 1);
+                    double d2 = 0;
+                    for (int c = 0; c < 8; c++) d2 += (F[q * 8 + c] - F[p * 8 + c]) * (F[q * 8 + c] - F[p * 8 + c]);
+                    const double wl = std::exp(-2.0 * d2 * dm[0] * dm[0]);
+                    const double db = (S[q * 4 + 1] - ip[1]) * dm[1] / cB, dr = (S[q * 4 + 2] - ip[2]) * dm[2] / cB;
+                    const double wc = std::exp(-2.0 * (wl * wl + db * db + dr * dr));
+                    const double wd = std::exp(-2.0 * 0.01 * (x * x + y * y));
+                    DY += wd * wl;
+                    DC += wd * wc;
+                }
+            const double* o = D + p * 4;
+            const double g0 = gD[p * 4], g1 = gD[p * 4 + 1], g2 = gD[p * 4 + 2];
+            double gdm[3] = {0, 0, 0};
+            for (int y = -R; y <= R; y++)
+                for (int x = -R; x <= R; x++) {
+                    const size_t q = (size_t)clampi(py + y, 0, h - 1) * w + clampi(px + x, 0, w - 1);
+                    const double* sq = S + q * 4;
+                    double d2 = 0;
+                    for (int c = 0; c < 8; c++) d2 += (F[q * 8 + c] - F[p * 8 + c]) * (F[q * 8 + c] - F[p * 8 + c]);
+                    const double wl = std::exp(-2.0 * d2 * dm[0] * dm[0]);
+                    const double Db = sq[1] - ip[1], Dr = sq[2] - ip[2];
+                    const double kb = dm[1] * dm[1] / (cB * cB), kr = dm[2] * dm[2] / (cB * cB);
+                    const double v = Db * Db * kb + Dr * Dr * kr;
+                    const double wd = std::exp(-2.0 * 0.01 * (x * x + y * y));
+                    const double WY = wd * wl, WC = wd * std::exp(-2.0 * (wl * wl + v));
+                    // output = sum W value / sum W
+                    const double gWY = g0 * (sq[0] - o[0]) / DY;
+                    const double gWC = (g1 * (sq[1] - o[1]) + g2 * (sq[2] - o[2])) / DC;
+                    gS[q * 4 + 0] += g0 * WY / DY;
+                    gS[q * 4 + 1] += g1 * WC / DC;
+                    gS[q * 4 + 2] += g2 * WC / DC;
+                    const double gwl = gWY * wd + gWC * WC * (-4.0 * wl);
+                    const double gv = gWC * WC * (-2.0);
+                    const double gu = gwl * (-2.0 * wl);  // wl = exp(-2 u), u = d2 dm0^2
+                    const double gd2 = gu * dm[0] * dm[0];
+                    gdm[0] += gu * 2.0 * d2 * dm[0];
+                    const double gDb = gv * 2.0 * Db * kb, gDr = gv * 2.0 * Dr * kr;
+                    gdm[1] += gv * 2.0 * Db * Db * dm[1] / (cB * cB);
+                    gdm[2] += gv * 2.0 * Dr * Dr * dm[2] / (cB * cB);
+                    gS[q * 4 + 1] += gDb;
+                    gS[p * 4 + 1] -= gDb;
+                    gS[q * 4 + 2] += gDr;
+                    gS[p * 4 + 2] -= gDr;
+                    for (int c = 0; c < 8; c++) {
+                        const double t = 2.0 * gd2 * (F[q * 8 + c] - F[p * 8 + c]);
+                        gF[q * 8 + c] += t;
+                        gF[p * 8 + c] -= t;
+                    }
+                }
+            for (int c = 0; c < 3; c++) {
+                gthr[c] += gdm[c] * (-dm[c] / thr[c]);
+                const double gsig = gdm[c] * (-dm[c] / sigma[c]);
+                gS[p * 4] += gsig * vb[c] / (2.0 * sigma[c]);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" double oracle_pyramid_loss_grad_f64(const double* image, const double* clean, int w, int h, int levels,
+                                               int radius, const double* thr, const double* chroma_boost,
+                                               const double* nlf, double* grad_thr, double* denoised0) {
+    std::vector<LevelState> Ls(levels);
+    for (int i = 0; i < levels; i++) {
+        Ls[i].w = w >> i;
+        Ls[i].h = h >> i;
+        Ls[i].n = (int)(((long)Ls[i].w * Ls[i].h) / 64);
+        if (Ls[i].n < 2) return NAN;
+    }
+    Ls[0].X.assign(image, image + (size_t)w * h * 4);
+    for (int i = 1; i < levels; i++) {
+        Ls[i].X.resize((size_t)Ls[i].w * Ls[i].h * 4);
+        downsample(Ls[i - 1].X.data(), Ls[i - 1].w, Ls[i - 1].h, 4, Ls[i].X.data(), Ls[i].w, Ls[i].h);
+    }
+    for (int i = levels - 1; i >= 0; i--) {
+        LevelState& L = Ls[i];
+        const size_t np = (size_t)L.w * L.h;
+        if (i == levels - 1) {
+            L.S = L.X;
+        } else {
+            L.S.resize(np * 4);
+            subtract_noise(L.X.data(), L.w, L.h, Ls[i + 1].X.data(), Ls[i + 1].D.data(), Ls[i + 1].w, Ls[i + 1].h,
+                           (const double*)nullptr, 1.0, 1.0, 0.0, 0.0, L.S.data());
+        }
+        std::vector<double> patches(np / 64 * 25 + 25);
+        collect_patches(L.S.data(), L.w, L.h, patches.data());
+        pca_full(patches.data(), L.n, L.V, L.lam, L.B, L.sgn, L.mu);
+        L.F.resize(np * 8);
+        pca_project(L.S.data(), L.w, L.h, L.B, L.F.data());
+        L.D.resize(np * 4);
+        block_match(L.S.data(), (const double*)nullptr, L.F.data(), L.w, L.h, nlf + i * 6, nlf + i * 6 + 3,
+                    thr + i * 3, chroma_boost[i], 0.0, 1.0, 0.0, radius, L.D.data());
+    }
+    double loss = 0;
+    const size_t np0 = (size_t)w * h;
+    std::vector<double> gD(np0 * 4, 0.0);
+    for (size_t p = 0; p < np0; p++)
+        for (int c = 0; c < 3; c++) {
+            const double d = Ls[0].D[p * 4 + c] - clean[p * 4 + c];
+            loss += 0.5 * d * d;
+            gD[p * 4 + c] = d;
+        }
+    if (denoised0) std::memcpy(denoised0, Ls[0].D.data(), sizeof(double) * np0 * 4);
+    if (!grad_thr) return loss;
+
+    for (int i = 0; i < levels; i++) {
+        LevelState& L = Ls[i];
+        const size_t np = (size_t)L.w * L.h;
+        std::vector<double> gS(np * 4, 0.0), gF(np * 8, 0.0);
+        double gthr[3] = {0, 0, 0};
+        bm_backward(L, thr + i * 3, chroma_boost[i], nlf + i * 6, nlf + i * 6 + 3, radius, gD.data(), gS.data(),
+                    gF.data(), gthr);
+        for (int c = 0; c < 3; c++) grad_thr[i * 3 + c] = gthr[c];
+        if (i == levels - 1) break;
+        // projection F = P B (demosaic.metal:774-796): gB and the patch scatter
+        double gB[200] = {0};
+        for (int py = 0; py < L.h; py++)
+            for (int px = 0; px < L.w; px++) {
+                const size_t p = (size_t)py * L.w + px;
+                int row = 0;
+                for (int j = -2; j <= 2; j++)
+                    for (int ii = -2; ii <= 2; ii++, row++) {
+                        const size_t t = (size_t)clampi(py + j, 0, L.h - 1) * L.w + clampi(px + ii, 0, L.w - 1);
+                        double acc = 0;
+                        for (int c = 0; c < 8; c++) {
+                            gB[row * 8 + c] += L.S[t * 4] * gF[p * 8 + c];
+                            acc += gF[p * 8 + c] * L.B[row * 8 + c];
+                        }
+                        gS[t * 4] += acc;
+                    }
+            }
+        // eigenvector derivative: gC = sym(V M V^T), M_jk = v_j^T g_k / (lambda_k - lambda_j)
+        double M[625] = {0};
+        for (int k = 0; k < 8; k++)
+            for (int j = 0; j < 25; j++) {
+                if (j == k || L.lam[k] == L.lam[j]) continue;
+                double vg = 0;
+                for (int r = 0; r < 25; r++) vg += L.V[r * 25 + j] * L.sgn[k] * gB[r * 8 + k];
+                M[j * 25 + k] = vg / (L.lam[k] - L.lam[j]);
+            }
+        double G[625] = {0}, gC[625];
+        for (int a = 0; a < 25; a++)
+            for (int b = 0; b < 25; b++) {
+                double s = 0;
+                for (int j = 0; j < 25; j++)
+                    for (int k = 0; k < 8; k++) s += L.V[a * 25 + j] * M[j * 25 + k] * L.V[b * 25 + k];
+                G[a * 25 + b] = s;
+            }
+        for (int a = 0; a < 25; a++)
+            for (int b = 0; b < 25; b++) gC[a * 25 + b] = 0.5 * (G[a * 25 + b] + G[b * 25 + a]);
+        // covariance -> sampled patches (PCA.cpp:31-32): dL/dx_k = 2 gC (x_k - mu) / (n - 1)
+        for (int y = 0; y < L.h / 8; y++)
+            for (int x = 0; x < L.w / 8; x++) {
+                const long slot = (long)y * L.w / 8 + x;
+                if (slot >= L.n) continue;
+                double xk[25];
+                size_t idx[25];
+                int e = 0;
+                for (int j = -2; j <= 2; j++)
+                    for (int ii = -2; ii <= 2; ii++, e++) {
+                        idx[e] = (size_t)clampi(8 * y + j, 0, L.h - 1) * L.w + clampi(8 * x + ii, 0, L.w - 1);
+                        xk[e] = L.S[idx[e] * 4] - L.mu[e];
+                    }
+                for (int a = 0; a < 25; a++) {
+                    double s = 0;
+                    for (int b = 0; b < 25; b++) s += gC[a * 25 + b] * xk[b];
+                    gS[idx[a] * 4] += 2.0 * s / (L.n - 1);
+                }
+            }
+        // subtract (demosaic.metal:922-959) with luma_weight 1, sharpening 1:
+        // S = max-clamped(X - up(X1) + up(D1)) -> gD1 = up^T(gS masked)
+        LevelState& C = Ls[i + 1];
+        std::vector<double> gD1((size_t)C.w * C.h * 4, 0.0);
+        for (int oy = 0; oy < L.h; oy++) {
+            int y0, y1;
+            double fy;
+            lin_taps<double>((long)(2 * oy + 1) * C.h, 2L * L.h, C.h, &y0, &y1, &fy);
+            for (int ox = 0; ox < L.w; ox++) {
+                int x0, x1;
+                double fx;
+                lin_taps<double>((long)(2 * ox + 1) * C.w, 2L * L.w, C.w, &x0, &x1, &fx);
+                const size_t o = (size_t)oy * L.w + ox;
+                for (int c = 0; c < 3; c++) {
+                    double g = gS[o * 4 + c];
+                    if (c == 0 && !(L.S[o * 4] > 0.0)) g = 0.0;
+                    gD1[((size_t)y0 * C.w + x0) * 4 + c] += (1 - fx) * (1 - fy) * g;
+                    gD1[((size_t)y0 * C.w + x1) * 4 + c] += fx * (1 - fy) * g;
+                    gD1[((size_t)y1 * C.w + x0) * 4 + c] += (1 - fx) * fy * g;
+                    gD1[((size_t)y1 * C.w + x1) * 4 + c] += fx * fy * g;
+                }
+            }
+        }
+        gD.swap(gD1);
+    }
+    return loss;
+}

@@ -121,4 +121,17 @@ int oracle_pyramid_denoise_f64(const double* image, const double* gradient, int 
                                double lensShading, int calibrate, double* denoised, double* subtracted,
                                double* bases, double* eigvals);
 
+// Differentiation form, FP64: the whole multi-level denoise with independent
+// per-level, per-channel thresholds thr[levels*3] (Y, Cb, Cr; the reference
+// ties Cb and Cr to DenoiseParameters::chroma), chromaBoost per level, no
+// gradient image, gradientBoost 0, sharpening 1, no lens shading (the
+// differentiation workload's setting).  Loss = 1/2 sum over pixels and Y, Cb,
+// Cr of (denoised0 - clean)^2.  Returns the loss; writes dL/dthr (nullable)
+// and denoised level 0 (nullable, w*h*4).  The backward pass differentiates
+// every stage, the PCA basis included (eigenvector derivative
+// dv_k = sum_{j!=k} v_j (v_j^T dC v_k) / (lambda_k - lambda_j)), by scatter.
+double oracle_pyramid_loss_grad_f64(const double* image, const double* clean, int w, int h, int levels, int radius,
+                                    const double* thr, const double* chroma_boost, const double* nlf,
+                                    double* grad_thr, double* denoised0);
+
 }  // extern "C"

@@ -474,3 +474,36 @@ def test_pyramid_bilateral_mode_runs_denoise_image():
     lvl2_in = oracle.downsample(oracle.downsample(noisy))
     ref = oracle.denoise_image(lvl2_in, nlf_for(3)[2, 0], nlf_for(3)[2, 1], [1, 1, 1])
     np.testing.assert_allclose(res.denoised[2], ref, atol=1e-6)
+
+
+# ---------------- differentiation form (FP64 adjoint) ----------------
+
+def test_adjoint_matches_central_finite_differences(oracle_threads):
+    """dL/dthr of the FP64 adjoint (eigenvector derivative included) against
+    central differences (eps = 1e-3) of the FP64 forward."""
+    noisy, clean = noisy_ycc(256, 256, seed=5, a=5e-3, b=2e-3, dtype=np.float64)
+    levels, radius = 4, 2
+    thr = np.array([[4.0, 2.0, 2.5], [3.0, 2.0, 2.0], [2.5, 1.5, 2.0], [2.0, 1.5, 1.5]])
+    cb = np.array([2.0, 1.5, 1.0, 1.0])
+    nlf = nlf_for(levels, 5e-3, 2e-3)
+    loss, g, den = oracle.pyramid_loss_grad(noisy, clean, thr, cb, nlf, levels=levels, radius=radius)
+    eps = 1e-3
+    fd = np.zeros_like(g)
+    for i in range(levels):
+        for c in range(3):
+            tp, tm = thr.copy(), thr.copy()
+            tp[i, c] += eps
+            tm[i, c] -= eps
+            lp = oracle.pyramid_loss_grad(noisy, clean, tp, cb, nlf, levels=levels, radius=radius, want_grad=False)[0]
+            lm = oracle.pyramid_loss_grad(noisy, clean, tm, cb, nlf, levels=levels, radius=radius, want_grad=False)[0]
+            fd[i, c] = (lp - lm) / (2 * eps)
+    scale = np.abs(fd).max()
+    np.testing.assert_allclose(g, fd, rtol=1e-4, atol=1e-6 * scale)
+    # the forward of the differentiable form is the ordinary pyramid when Cb and Cr thresholds agree
+    thr2 = thr.copy()
+    thr2[:, 2] = thr2[:, 1]
+    _, _, den2 = oracle.pyramid_loss_grad(noisy, clean, thr2, cb, nlf, levels=levels, radius=radius, want_grad=False)
+    prm = [dict(luma=thr2[i, 0], chroma=thr2[i, 1], chromaBoost=cb[i], gradientBoost=0.0, gradientThreshold=1.0,
+                sharpening=1.0) for i in range(levels)]
+    ref = oracle.pyramid_denoise(noisy, prm, nlf, levels=levels, radius=radius, dtype=np.float64)
+    np.testing.assert_allclose(den2, ref.denoised[0], atol=1e-12)
