@@ -27,7 +27,8 @@ STAGES = ("downsample", "nlf", "subtract", "pca", "block_match")
 EXPORTED = (
     "gcam_abi_version", "gcam_status_string", "gcam_last_error",
     "gcam_pyramid_create", "gcam_pyramid_destroy", "gcam_pyramid_denoise", "gcam_pyramid_level",
-    "gcam_pyramid_basis", "gcam_pyramid_set_profiling", "gcam_pyramid_stage_times", "gcam_downsample_image", "gcam_subtract_noise_image", "gcam_collect_patches",
+    "gcam_pyramid_basis", "gcam_pyramid_set_profiling", "gcam_pyramid_stage_times",
+    "gcam_pyramid_denoise_fwd", "gcam_pyramid_denoise_bwd", "gcam_downsample_image", "gcam_subtract_noise_image", "gcam_collect_patches",
     "gcam_build_pca_space", "gcam_pca_projection", "gcam_block_matching_denoise_image",
     "gcam_block_matching_denoise_image_lp", "gcam_denoise_image",
     "gcam_basic_noise_statistics", "gcam_measure_ycbcr_nlf", "gcam_synth_frame",
@@ -88,6 +89,9 @@ def _declare(L: ctypes.CDLL) -> None:
                                        _P, _IP]
     L.gcam_pyramid_level.argtypes = [_P, _I, _I, _IP]
     L.gcam_pyramid_basis.argtypes = [_P, _I, _P, _P]
+    L.gcam_pyramid_denoise_fwd.argtypes = [_P, ctypes.POINTER(DenoiseParamsC), _P, _IP, ctypes.POINTER(NlfC), _P,
+                                           _IP]
+    L.gcam_pyramid_denoise_bwd.argtypes = [_P, _IP, _P, _P]
     L.gcam_pyramid_set_profiling.argtypes = [_P, _I]
     L.gcam_pyramid_stage_times.argtypes = [_P, _P]
     L.gcam_downsample_image.argtypes = [_IP, _IP, _P]

@@ -41,6 +41,21 @@ struct gcam_pyramid {
     float* basis = nullptr;    // levels x 25 x 8
     double* eigvals = nullptr; // levels x 25
     int feature_level = -1;
+    // differentiation form (allocated by the first gcam_pyramid_denoise_fwd)
+    char* adj_arena = nullptr;
+    double* eig_full = nullptr;       // levels x kEigStride
+    float4* adj_coef = nullptr;       // 3 float4 per level-0 pixel
+    float4* adj_gS = nullptr;         // level-0 sized
+    float4* adj_gF = nullptr;         // 2 float4 per level-0 pixel
+    float4* adj_gD[2] = {};           // ping-pong dL/d(denoised), level-1 sized
+    float* adj_gthr_part = nullptr;
+    float* adj_gthr = nullptr;        // levels x 3
+    double* adj_gB_part = nullptr;
+    double* adj_gC = nullptr;
+    bool fwd_ready = false;
+    gcam_image fwd_image = {};
+    float fwd_thr[GCAM_MAX_LEVELS][3] = {};
+    float fwd_cb[GCAM_MAX_LEVELS] = {};
     bool profiling = false;
     bool ran[GCAM_MAX_LEVELS][GCAM_STAGE_COUNT] = {};
     hipEvent_t ev[GCAM_MAX_LEVELS][GCAM_STAGE_COUNT][2] = {};
@@ -186,13 +201,17 @@ void gcam_pyramid_destroy(gcam_pyramid* p) {
                 if (e) (void)hipEventDestroy(e);
     if (p->arena) (void)hipFree(p->arena);
     if (p->grad_arena) (void)hipFree(p->grad_arena);
+    if (p->adj_arena) (void)hipFree(p->adj_arena);
     delete p;
 }
 
-gcam_status gcam_pyramid_denoise(gcam_pyramid* p, const gcam_denoise_params* params, const gcam_image* image,
-                                 const gcam_image* gradient, gcam_ycbcr_nlf* nlf, float exposure_multiplier,
-                                 float lens_shading_correction, int32_t calibrate_from_image, void* stream,
-                                 gcam_image* out) {
+// The orchestration behind gcam_pyramid_denoise and gcam_pyramid_denoise_fwd.
+// thr (nullable): per-level (Y, Cb, Cr) thresholds replacing (luma, chroma,
+// chroma); keep_eig: keep each level's full eigen-decomposition.
+static gcam_status denoise_impl(gcam_pyramid* p, const gcam_denoise_params* params, const float (*thr)[3],
+                                const gcam_image* image, const gcam_image* gradient, gcam_ycbcr_nlf* nlf,
+                                float exposure_multiplier, float lens_shading_correction, int32_t calibrate_from_image,
+                                void* stream, gcam_image* out, bool keep_eig) {
     if (!p) return set_error(GCAM_ERR_INVALID_ARGUMENT, "pyramid is NULL");
     if (!params) return set_error(GCAM_ERR_INVALID_ARGUMENT, "params is NULL");
     if (!nlf) return set_error(GCAM_ERR_INVALID_ARGUMENT, "nlf is NULL");
@@ -207,7 +226,10 @@ gcam_status gcam_pyramid_denoise(gcam_pyramid* p, const gcam_denoise_params* par
         if (!(q.luma > 0.0f) || !(q.chroma > 0.0f) || !(q.chromaBoost > 0.0f))
             return set_error(GCAM_ERR_INVALID_ARGUMENT,
                              "level " + std::to_string(i) + ": luma, chroma and chromaBoost must be > 0");
+        if (thr && !(thr[i][0] > 0.0f && thr[i][1] > 0.0f && thr[i][2] > 0.0f))
+            return set_error(GCAM_ERR_INVALID_ARGUMENT, "level " + std::to_string(i) + ": thresholds must be > 0");
     }
+    p->fwd_ready = false;
     DeviceGuard guard(p->device);
     hipStream_t s = as_stream(stream);
     const int L = p->levels;
@@ -286,9 +308,9 @@ gcam_status gcam_pyramid_denoise(gcam_pyramid* p, const gcam_denoise_params* par
             GCAM_MARK(i, GCAM_STAGE_SUBTRACT, 1);
         }
         BmParams bp;
-        bp.thr[0] = params[i].luma;  // nflMultiplier (:52-56)
-        bp.thr[1] = params[i].chroma;
-        bp.thr[2] = params[i].chroma;
+        bp.thr[0] = thr ? thr[i][0] : params[i].luma;  // nflMultiplier (:52-56)
+        bp.thr[1] = thr ? thr[i][1] : params[i].chroma;
+        bp.thr[2] = thr ? thr[i][2] : params[i].chroma;
         bp.chromaBoost = params[i].chromaBoost;
         bp.gradientBoost = params[i].gradientBoost;
         bp.gradientThreshold = params[i].gradientThreshold;
@@ -302,9 +324,11 @@ gcam_status gcam_pyramid_denoise(gcam_pyramid* p, const gcam_denoise_params* par
         }
         GCAM_MARK(i, GCAM_STAGE_PCA, 0);
         const int64_t n = ((int64_t)p->W[i] * p->H[i]) / 64;  // :142
-        st = launch_image_moments(layer, p->moment_partials, s);
+        int nblocks = 0;
+        st = launch_image_moments(layer, p->moment_partials, &nblocks, s);
         if (st != GCAM_OK) return st;
-        st = launch_pca_solve(p->moment_partials, n, p->basis + 200 * i, p->eigvals + 25 * i, s);
+        st = launch_pca_solve(p->moment_partials, nblocks, n, p->basis + 200 * i, p->eigvals + 25 * i,
+                              keep_eig ? p->eig_full + kEigStride * i : nullptr, s);
         if (st != GCAM_OK) return st;
         const View<float> fv = view<float>(p->feat, p->W[i], p->H[i]);
         if (p->precision == GCAM_PRECISION_FP32) {
@@ -336,6 +360,118 @@ gcam_status gcam_pyramid_denoise(gcam_pyramid* p, const gcam_denoise_params* par
             }
     }
     if (out) *out = make_image(p->den[0], p->W[0], p->H[0], GCAM_PIXEL_RGBA32F);
+    return GCAM_OK;
+}
+
+gcam_status gcam_pyramid_denoise(gcam_pyramid* p, const gcam_denoise_params* params, const gcam_image* image,
+                                 const gcam_image* gradient, gcam_ycbcr_nlf* nlf, float exposure_multiplier,
+                                 float lens_shading_correction, int32_t calibrate_from_image, void* stream,
+                                 gcam_image* out) {
+    return denoise_impl(p, params, nullptr, image, gradient, nlf, exposure_multiplier, lens_shading_correction,
+                        calibrate_from_image, stream, out, false);
+}
+
+static gcam_status alloc_adjoint(gcam_pyramid* p) {
+    if (p->adj_arena) return GCAM_OK;
+    const size_t n0 = (size_t)p->W[0] * p->H[0];
+    const size_t n1 = p->levels > 1 ? (size_t)p->W[1] * p->H[1] : 1;
+    std::vector<std::pair<void**, size_t>> parts = {
+        {(void**)&p->eig_full, sizeof(double) * kEigStride * p->levels},
+        {(void**)&p->adj_coef, sizeof(float4) * 3 * n0},
+        {(void**)&p->adj_gS, sizeof(float4) * n0},
+        {(void**)&p->adj_gF, sizeof(float4) * 2 * n0},
+        {(void**)&p->adj_gD[0], sizeof(float4) * n1},
+        {(void**)&p->adj_gD[1], sizeof(float4) * n1},
+        {(void**)&p->adj_gthr_part, sizeof(float) * 3 * adjoint_thr_blocks(p->W[0], p->H[0])},
+        {(void**)&p->adj_gthr, sizeof(float) * 3 * p->levels},
+        {(void**)&p->adj_gB_part, sizeof(double) * kMomentBlocks * kPatch * kComponents},
+        {(void**)&p->adj_gC, sizeof(double) * 625},
+    };
+    size_t total = 0;
+    for (auto& pr : parts) total += align256(pr.second);
+    GCAM_HIP_TRY(hipMalloc((void**)&p->adj_arena, total));
+    size_t off = 0;
+    for (auto& pr : parts) {
+        *pr.first = p->adj_arena + off;
+        off += align256(pr.second);
+    }
+    return GCAM_OK;
+}
+
+gcam_status gcam_pyramid_denoise_fwd(gcam_pyramid* p, const gcam_denoise_params* params, const float* thresholds,
+                                     const gcam_image* image, const gcam_ycbcr_nlf* nlf, void* stream,
+                                     gcam_image* out) {
+    if (!p || !params || !thresholds || !nlf) return set_error(GCAM_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (p->precision != GCAM_PRECISION_FP32 || p->bilateral)
+        return set_error(GCAM_ERR_UNSUPPORTED, "the differentiable form runs the FP32 PCA path");
+    for (int i = 0; i < p->levels; i++)
+        if (params[i].gradientBoost != 0.0f || params[i].sharpening > 1.0f)
+            return set_error(GCAM_ERR_UNSUPPORTED,
+                             "the differentiable form needs gradientBoost == 0 and sharpening <= 1 on every level");
+    DeviceGuard guard(p->device);
+    gcam_status st = alloc_adjoint(p);
+    if (st != GCAM_OK) return st;
+    gcam_ycbcr_nlf nl[GCAM_MAX_LEVELS];
+    std::memcpy(nl, nlf, sizeof(gcam_ycbcr_nlf) * p->levels);
+    const float(*thr)[3] = reinterpret_cast<const float(*)[3]>(thresholds);
+    st = denoise_impl(p, params, thr, image, nullptr, nl, 1.0f, 0.0f, 0, stream, out, true);
+    if (st != GCAM_OK) return st;
+    for (int i = 0; i < p->levels; i++) {
+        for (int c = 0; c < 3; c++) p->fwd_thr[i][c] = thr[i][c];
+        p->fwd_cb[i] = params[i].chromaBoost;
+    }
+    p->fwd_image = *image;
+    p->fwd_ready = true;
+    return GCAM_OK;
+}
+
+gcam_status gcam_pyramid_denoise_bwd(gcam_pyramid* p, const gcam_image* grad_output, float* grad_thresholds,
+                                     void* stream) {
+    if (!p || !grad_thresholds) return set_error(GCAM_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!p->fwd_ready) return set_error(GCAM_ERR_INVALID_ARGUMENT, "no gcam_pyramid_denoise_fwd to differentiate");
+    gcam_status st = check_image(grad_output, GCAM_PIXEL_RGBA32F, "grad_output", p->width, p->height);
+    if (st != GCAM_OK) return st;
+    DeviceGuard guard(p->device);
+    hipStream_t s = as_stream(stream);
+    const int L = p->levels;
+    for (int i = 0; i < L; i++) {
+        const int w = p->W[i], h = p->H[i];
+        View<const float4> S;
+        if (i < L - 1)
+            S = view<const float4>(p->sub[i], w, h);
+        else if (i > 0)
+            S = view<const float4>(p->layer[i], w, h);
+        else
+            S = view<const float4>(p->fwd_image);
+        st = launch_pca_project(S, p->basis + 200 * i, view<float>(p->feat, w, h), s);  // features recomputed
+        if (st != GCAM_OK) return st;
+        AdjointLevel a;
+        a.S = S;
+        a.D = view<const float4>(p->den[i], w, h);
+        a.gD = i == 0 ? view<const float4>(*grad_output) : view<const float4>(p->adj_gD[(i - 1) % 2], w, h);
+        a.feat = p->feat;
+        a.basis = p->basis + 200 * i;
+        a.eig = p->eig_full + kEigStride * i;
+        a.nlf = p->nlf_dev + i;
+        for (int c = 0; c < 3; c++) a.thr[c] = p->fwd_thr[i][c];
+        a.chromaBoost = p->fwd_cb[i];
+        a.radius = p->radius;
+        a.n = (double)(((int64_t)w * h) / 64);
+        a.coef = p->adj_coef;
+        a.gS = p->adj_gS;
+        a.gF = p->adj_gF;
+        a.gthr_part = p->adj_gthr_part;
+        a.gthr = p->adj_gthr + 3 * i;
+        a.gB_part = p->adj_gB_part;
+        a.gC = p->adj_gC;
+        a.gD1 = i < L - 1 ? p->adj_gD[i % 2] : nullptr;
+        a.w1 = i < L - 1 ? p->W[i + 1] : 0;
+        a.h1 = i < L - 1 ? p->H[i + 1] : 0;
+        st = launch_level_backward(a, s);
+        if (st != GCAM_OK) return st;
+    }
+    GCAM_HIP_TRY(hipMemcpyAsync(grad_thresholds, p->adj_gthr, sizeof(float) * 3 * L, hipMemcpyDeviceToHost, s));
+    GCAM_HIP_TRY(hipStreamSynchronize(s));
     return GCAM_OK;
 }
 
@@ -455,8 +591,9 @@ gcam_status gcam_build_pca_space(const float* patches, int64_t n, float* basis, 
     hipStream_t s = as_stream(stream);
     double* partials = nullptr;
     GCAM_HIP_TRY(hipMallocAsync((void**)&partials, sizeof(double) * kMomentBlocks * kMoments, s));
-    gcam_status st = launch_buffer_moments(patches, n, partials, s);
-    if (st == GCAM_OK) st = launch_pca_solve(partials, n, basis, eigenvalues, s);
+    int nblocks = 0;
+    gcam_status st = launch_buffer_moments(patches, n, partials, &nblocks, s);
+    if (st == GCAM_OK) st = launch_pca_solve(partials, nblocks, n, basis, eigenvalues, nullptr, s);
     GCAM_HIP_TRY(hipFreeAsync(partials, s));
     return st;
 }

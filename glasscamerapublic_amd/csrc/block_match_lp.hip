@@ -11,10 +11,11 @@
 //    the tile's first pixel and scaled by the level scale s (~1/sigma of the
 //    level).  Feature distances are invariant to the centring; the scale
 //    divides the luma coefficient by s^2.
-// 2. Projection F = P B on MFMA: v_mfma_f32_16x16x32_{f16,bf16}, 16 pixels x
-//    (25 patch values padded to K = 32) x (8 components padded to N = 16),
-//    FP32 accumulation.  Each luma value enters as a 16-bit high part plus a
-//    16-bit remainder (two MFMAs), so the projection input keeps ~22 bits.
+// 2. Projection on MFMA, transposed (F^T = B^T P^T): v_mfma_f32_16x16x32_{f16,
+//    bf16}, (8 components padded to 16) x (25 patch values padded to K = 32)
+//    x 16 pixels, FP32 accumulation, so each lane ends with 4 consecutive
+//    components of one pixel.  With BF16 (8-bit mantissa) each luma value
+//    enters as a high part plus its rounding remainder (two MFMAs).
 // 3. Component 0 (the largest-variance one, in practice the patch mean) spans
 //    the widest range across a tile, so its rounding error would dominate the
 //    distances: it stays FP32.  Components 1..7 are rounded to FP16/BF16 and
@@ -25,6 +26,7 @@
 #include "gcam_internal.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace gcam {
 
@@ -119,55 +121,63 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
         const float4 v = in.at(clampi(x0 - R + fx, 0, in.w - 1), clampi(y0 - R + fy, 0, in.h - 1));
         ycn[e] = make_float4(v.x, v.y, v.z, 0.0f);
     }
-    // B operand of the projection (constant): lane holds basis[k][j] for
-    // j = lane % 16, k = 8 (lane / 16) + v; zero outside 25 x 8.
-    h8 bop;
+    // A operand of the projection (constant): the transposed basis, lane holds
+    // basis[k][i] for component i = lane % 16, k = 8 (lane / 16) + v; zero
+    // outside 25 x 8.
+    h8 aop;
     {
-        const int j = lane & 15, kg = lane >> 4;
+        const int i = lane & 15, kg = lane >> 4;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
             const int k = 8 * kg + v;
-            bop[v] = (H)((k < kPatch && j < kComponents) ? basis[k * kComponents + j] : 0.0f);
+            aop[v] = (H)((k < kPatch && i < kComponents) ? basis[k * kComponents + i] : 0.0f);
         }
     }
     __syncthreads();
 
-    // Projection, 16 feature-region pixels per MFMA.  A operand: lane holds
-    // patch values k = 8 (lane / 16) + v of pixel lane % 16 of the group.
+    // Projection C[component][pixel], 16 feature-region pixels per MFMA.  B
+    // operand: lane holds patch values k = 8 (lane / 16) + v of pixel
+    // lane % 16 of the group; the patch is centred on the pixel's clamped
+    // image position (clamp-to-edge, like the FP32 projection).  BF16 adds the
+    // rounding remainder of each luma value as a second product.
+    constexpr bool kSplit = !std::is_same<H, _Float16>::value;
     for (int g = wave; g < T::GROUPS; g += T::WAVES) {
-        const int i = lane & 15, kg = lane >> 4;
-        const int e = min(g * 16 + i, T::FN - 1);
-        const int fy = e / T::FW, fx = e % T::FW;
-        h8 ahi, alo;
+        const int j = lane & 15, kg = lane >> 4;
+        const int e = min(g * 16 + j, T::FN - 1);
+        const int cx = clampi(x0 - R + e % T::FW, 0, in.w - 1) - (x0 - R);
+        const int cy = clampi(y0 - R + e / T::FW, 0, in.h - 1) - (y0 - R);
+        h8 bhi, blo;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
             const int k = 8 * kg + v;
-            const float yv = k < kPatch ? lum[(fy + k / 5) * T::LW + fx + k % 5] : 0.0f;
+            const float yv = k < kPatch ? lum[(cy + k / 5) * T::LW + cx + k % 5] : 0.0f;
             const H hi = (H)yv;
-            ahi[v] = hi;
-            alo[v] = (H)(yv - (float)hi);
+            bhi[v] = hi;
+            if (kSplit) blo[v] = (H)(yv - (float)hi);
         }
         f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-        c = P::mfma(ahi, bop, c);
-        c = P::mfma(alo, bop, c);
-        // C: lane holds pixel 4 (lane / 16) + r of the group, component lane % 16.
-        const int j = lane & 15;
+        c = P::mfma(aop, bhi, c);
+        if (kSplit) c = P::mfma(aop, blo, c);
+        // C: lane holds components 4 (lane / 16) + r of pixel lane % 16.  Lanes
+        // 0-15 keep component 0 in FP32 and write halves 0-3 = (0, c1, c2, c3);
+        // lanes 16-31 write halves 4-7 = (c4 .. c7).
+        typedef H h4 __attribute__((ext_vector_type(4)));
+        h4 hv;
+        float n = 0.0f;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const H hq = (H)c[r];
+            const H hq = (kg == 0 && r == 0) ? (H)0.0f : (H)c[r];
+            hv[r] = hq;
             const float fq = (float)hq;
-            float n = (j >= 1 && j < kComponents) ? fq * fq : 0.0f;
-            n += __shfl_xor(n, 1);
-            n += __shfl_xor(n, 2);
-            n += __shfl_xor(n, 4);
-            const int er = g * 16 + 4 * kg + r;
-            if (er < T::FN) {
-                if (j >= 1 && j < kComponents) reinterpret_cast<H*>(&fh[er])[j - 1] = hq;
-                if (j == kComponents) reinterpret_cast<H*>(&fh[er])[kComponents - 1] = (H)0.0f;
-                if (j == 0) {
-                    f0s[er] = c[r];
-                    ycn[er].w = n;
-                }
+            n = fmaf(fq, fq, n);
+        }
+        n += __shfl_xor(n, 16);
+        const int er = g * 16 + j;
+        if (er < T::FN && kg < 2) {
+            reinterpret_cast<h4*>(&fh[er])[kg] = hv;
+            if (kg == 0) {
+                f0s[er] = c[0];
+                ycn[er].w = n;
             }
         }
     }

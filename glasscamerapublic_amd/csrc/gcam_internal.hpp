@@ -88,12 +88,17 @@ gcam_status launch_nlf_fit(View<const float4> stats, float exposure_multiplier, 
 gcam_status launch_set_nlf(const gcam_ycbcr_nlf* host_levels, int levels, NlfDev* dev, hipStream_t s);
 
 gcam_status launch_collect_patches(View<const float4> in, float* patches, int64_t n, hipStream_t s);
-// Moments of the patches sampled from an image (never materialized).
-gcam_status launch_image_moments(View<const float4> in, double* partials, hipStream_t s);
+// Jacobi stops when the off-diagonal energy is below this fraction of the
+// total (entries ~1e-13 of the matrix norm: above FP64 rounding noise).
+constexpr double kJacobiTol = 1e-26;
+// Moments of the patches sampled from an image (never materialized); *nblocks
+// receives the number of per-workgroup partials written.
+gcam_status launch_image_moments(View<const float4> in, double* partials, int* nblocks, hipStream_t s);
 // Moments of a device patch buffer (n x 25).
-gcam_status launch_buffer_moments(const float* patches, int64_t n, double* partials, hipStream_t s);
+gcam_status launch_buffer_moments(const float* patches, int64_t n, double* partials, int* nblocks, hipStream_t s);
 // Reduce the partials, form the covariance, eigensolve, write basis (25 x 8 f32) and eigenvalues.
-gcam_status launch_pca_solve(const double* partials, int64_t n, float* basis, double* eigvals, hipStream_t s);
+gcam_status launch_pca_solve(const double* partials, int nblocks, int64_t n, float* basis, double* eigvals,
+                             double* eig_full, hipStream_t s);
 gcam_status launch_pca_project(View<const float4> in, const float* basis, View<float> feat8, hipStream_t s);
 
 gcam_status launch_block_match(View<const float4> in, View<const float2> grad, View<const float> feat8,
@@ -111,5 +116,34 @@ gcam_status launch_denoise_image(View<const float4> in, View<const float2> grad,
 
 gcam_status launch_synth(int w, int h, const gcam_synth_params& p, uint16_t* raw, View<float4> ycc,
                          View<float4> clean, hipStream_t s);
+
+// ---- reverse pass (differentiation form) ----
+// Per level, the full eigen-decomposition kept by the forward: V (25 x 25,
+// column = eigenvector, descending) at 0, eigenvalues at 625, the 8 basis
+// orientations at 650, the mean patch at 658.
+constexpr int kEigStride = 704;
+
+struct AdjointLevel {
+    View<const float4> S, D, gD;  // layer, denoised, dL/d(denoised) of this level
+    const float* feat;            // features of S (8 floats per pixel, packed)
+    const float* basis;           // 25 x 8
+    const double* eig;            // kEigStride doubles
+    const NlfDev* nlf;
+    float thr[3], chromaBoost;
+    int radius;
+    double n;                     // sampled patch slots, w*h/64
+    float4* coef;                 // 3 float4 per pixel (scratch)
+    float4* gS;                   // dL/dS (Y, Cb, Cr)
+    float4* gF;                   // dL/dF (2 float4 per pixel)
+    float* gthr_part;             // adjoint_thr_blocks(w, h) x 3
+    float* gthr;                  // 3 floats: dL/dthr of the level
+    double* gB_part;              // kMomentBlocks x 200
+    double* gC;                   // 625
+    float4* gD1;                  // dL/d(denoised) of level i+1, or nullptr at the coarsest level
+    int w1, h1;
+};
+
+int adjoint_thr_blocks(int w, int h);
+gcam_status launch_level_backward(const AdjointLevel& level, hipStream_t s);
 
 }  // namespace gcam

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_fill_zero(float* p, int64_t count) {
 // total.  The 8 largest eigenpairs, largest first (PCA.cpp:49-55), are written
 // as basis[r * 8 + c]; ties in eigenvalue keep index order.
 __global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int nblocks, double n, float* basis,
-                                                   double* eigvals) {
+                                                   double* eigvals, double* eig_full) {
     constexpr int N = kPatch;
     constexpr int LD = N + 1;
     __shared__ double mom[kMoments];
@@ -135,12 +135,17 @@ __global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int n
     __shared__ double cs[12][2];
     __shared__ int pq[12][2];
     __shared__ double red[2][4];
-    __shared__ int rank_of[N];
     const int tid = threadIdx.x;
 
     for (int e = tid; e < kMoments; e += 256) {
         double a = 0.0;
-        for (int b = 0; b < nblocks; b++) a += partials[(size_t)b * kMoments + e];
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // 8 loads in flight; fixed summation order
+        int b = 0;
+        for (; b + 8 <= nblocks; b += 8)
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc[u] += partials[(size_t)(b + u) * kMoments + e];
+        for (; b < nblocks; b++) acc[0] += partials[(size_t)b * kMoments + e];
+        for (int u = 0; u < 8; u++) a += acc[u];
         mom[e] = a;
     }
     __syncthreads();
@@ -178,7 +183,7 @@ __global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int n
         off = red[0][0] + red[0][1] + red[0][2] + red[0][3];
         tot = red[1][0] + red[1][1] + red[1][2] + red[1][3];
         __syncthreads();
-        if (off == 0.0 || off <= 1e-30 * tot) break;
+        if (off == 0.0 || off <= kJacobiTol * tot) break;
 
         for (int round = 0; round < N; round++) {
             if (tid < 12) {
@@ -236,13 +241,21 @@ __global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int n
             const double dj = A[j * LD + j];
             rank += (dj > di || (dj == di && j < tid)) ? 1 : 0;
         }
-        rank_of[tid] = rank;
         if (eigvals) eigvals[rank] = di;
+        double sg = 1.0;
         if (rank < kComponents) {
             double dot = 0.0;
             for (int r = 0; r < N; r++) dot += V[r * LD + tid] * sign_ref(r);
-            const double sg = dot < 0.0 ? -1.0 : 1.0;
+            sg = dot < 0.0 ? -1.0 : 1.0;
             for (int r = 0; r < N; r++) basis[r * kComponents + rank] = (float)(sg * V[r * LD + tid]);
+        }
+        // Full decomposition for the reverse pass: V (column = eigenvector,
+        // descending), eigenvalues, the 8 orientations, the mean patch.
+        if (eig_full) {
+            for (int r = 0; r < N; r++) eig_full[r * N + rank] = V[r * LD + tid];
+            eig_full[625 + rank] = di;
+            if (rank < kComponents) eig_full[650 + rank] = sg;
+            eig_full[658 + tid] = mom[tid] / n;
         }
     }
 }
@@ -292,20 +305,28 @@ gcam_status launch_collect_patches(View<const float4> in, float* patches, int64_
     return GCAM_OK;
 }
 
-gcam_status launch_image_moments(View<const float4> in, double* partials, hipStream_t s) {
-    k_image_moments<<<kMomentBlocks, 256, 0, s>>>(in, partials);
+// Workgroups of the moment reduction: one per 256 patches, at most kMomentBlocks.
+static int moment_blocks(int64_t npatch) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(kMomentBlocks, (npatch + 255) / 256));
+}
+
+gcam_status launch_image_moments(View<const float4> in, double* partials, int* nblocks, hipStream_t s) {
+    *nblocks = moment_blocks((int64_t)(in.w / 8) * (in.h / 8));
+    k_image_moments<<<*nblocks, 256, 0, s>>>(in, partials);
     GCAM_LAUNCH_CHECK("patch moments");
     return GCAM_OK;
 }
 
-gcam_status launch_buffer_moments(const float* patches, int64_t n, double* partials, hipStream_t s) {
-    k_buffer_moments<<<kMomentBlocks, 256, 0, s>>>(patches, n, partials);
+gcam_status launch_buffer_moments(const float* patches, int64_t n, double* partials, int* nblocks, hipStream_t s) {
+    *nblocks = moment_blocks(n);
+    k_buffer_moments<<<*nblocks, 256, 0, s>>>(patches, n, partials);
     GCAM_LAUNCH_CHECK("patch moments");
     return GCAM_OK;
 }
 
-gcam_status launch_pca_solve(const double* partials, int64_t n, float* basis, double* eigvals, hipStream_t s) {
-    k_pca_solve<<<1, 256, 0, s>>>(partials, kMomentBlocks, (double)n, basis, eigvals);
+gcam_status launch_pca_solve(const double* partials, int nblocks, int64_t n, float* basis, double* eigvals,
+                             double* eig_full, hipStream_t s) {
+    k_pca_solve<<<1, 256, 0, s>>>(partials, nblocks, (double)n, basis, eigvals, eig_full);
     GCAM_LAUNCH_CHECK("PCA eigensolve");
     return GCAM_OK;
 }

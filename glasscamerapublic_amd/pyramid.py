@@ -175,6 +175,42 @@ class PyramidProcessor:
     def imagePyramid(self):
         return [self._level(i, _lib.GCAM_LEVEL_NOISY, 4) for i in range(1, self.levels)]
 
+    def _params_nlf(self, denoiseParameters, nlfParameters):
+        P = (DenoiseParamsC * self.levels)()
+        for i, dp in enumerate(denoiseParameters):
+            for f in fields(DenoiseParameters):
+                setattr(P[i], f.name, float(getattr(dp, f.name)))
+        N = (NlfC * self.levels)()
+        for i, (A, B) in enumerate(nlfParameters):
+            for c in range(3):
+                N[i].a[c] = float(A[c])
+                N[i].b[c] = float(B[c])
+        return P, N
+
+    def denoise_fwd(self, denoiseParameters, thresholds, image: torch.Tensor, nlfParameters,
+                    stream=None) -> torch.Tensor:
+        """Differentiable forward: denoise with per-level (Y, Cb, Cr) thresholds (levels x 3).
+
+        Keeps what denoise_bwd needs; `image` must stay alive until then.
+        """
+        P, N = self._params_nlf(denoiseParameters, nlfParameters)
+        th = (ctypes.c_float * (3 * self.levels))(*[float(v) for v in torch.as_tensor(thresholds).reshape(-1)])
+        out = Image()
+        with torch.cuda.device(self.device):
+            check(lib().gcam_pyramid_denoise_fwd(self._h, P, th, ctypes.byref(as_image(image, GCAM_PIXEL_RGBA32F)),
+                                                 N, ctypes.c_void_p(_stream(stream)), ctypes.byref(out)))
+        self._fwd_image = image
+        return _wrap(out, 4, self.device)
+
+    def denoise_bwd(self, grad_output: torch.Tensor, stream=None) -> torch.Tensor:
+        """dL/dthresholds (levels x 3, float32 CPU tensor) of the last denoise_fwd, given dL/d(output)."""
+        g = (ctypes.c_float * (3 * self.levels))()
+        grad_output = grad_output.contiguous()
+        with torch.cuda.device(self.device):
+            check(lib().gcam_pyramid_denoise_bwd(self._h, ctypes.byref(as_image(grad_output, GCAM_PIXEL_RGBA32F)),
+                                                 g, ctypes.c_void_p(_stream(stream))))
+        return torch.tensor(list(g), dtype=torch.float32).reshape(self.levels, 3)
+
     def set_profiling(self, enable: bool = True) -> None:
         """Record HIP events around every stage of each denoise call."""
         check(lib().gcam_pyramid_set_profiling(self._h, int(bool(enable))))

@@ -156,6 +156,29 @@ typedef enum gcam_stage {
 gcam_status gcam_pyramid_set_profiling(gcam_pyramid* pyramid, int32_t enable);
 gcam_status gcam_pyramid_stage_times(const gcam_pyramid* pyramid, float* ms);
 
+/* ---- differentiation form (no counterpart in the reference) ----
+ *
+ * gcam_pyramid_denoise_fwd: PyramidProcessor::denoise
+ * (MetalPipeline/pyramid_processor.cpp:96-172) with independent thresholds per
+ * level and channel, thresholds[level * 3 + c] for c = Y, Cb, Cr (the
+ * reference ties Cb and Cr to DenoiseParameters::chroma).  The handle keeps
+ * what the reverse pass needs (each level's layer, output, and full PCA
+ * eigen-decomposition); `image` must stay valid until the backward call.
+ * Requires an FP32 PCA handle, gradientBoost == 0 and sharpening <= 1 on every
+ * level, no gradient image and no lens shading (the differentiation workload's
+ * setting); other cases return GCAM_ERR_UNSUPPORTED.
+ *
+ * gcam_pyramid_denoise_bwd: reverse pass of the last forward on the same
+ * handle.  grad_output = dL/d(denoised level 0) (RGBA32F device image, .w
+ * ignored); writes dL/dthresholds (levels * 3 floats, host).  Match weights
+ * are recomputed, not stored; the PCA basis is differentiated too.
+ * Synchronizes the stream. */
+gcam_status gcam_pyramid_denoise_fwd(gcam_pyramid* pyramid, const gcam_denoise_params* params,
+                                     const float* thresholds, const gcam_image* image, const gcam_ycbcr_nlf* nlf,
+                                     void* stream, gcam_image* out);
+gcam_status gcam_pyramid_denoise_bwd(gcam_pyramid* pyramid, const gcam_image* grad_output, float* grad_thresholds,
+                                     void* stream);
+
 /* ---- per-operation entry points (MetalPipeline/demosaic_kernels.hpp wrappers) ---- */
 
 /* resampleImageKernel (demosaic_kernels.hpp:470-483) running downsampleImageXYZ

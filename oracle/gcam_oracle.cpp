@@ -248,7 +248,9 @@ void oracle_eigh_jacobi(const double* cov, int n, double* eigvals, double* eigve
                 tot += e;
                 if (p != q) off += e;
             }
-        if (off <= 1e-30 * tot || off == 0.0) break;
+        // stop once the off-diagonal energy is below 1e-26 of the total
+        // (entries ~1e-13 of the norm, above FP64 rounding noise)
+        if (off <= 1e-26 * tot || off == 0.0) break;
         for (int p = 0; p < n - 1; p++) {
             for (int q = p + 1; q < n; q++) {
                 double apq = a[(size_t)p * n + q];
