@@ -24,6 +24,7 @@ class _DenoiseThresholds(torch.autograd.Function):
             out = proc.denoise_fwd(params, thresholds.detach().cpu(), img, nl)
             outs.append(out[..., :3].clone())
         ctx.processors = processors
+        ctx.thr_device = thresholds.device
         return torch.stack(outs)
 
     @staticmethod
@@ -34,7 +35,7 @@ class _DenoiseThresholds(torch.autograd.Function):
             g[..., :3] = grad_out[k]
             gk = proc.denoise_bwd(g)
             total = gk if total is None else total + gk
-        return total.to(grad_out.device), None, None, None, None
+        return total.to(ctx.thr_device), None, None, None, None
 
 
 def denoise_thresholds(thresholds: torch.Tensor, images: Sequence[torch.Tensor],

@@ -116,15 +116,36 @@ __global__ __launch_bounds__(256) void k_fill_zero(float* p, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) p[i] = 0.0f;
 }
 
+// Synchronization point of one wavefront's lanes over LDS.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// FP64 reciprocal square root and reciprocal from the hardware estimates
+// (v_rsq_f64, v_rcp_f64) refined by two Newton steps each.
+__device__ __forceinline__ double rsqrt_f64(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+    r = r * (1.5 - 0.5 * x * r * r);
+    return r * (1.5 - 0.5 * x * r * r);
+}
+__device__ __forceinline__ double rcp_f64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = r * (2.0 - x * r);
+    return r * (2.0 - x * r);
+}
+
 // Covariance + cyclic Jacobi (parallel circle ordering) + basis selection.
 //
 // C = (sum x x^T - (sum x)(sum x)^T / n) / (n - 1)   -- PCA.cpp:31-32
 // Jacobi: each sweep runs 25 rounds; round r rotates the 12 disjoint pairs
 // ((r + k) mod 25, (r - k) mod 25), k = 1..12, so every off-diagonal pair is
 // annihilated once per sweep (index r idles).  Rotation as in Golub & Van Loan
-// Alg. 8.5.2.  Sweeps stop when the off-diagonal energy is below 1e-30 of the
-// total.  The 8 largest eigenpairs, largest first (PCA.cpp:49-55), are written
-// as basis[r * 8 + c]; ties in eigenvalue keep index order.
+// Alg. 8.5.2 (c^2 + s^2 = 1 to FP64 precision, so V stays orthogonal).  Sweeps
+// stop when the off-diagonal energy is below kJacobiTol of the total.  The 8
+// largest eigenpairs, largest first (PCA.cpp:49-55), are written as
+// basis[r * 8 + c]; ties in eigenvalue keep index order.
 __global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int nblocks, double n, float* basis,
                                                    double* eigvals, double* eig_full) {
     constexpr int N = kPatch;
@@ -134,7 +155,6 @@ __global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int n
     __shared__ double V[N * LD];
     __shared__ double cs[12][2];
     __shared__ int pq[12][2];
-    __shared__ double red[2][4];
     const int tid = threadIdx.x;
 
     for (int e = tid; e < kMoments; e += 256) {
@@ -163,76 +183,111 @@ __global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int n
     }
     __syncthreads();
 
-    for (int sweep = 0; sweep < 40; sweep++) {
-        double off = 0.0, tot = 0.0;
-        for (int e = tid; e < N * N; e += 256) {
-            const int r = e / N, s = e % N;
-            const double v = A[r * LD + s] * A[r * LD + s];
-            tot += v;
-            if (r != s) off += v;
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            off += __shfl_down(off, o);
-            tot += __shfl_down(tot, o);
-        }
-        if ((tid & 63) == 0) {
-            red[0][tid >> 6] = off;
-            red[1][tid >> 6] = tot;
-        }
-        __syncthreads();
-        off = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-        tot = red[1][0] + red[1][1] + red[1][2] + red[1][3];
-        __syncthreads();
-        if (off == 0.0 || off <= kJacobiTol * tot) break;
+    // The rotations run on wavefront 0 alone: its lanes share LDS without
+    // workgroup barriers (a wavefront's LDS accesses complete in order; the
+    // fences keep the compiler from reordering them across the sync points).
+    if (tid < 64) {
+        const int lane = tid;
+        for (int sweep = 0; sweep < 40; sweep++) {
+            double off = 0.0, tot = 0.0;
+            for (int e = lane; e < N * N; e += 64) {
+                const int r = e / N, s = e % N;
+                const double v = A[r * LD + s] * A[r * LD + s];
+                tot += v;
+                if (r != s) off += v;
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                off += __shfl_xor(off, o);
+                tot += __shfl_xor(tot, o);
+            }
+            if (off == 0.0 || off <= kJacobiTol * tot) break;  // wave-uniform
 
-        for (int round = 0; round < N; round++) {
-            if (tid < 12) {
-                int p = (round + tid + 1) % N, q = (round - tid - 1 + N) % N;
-                if (p > q) {
-                    const int t = p;
-                    p = q;
-                    q = t;
+            for (int round = 0; round < N; round++) {
+                if (lane < 12) {
+                    int p = (round + lane + 1) % N, q = (round - lane - 1 + N) % N;
+                    if (p > q) {
+                        const int t = p;
+                        p = q;
+                        q = t;
+                    }
+                    // t = sign(theta) / (|theta| + sqrt(theta^2 + 1)), theta = d / e with
+                    // d = a_qq - a_pp, e = 2 a_pq:  t = sign(d e) |e| / (|d| + hypot(d, e)).
+                    const double e = 2.0 * A[p * LD + q], d = A[q * LD + q] - A[p * LD + p];
+                    double c = 1.0, s = 0.0;
+                    if (e != 0.0) {
+                        const double h2 = d * d + e * e;
+                        const double hyp = h2 * rsqrt_f64(h2);
+                        double t = fabs(e) * rcp_f64(fabs(d) + hyp);
+                        if (d * e < 0.0) t = -t;
+                        c = rsqrt_f64(1.0 + t * t);
+                        s = t * c;
+                    }
+                    cs[lane][0] = c;
+                    cs[lane][1] = s;
+                    pq[lane][0] = p;
+                    pq[lane][1] = q;
                 }
-                const double apq = A[p * LD + q];
-                double c = 1.0, s = 0.0;
-                if (apq != 0.0) {
-                    const double theta = (A[q * LD + q] - A[p * LD + p]) / (2.0 * apq);
-                    const double at = fabs(theta);
-                    double t = at > 1e150 ? 0.5 / at : 1.0 / (at + sqrt(theta * theta + 1.0));
-                    if (theta < 0.0) t = -t;
-                    c = 1.0 / sqrt(t * t + 1.0);
-                    s = t * c;
+                wave_sync();
+                // A <- J^T A: rows p, q of every pair; all loads first, then the stores.
+                {
+                    constexpr int ITEMS = (12 * N + 63) / 64;
+                    double ap[ITEMS], aq[ITEMS], cc[ITEMS], ss[ITEMS];
+                    int ip[ITEMS], iq[ITEMS];
+#pragma unroll
+                    for (int m = 0; m < ITEMS; m++) {
+                        const int t = min(lane + 64 * m, 12 * N - 1);
+                        const int k = t / N, j = t % N;
+                        ip[m] = pq[k][0] * LD + j;
+                        iq[m] = pq[k][1] * LD + j;
+                        cc[m] = cs[k][0];
+                        ss[m] = cs[k][1];
+                        ap[m] = A[ip[m]];
+                        aq[m] = A[iq[m]];
+                    }
+                    wave_sync();
+#pragma unroll
+                    for (int m = 0; m < ITEMS; m++) {
+                        if (lane + 64 * m < 12 * N) {
+                            A[ip[m]] = cc[m] * ap[m] - ss[m] * aq[m];
+                            A[iq[m]] = ss[m] * ap[m] + cc[m] * aq[m];
+                        }
+                    }
                 }
-                cs[tid][0] = c;
-                cs[tid][1] = s;
-                pq[tid][0] = p;
-                pq[tid][1] = q;
+                wave_sync();
+                // A <- A J and V <- V J: columns p, q of every pair.
+                {
+                    constexpr int ITEMS = (24 * N + 63) / 64;
+                    double mp[ITEMS], mq[ITEMS], cc[ITEMS], ss[ITEMS];
+                    int ip[ITEMS], iq[ITEMS];
+                    bool isA[ITEMS];
+#pragma unroll
+                    for (int m = 0; m < ITEMS; m++) {
+                        const int t = min(lane + 64 * m, 24 * N - 1);
+                        const int u = t / N, j = t % N, k = u % 12;
+                        isA[m] = u < 12;
+                        ip[m] = j * LD + pq[k][0];
+                        iq[m] = j * LD + pq[k][1];
+                        cc[m] = cs[k][0];
+                        ss[m] = cs[k][1];
+                        const double* M = isA[m] ? A : V;
+                        mp[m] = M[ip[m]];
+                        mq[m] = M[iq[m]];
+                    }
+                    wave_sync();
+#pragma unroll
+                    for (int m = 0; m < ITEMS; m++) {
+                        if (lane + 64 * m < 24 * N) {
+                            double* M = isA[m] ? A : V;
+                            M[ip[m]] = cc[m] * mp[m] - ss[m] * mq[m];
+                            M[iq[m]] = ss[m] * mp[m] + cc[m] * mq[m];
+                        }
+                    }
+                }
+                wave_sync();
             }
-            __syncthreads();
-            // A <- J^T A: rows p, q of every pair
-            for (int t = tid; t < 12 * N; t += 256) {
-                const int k = t / N, j = t % N;
-                const int p = pq[k][0], q = pq[k][1];
-                const double c = cs[k][0], s = cs[k][1];
-                const double ap = A[p * LD + j], aq = A[q * LD + j];
-                A[p * LD + j] = c * ap - s * aq;
-                A[q * LD + j] = s * ap + c * aq;
-            }
-            __syncthreads();
-            // A <- A J and V <- V J: columns p, q of every pair
-            for (int t = tid; t < 24 * N; t += 256) {
-                const int u = t / N, j = t % N;
-                const int k = u % 12;
-                double* M = u < 12 ? A : V;
-                const int p = pq[k][0], q = pq[k][1];
-                const double c = cs[k][0], s = cs[k][1];
-                const double mp = M[j * LD + p], mq = M[j * LD + q];
-                M[j * LD + p] = c * mp - s * mq;
-                M[j * LD + q] = s * mp + c * mq;
-            }
-            __syncthreads();
         }
     }
+    __syncthreads();
 
     if (tid < N) {
         const double di = A[tid * LD + tid];
