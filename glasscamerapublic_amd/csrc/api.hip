@@ -132,7 +132,7 @@ gcam_status gcam_pyramid_create(int32_t width, int32_t height, int32_t levels, c
     for (int i = 0; i < levels - 1; i++) parts.push_back({(void**)&p->sub[i], sizeof(float4) * p->W[i] * p->H[i]});
     for (int i = 0; i < levels; i++) parts.push_back({(void**)&p->den[i], sizeof(float4) * p->W[i] * p->H[i]});
     parts.push_back({(void**)&p->feat, sizeof(float) * 8 * (size_t)width * height});
-    parts.push_back({(void**)&p->moment_partials, sizeof(double) * kMomentBlocks * kMoments});
+    parts.push_back({(void**)&p->moment_partials, sizeof(double) * kMomentSlots * kMoments});
     parts.push_back({(void**)&p->nlf_partials, sizeof(double) * kNlfBlocks * 13});
     parts.push_back({(void**)&p->nlf_state, sizeof(NlfFitState)});
     parts.push_back({(void**)&p->nlf_dev, sizeof(NlfDev) * GCAM_MAX_LEVELS});
@@ -553,7 +553,7 @@ gcam_status gcam_build_pca_space(const float* patches, int64_t n, float* basis, 
     if (n < 2) return set_error(GCAM_ERR_INVALID_ARGUMENT, "build_pca_space needs at least 2 patches");
     hipStream_t s = as_stream(stream);
     double* partials = nullptr;
-    GCAM_HIP_TRY(hipMallocAsync((void**)&partials, sizeof(double) * kMomentBlocks * kMoments, s));
+    GCAM_HIP_TRY(hipMallocAsync((void**)&partials, sizeof(double) * kMomentSlots * kMoments, s));
     int nblocks = 0;
     gcam_status st = launch_buffer_moments(patches, n, partials, &nblocks, s);
     if (st == GCAM_OK) st = launch_pca_solve(partials, nblocks, n, basis, eigenvalues, nullptr, s);

@@ -15,6 +15,7 @@ constexpr int kPatch = 25;      // 5 x 5 luma patch
 constexpr int kComponents = 8;  // PCA components kept
 constexpr int kMoments = kPatch + kPatch * (kPatch + 1) / 2;  // sum x (25) + upper triangle of sum x x^T (325)
 constexpr int kMomentBlocks = 240;  // fixed grid of the moment reduction (deterministic order)
+constexpr int kMomentSlots = kMomentBlocks + 1;  // partials of each workgroup, then their sum
 constexpr int kNlfBlocks = 256;     // fixed grid of the noise-model reductions
 
 // Device view of an image: base pointer, size and row pitch in pixels.
@@ -97,7 +98,7 @@ gcam_status launch_image_moments(View<const float4> in, double* partials, int* n
 // Moments of a device patch buffer (n x 25).
 gcam_status launch_buffer_moments(const float* patches, int64_t n, double* partials, int* nblocks, hipStream_t s);
 // Reduce the partials, form the covariance, eigensolve, write basis (25 x 8 f32) and eigenvalues.
-gcam_status launch_pca_solve(const double* partials, int nblocks, int64_t n, float* basis, double* eigvals,
+gcam_status launch_pca_solve(double* partials, int nblocks, int64_t n, float* basis, double* eigvals,
                              double* eig_full, hipStream_t s);
 gcam_status launch_pca_project(View<const float4> in, const float* basis, View<float> feat8, hipStream_t s);
 

@@ -42,61 +42,117 @@ __device__ __forceinline__ void moment_rs(int e, int& r, int& s) {
     s = r + t;
 }
 
-// Moment accumulation over rounds of 256 patches staged in LDS.  Each thread
-// owns moment entries tid and tid + 256 (350 in all).  `fetch(k, lds_row)`
-// writes the 25 values of patch k (zeros when k is out of range).
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Moments of workgroup `blk` of `nblk` as a Gram product on the FP64 matrix
+// cores: with X the npatch x 32 patch matrix (25 luma values, 7 zero
+// columns), X^T X is accumulated 4 patches at a time by
+// v_mfma_f64_16x16x4f64 in three 16 x 16 tiles (00, 01, 11; 10 is the
+// transpose of 01).  Operand layout: lane l supplies X[4 g + l/16][l%16 (+16)]
+// as both the A (16 x 4) and the B (4 x 16) operand; the result tile holds
+// C[l/16 + 4 r][l%16] in register r of lane l (measured on gfx950; unlike
+// the FP32 16 x 16 forms, rows interleave by 4).  sum x accumulates on the
+// vector ALU.  The four wavefronts combine in LDS in a fixed order, so the
+// partial of a workgroup is deterministic.  `fetch(k, f)` returns feature f
+// of patch k (0 for k >= npatch or f >= 25).
 template <class Fetch>
-__device__ void moments_body(int64_t npatch, double* partials, Fetch fetch) {
-    __shared__ float pt[256 * kPatch];
-    int r0, s0, r1 = 0, s1 = -1;
-    moment_rs(threadIdx.x, r0, s0);
-    const bool has1 = threadIdx.x + 256 < kMoments;
-    if (has1) moment_rs(threadIdx.x + 256, r1, s1);
-    double acc0 = 0.0, acc1 = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * 256;
-    for (int64_t base = (int64_t)blockIdx.x * 256; base < npatch; base += stride) {
-        const int64_t k = base + threadIdx.x;
-        fetch(k, &pt[threadIdx.x * kPatch]);
-        __syncthreads();
-        const int cnt = (int)min<int64_t>(256, npatch - base);
-        for (int i = 0; i < cnt; i++) {
-            const float* row = &pt[i * kPatch];
-            const double a0 = row[r0];
-            acc0 += s0 < 0 ? a0 : a0 * (double)row[s0];
-            if (has1) {
-                const double a1 = row[r1];
-                acc1 += s1 < 0 ? a1 : a1 * (double)row[s1];
-            }
+__device__ void moments_body(int blk, int nblk, int64_t npatch, double* partial, Fetch fetch) {
+    __shared__ double gram[4][32][33];
+    __shared__ double sums[4][64][2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int f = lane & 15, kq = lane >> 4;
+    f64x4 c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+    double slo = 0.0, shi = 0.0;
+    const int64_t ngroups = (npatch + 3) / 4;
+    const int64_t stride = (int64_t)nblk * 4;
+    constexpr int U = 8;  // groups in flight per wavefront
+    for (int64_t g0 = (int64_t)blk * 4 + wave; g0 < ngroups; g0 += U * stride) {
+        float lo[U], hi[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t k = (g0 + u * stride) * 4 + kq;
+            lo[u] = fetch(k, f);
+            hi[u] = fetch(k, f + 16);
         }
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const double dl = lo[u], dh = hi[u];
+            slo += dl;
+            shi += dh;
+            c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(dl, dl, c00, 0, 0, 0);
+            c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(dl, dh, c01, 0, 0, 0);
+            c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(dh, dh, c11, 0, 0, 0);
+        }
     }
-    partials[(size_t)blockIdx.x * kMoments + threadIdx.x] = acc0;
-    if (has1) partials[(size_t)blockIdx.x * kMoments + threadIdx.x + 256] = acc1;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int i = kq + 4 * r;
+        gram[wave][i][f] = c00[r];
+        gram[wave][i][16 + f] = c01[r];
+        gram[wave][16 + i][16 + f] = c11[r];
+    }
+    sums[wave][lane][0] = slo;
+    sums[wave][lane][1] = shi;
+    __syncthreads();
+    for (int e = tid; e < kMoments; e += 256) {
+        int r, s;
+        moment_rs(e, r, s);
+        double acc = 0.0;
+        if (s < 0) {
+            for (int w = 0; w < 4; w++)
+                for (int q = 0; q < 4; q++) acc += sums[w][16 * q + (r & 15)][r >> 4];
+        } else {
+            for (int w = 0; w < 4; w++) acc += gram[w][r][s];
+        }
+        partial[e] = acc;
+    }
 }
 
 // Patches sampled from the image at 8*(x, y), x < w/8, y < h/8, luma only,
 // clamp-to-edge reads (collectPatches).  Slots the grid does not reach are
 // zero rows: they add nothing to the moments but count in n.
+__device__ __forceinline__ float image_patch_value(const View<const float4>& in, int gw, int64_t npatch, int64_t k,
+                                                   int f) {
+    if (k >= npatch || f >= kPatch) return 0.0f;
+    const int kk = (int)k;  // npatch <= 16384^2 / 64
+    const int x = kk % gw, y = kk / gw;
+    const int j = f / 5 - 2, i = f % 5 - 2;
+    return in.at(clampi(8 * x + i, 0, in.w - 1), clampi(8 * y + j, 0, in.h - 1)).x;
+}
+
 __global__ __launch_bounds__(256) void k_image_moments(View<const float4> in, double* partials) {
     const int gw = in.w / 8, gh = in.h / 8;
     const int64_t npatch = (int64_t)gw * gh;
-    moments_body(npatch, partials, [&](int64_t k, float* row) {
-        if (k < npatch) {
-            const int x = (int)(k % gw), y = (int)(k / gw);
-            for (int j = -2; j <= 2; j++) {
-                const int sy = clampi(8 * y + j, 0, in.h - 1);
-                for (int i = -2; i <= 2; i++) row[(j + 2) * 5 + (i + 2)] = in.at(clampi(8 * x + i, 0, in.w - 1), sy).x;
-            }
-        } else {
-            for (int e = 0; e < kPatch; e++) row[e] = 0.0f;
-        }
-    });
+    moments_body(blockIdx.x, gridDim.x, npatch, partials + (size_t)blockIdx.x * kMoments,
+                 [&](int64_t k, int f) { return image_patch_value(in, gw, npatch, k, f); });
 }
 
 __global__ __launch_bounds__(256) void k_buffer_moments(const float* patches, int64_t n, double* partials) {
-    moments_body(n, partials, [&](int64_t k, float* row) {
-        for (int e = 0; e < kPatch; e++) row[e] = k < n ? patches[k * kPatch + e] : 0.0f;
+    moments_body(blockIdx.x, gridDim.x, n, partials + (size_t)blockIdx.x * kMoments, [&](int64_t k, int f) {
+        return (k < n && f < kPatch) ? patches[k * kPatch + f] : 0.0f;
     });
+}
+
+// Sum of the per-workgroup partials, entry by entry: workgroup x owns 14
+// entries; thread t adds the partials b = t/14, t/14 + 18, ... of entry
+// 14 x + t%14, and the 18 sums are added in index order (deterministic).
+constexpr int kRedEntries = 14, kRedLanes = 18;
+static_assert(kMoments % kRedEntries == 0, "moment reduction tiling");
+__global__ __launch_bounds__(256) void k_moments_reduce(const double* partials, int nblocks, double* mom) {
+    __shared__ double part[kRedLanes][kRedEntries];
+    const int t = threadIdx.x;
+    const int e = blockIdx.x * kRedEntries + t % kRedEntries, sub = t / kRedEntries;
+    if (sub < kRedLanes) {
+        double acc = 0.0;
+        for (int b = sub; b < nblocks; b += kRedLanes) acc += partials[(size_t)b * kMoments + e];
+        part[sub][t % kRedEntries] = acc;
+    }
+    __syncthreads();
+    if (t < kRedEntries) {
+        double acc = 0.0;
+        for (int q = 0; q < kRedLanes; q++) acc += part[q][t];
+        mom[blockIdx.x * kRedEntries + t] = acc;
+    }
 }
 
 // collectPatches materialized (per-operation entry point): slot = y*w/8 + x.
@@ -114,13 +170,6 @@ __global__ __launch_bounds__(256) void k_collect_patches(View<const float4> in, 
 
 __global__ __launch_bounds__(256) void k_fill_zero(float* p, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) p[i] = 0.0f;
-}
-
-// Synchronization point of one wavefront's lanes over LDS.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // FP64 reciprocal square root and reciprocal from the hardware estimates
@@ -146,28 +195,17 @@ __device__ __forceinline__ double rcp_f64(double x) {
 // stop when the off-diagonal energy is below kJacobiTol of the total.  The 8
 // largest eigenpairs, largest first (PCA.cpp:49-55), are written as
 // basis[r * 8 + c]; ties in eigenvalue keep index order.
-__global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int nblocks, double n, float* basis,
-                                                   double* eigvals, double* eig_full) {
+__device__ void pca_solve_body(const double* moments, double n, float* basis, double* eigvals, double* eig_full) {
     constexpr int N = kPatch;
     constexpr int LD = N + 1;
     __shared__ double mom[kMoments];
     __shared__ double A[N * LD];
     __shared__ double V[N * LD];
-    __shared__ double cs[12][2];
-    __shared__ int pq[12][2];
+    __shared__ double2 rot[13];
+    __shared__ double2 energy[4];
     const int tid = threadIdx.x;
 
-    for (int e = tid; e < kMoments; e += 256) {
-        double a = 0.0;
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // 8 loads in flight; fixed summation order
-        int b = 0;
-        for (; b + 8 <= nblocks; b += 8)
-#pragma unroll
-            for (int u = 0; u < 8; u++) acc[u] += partials[(size_t)(b + u) * kMoments + e];
-        for (; b < nblocks; b++) acc[0] += partials[(size_t)b * kMoments + e];
-        for (int u = 0; u < 8; u++) a += acc[u];
-        mom[e] = a;
-    }
+    for (int e = tid; e < kMoments; e += 256) mom[e] = moments[e];
     __syncthreads();
     for (int e = tid; e < kMoments; e += 256) {
         if (e < N) continue;
@@ -183,33 +221,80 @@ __global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int n
     }
     __syncthreads();
 
-    // The rotations run on wavefront 0 alone: its lanes share LDS without
-    // workgroup barriers (a wavefront's LDS accesses complete in order; the
-    // fences keep the compiler from reordering them across the sync points).
-    if (tid < 64) {
-        const int lane = tid;
+    // A round applies its 12 rotations at once, A <- J^T A J and V <- V J.
+    // Round r pairs the indices into super-indices u = 0..11, (p_u, q_u) =
+    // the sorted ((r + u + 1) mod 25, (r - u - 1) mod 25), and u = 12 = {r}
+    // (identity).  J acts on the 2 x 2 blocks A[{p_u, q_u}][{p_v, q_v}]:
+    // Y = J_u^T X J_v with J = [[c, s], [-s, c]] (rows/columns p, q), and the
+    // pair entry of a diagonal block is set to zero.  Every entry belongs to
+    // exactly one block, so the update is in place: threads 0..90 own one
+    // block each (both triangles), threads 128..255 the 25 x 13 row pieces
+    // of V; lanes 0..11 of wavefront 0 compute the rotations.
+    {
+        constexpr int NU = N * (N + 1) / 2;  // upper triangle, a <= b
+        constexpr int IU = (NU + 255) / 256;
+        constexpr int NS = 13, NB = NS * (NS + 1) / 2;  // super-indices, blocks
+        constexpr int IV = (N * NS + 127) / 128;
+        int ua[IU], ub[IU];
+#pragma unroll
+        for (int m = 0; m < IU; m++) {
+            int t = min(tid + 256 * m, NU - 1), a = 0;
+            while (t >= N - a) {
+                t -= N - a;
+                a++;
+            }
+            ua[m] = a;
+            ub[m] = a + t;
+        }
+        int bu = 0, bv = 0;  // block of threads < NB
+        {
+            int t = min(tid, NB - 1);
+            while (t >= NS - bu) {
+                t -= NS - bu;
+                bu++;
+            }
+            bv = bu + t;
+        }
+        if (tid == 0) rot[NS - 1] = make_double2(1.0, 0.0);
         for (int sweep = 0; sweep < 40; sweep++) {
+            // Off-diagonal and total energy: per-thread entries, wavefront
+            // sums, then the 4 wavefront sums in a fixed order (uniform).
             double off = 0.0, tot = 0.0;
-            for (int e = lane; e < N * N; e += 64) {
-                const int r = e / N, s = e % N;
-                const double v = A[r * LD + s] * A[r * LD + s];
-                tot += v;
-                if (r != s) off += v;
+#pragma unroll
+            for (int m = 0; m < IU; m++) {
+                if (tid + 256 * m < NU) {
+                    const double x = A[ua[m] * LD + ub[m]];
+                    if (ua[m] == ub[m]) {
+                        tot += x * x;
+                    } else {
+                        off += 2.0 * x * x;
+                    }
+                }
             }
             for (int o = 32; o > 0; o >>= 1) {
                 off += __shfl_xor(off, o);
                 tot += __shfl_xor(tot, o);
             }
-            if (off == 0.0 || off <= kJacobiTol * tot) break;  // wave-uniform
+            if ((tid & 63) == 0) energy[tid >> 6] = make_double2(off, tot);
+            __syncthreads();
+            off = energy[0].x + energy[1].x + energy[2].x + energy[3].x;
+            tot = energy[0].y + energy[1].y + energy[2].y + energy[3].y + off;
+            if (off == 0.0 || off <= kJacobiTol * tot) break;
 
             for (int round = 0; round < N; round++) {
-                if (lane < 12) {
-                    int p = (round + lane + 1) % N, q = (round - lane - 1 + N) % N;
-                    if (p > q) {
-                        const int t = p;
-                        p = q;
-                        q = t;
+                // pair of super-index u in this round (u = NS - 1: the idle index)
+                auto pair_of = [&](int u, int& p, int& q) {
+                    if (u == NS - 1) {
+                        p = q = round;
+                        return;
                     }
+                    const int i = (round + u + 1) % N, j = (round - u - 1 + N) % N;
+                    p = min(i, j);
+                    q = max(i, j);
+                };
+                if (tid < NS - 1) {
+                    int p, q;
+                    pair_of(tid, p, q);
                     // t = sign(theta) / (|theta| + sqrt(theta^2 + 1)), theta = d / e with
                     // d = a_qq - a_pp, e = 2 a_pq:  t = sign(d e) |e| / (|d| + hypot(d, e)).
                     const double e = 2.0 * A[p * LD + q], d = A[q * LD + q] - A[p * LD + p];
@@ -222,68 +307,57 @@ __global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int n
                         c = rsqrt_f64(1.0 + t * t);
                         s = t * c;
                     }
-                    cs[lane][0] = c;
-                    cs[lane][1] = s;
-                    pq[lane][0] = p;
-                    pq[lane][1] = q;
+                    rot[tid] = make_double2(c, s);
                 }
-                wave_sync();
-                // A <- J^T A: rows p, q of every pair; all loads first, then the stores.
-                {
-                    constexpr int ITEMS = (12 * N + 63) / 64;
-                    double ap[ITEMS], aq[ITEMS], cc[ITEMS], ss[ITEMS];
-                    int ip[ITEMS], iq[ITEMS];
+                __syncthreads();
+                if (tid < NB) {
+                    int pu, qu, pv, qv;
+                    pair_of(bu, pu, qu);
+                    pair_of(bv, pv, qv);
+                    const double2 ru = rot[bu], rv = rot[bv];
+                    const double x00 = A[pu * LD + pv], x01 = A[pu * LD + qv];
+                    const double x10 = A[qu * LD + pv], x11 = A[qu * LD + qv];
+                    // Z = X J_v, Y = J_u^T Z
+                    const double z00 = x00 * rv.x - x01 * rv.y, z01 = x00 * rv.y + x01 * rv.x;
+                    const double z10 = x10 * rv.x - x11 * rv.y, z11 = x10 * rv.y + x11 * rv.x;
+                    const double y00 = ru.x * z00 - ru.y * z10, y01 = ru.x * z01 - ru.y * z11;
+                    double y10 = ru.y * z00 + ru.x * z10;
+                    const double y11 = ru.y * z01 + ru.x * z11;
+                    double y01w = y01;
+                    if (bu == bv && bu != NS - 1) y01w = y10 = 0.0;  // the annihilated pair
+                    A[pu * LD + pv] = y00;
+                    A[pu * LD + qv] = y01w;
+                    A[qu * LD + pv] = y10;
+                    A[qu * LD + qv] = y11;
+                    A[pv * LD + pu] = y00;
+                    A[qv * LD + pu] = y01w;
+                    A[pv * LD + qu] = y10;
+                    A[qv * LD + qu] = y11;
+                } else if (tid >= 128) {
+                    double2 rr[IV];
+                    double v0[IV], v1[IV];
+                    int ip[IV], iq[IV];
 #pragma unroll
-                    for (int m = 0; m < ITEMS; m++) {
-                        const int t = min(lane + 64 * m, 12 * N - 1);
-                        const int k = t / N, j = t % N;
-                        ip[m] = pq[k][0] * LD + j;
-                        iq[m] = pq[k][1] * LD + j;
-                        cc[m] = cs[k][0];
-                        ss[m] = cs[k][1];
-                        ap[m] = A[ip[m]];
-                        aq[m] = A[iq[m]];
+                    for (int m = 0; m < IV; m++) {
+                        const int it = min(tid - 128 + 128 * m, N * NS - 1);
+                        const int j = it / NS, v = it % NS;
+                        int p, q;
+                        pair_of(v, p, q);
+                        ip[m] = j * LD + p;
+                        iq[m] = j * LD + q;
+                        rr[m] = rot[v];
+                        v0[m] = V[ip[m]];
+                        v1[m] = V[iq[m]];
                     }
-                    wave_sync();
 #pragma unroll
-                    for (int m = 0; m < ITEMS; m++) {
-                        if (lane + 64 * m < 12 * N) {
-                            A[ip[m]] = cc[m] * ap[m] - ss[m] * aq[m];
-                            A[iq[m]] = ss[m] * ap[m] + cc[m] * aq[m];
+                    for (int m = 0; m < IV; m++) {
+                        if (tid - 128 + 128 * m < N * NS) {
+                            V[ip[m]] = rr[m].x * v0[m] - rr[m].y * v1[m];
+                            V[iq[m]] = rr[m].y * v0[m] + rr[m].x * v1[m];
                         }
                     }
                 }
-                wave_sync();
-                // A <- A J and V <- V J: columns p, q of every pair.
-                {
-                    constexpr int ITEMS = (24 * N + 63) / 64;
-                    double mp[ITEMS], mq[ITEMS], cc[ITEMS], ss[ITEMS];
-                    int ip[ITEMS], iq[ITEMS];
-                    bool isA[ITEMS];
-#pragma unroll
-                    for (int m = 0; m < ITEMS; m++) {
-                        const int t = min(lane + 64 * m, 24 * N - 1);
-                        const int u = t / N, j = t % N, k = u % 12;
-                        isA[m] = u < 12;
-                        ip[m] = j * LD + pq[k][0];
-                        iq[m] = j * LD + pq[k][1];
-                        cc[m] = cs[k][0];
-                        ss[m] = cs[k][1];
-                        const double* M = isA[m] ? A : V;
-                        mp[m] = M[ip[m]];
-                        mq[m] = M[iq[m]];
-                    }
-                    wave_sync();
-#pragma unroll
-                    for (int m = 0; m < ITEMS; m++) {
-                        if (lane + 64 * m < 24 * N) {
-                            double* M = isA[m] ? A : V;
-                            M[ip[m]] = cc[m] * mp[m] - ss[m] * mq[m];
-                            M[iq[m]] = ss[m] * mp[m] + cc[m] * mq[m];
-                        }
-                    }
-                }
-                wave_sync();
+                __syncthreads();
             }
         }
     }
@@ -313,6 +387,11 @@ __global__ __launch_bounds__(256) void k_pca_solve(const double* partials, int n
             eig_full[658 + tid] = mom[tid] / n;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_pca_solve(const double* moments, double n, float* basis, double* eigvals,
+                                                   double* eig_full) {
+    pca_solve_body(moments, n, basis, eigvals, eig_full);
 }
 
 // pcaProjection: f[c] = sum_row basis[row][c] * Y(p + off(row)), rows j-outer /
@@ -379,9 +458,11 @@ gcam_status launch_buffer_moments(const float* patches, int64_t n, double* parti
     return GCAM_OK;
 }
 
-gcam_status launch_pca_solve(const double* partials, int nblocks, int64_t n, float* basis, double* eigvals,
+gcam_status launch_pca_solve(double* partials, int nblocks, int64_t n, float* basis, double* eigvals,
                              double* eig_full, hipStream_t s) {
-    k_pca_solve<<<1, 256, 0, s>>>(partials, nblocks, (double)n, basis, eigvals, eig_full);
+    double* mom = partials + (size_t)kMomentBlocks * kMoments;
+    k_moments_reduce<<<kMoments / kRedEntries, 256, 0, s>>>(partials, nblocks, mom);
+    k_pca_solve<<<1, 256, 0, s>>>(mom, (double)n, basis, eigvals, eig_full);
     GCAM_LAUNCH_CHECK("PCA eigensolve");
     return GCAM_OK;
 }
