@@ -3,17 +3,23 @@
 
 One step = one whole ``PyramidProcessor::denoise`` call (5-level pyramid,
 5x5 patches -> 8 PCA components, 9x9 search window) over one synthetic 12 MP
-frame already resident in HBM.  Prints ONE JSON line (rank 0) with the
-throughput in Mpix/s of the base level, the roofline of the dominant kernel
-(block matching, timed live with HIP events on its own stream) and the CPU
-oracle timed on this host's cores on the same frame.
+frame already resident in HBM.  The reported form is the precision form
+(FP16 MFMA projection and FP16 distance products, FP32 accumulation); the
+plain FP32 form is timed beside it on the same frame, and the PSNR loss of the
+reported form against the FP64 oracle is measured on that frame.  Prints ONE
+JSON line with the throughput in Mpix/s of the base level, the roofline of
+the dominant kernel (block matching, timed with HIP events recorded by the
+library on the stream it launches on) and the CPU oracle timed on this host's
+cores on the same frame.
 
-    python bench.py [--steps 20] [--warmup 3] [--no-cpu-baseline]
+    python bench.py [--steps 20] [--warmup 3] [--precision fp16] [--no-cpu-baseline]
+    python bench.py --workload burst32      # the 32-frame mixed-size burst (batched form)
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
@@ -28,6 +34,8 @@ WORKLOADS = {
     "1mp": (1024, 1024, 4, 3, 2e-3, 1e-3),
     "48mp": (8064, 6048, 5, 4, 4e-3, 2e-3),
 }
+BURST_SIZES = [(3264, 2448), (4032, 3024), (5472, 3648), (8064, 6048)]
+DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}
 
 
 def algorithmic_bytes_per_base_pixel(levels: int) -> float:
@@ -44,23 +52,40 @@ def cpu_cores() -> int:
         return os.cpu_count() or 1
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="12mp", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"])
-    ap.add_argument("--bm-rows", type=int, default=0, help="block-matching output rows per thread (0 = default)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: min(cores, 16))")
-    args = ap.parse_args()
+def psnr(a, b) -> float:
+    import numpy as np
+    mse = float(np.mean((a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64)) ** 2))
+    return 10.0 * math.log10(1.0 / mse)
 
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "run with --gpus 1", file=sys.stderr)
-        return 2
 
+def time_single(gcam, torch, w, h, levels, radius, precision, bm_rows, ycc, params, nlf, steps, warmup, dev):
+    """K timed steps of one processor; the library's stage events time each stage on its stream."""
+    stream = torch.cuda.current_stream()
+    proc = gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius, precision=precision, device=dev,
+                                 bm_rows_per_thread=bm_rows)
+    for _ in range(warmup):
+        proc.denoise(params, ycc, None, nlf)
+    proc.set_profiling(True)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for k in range(steps):
+        proc.denoise(params, ycc, None, nlf)
+        evs[k + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = [evs[k].elapsed_time(evs[k + 1]) for k in range(steps)]
+    stages = proc.stage_times()
+    proc.set_profiling(False)
+    out = proc.denoisedImagePyramid[0].cpu().numpy()
+    proc.close()
+    total_ms = evs[0].elapsed_time(evs[-1])
+    return {"ms_per_step": total_ms / steps, "median_ms": statistics.median(step_ms),
+            "wall_ms_per_step": wall * 1e3 / steps, "stages": stages, "out": out}
+
+
+def single_frame(args) -> dict:
     import numpy as np
     import torch
 
@@ -71,74 +96,57 @@ def main() -> int:
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
 
-    ycc = gcam.synth_frame(w, h, a, b, scene_seed=1234, noise_seed=5678, device=dev)
-    proc = gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius, precision=args.precision, device=dev,
-                                 bm_rows_per_thread=args.bm_rows)
+    ycc, clean = gcam.synth_frame(w, h, a, b, scene_seed=1234, noise_seed=5678, device=dev, want_clean=True)
     params = [gcam.DenoiseParameters() for _ in range(levels)]
     nlf = [((1e-8,) * 3, (1e-8,) * 3) for _ in range(levels)]
 
     # Noise model: MeasureYCbCrNLF on every level, once per frame, outside the timed region.
-    proc.denoise(params, ycc, None, nlf, calibrateFromImage=True)
+    cal = gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius, device=dev)
+    cal.denoise(params, ycc, None, nlf, calibrateFromImage=True)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    proc.denoise(params, ycc, None, list(nlf), calibrateFromImage=True)
+    cal.denoise(params, ycc, None, list(nlf), calibrateFromImage=True)
     e1.record(stream)
     torch.cuda.synchronize()
     calib_ms = e0.elapsed_time(e1)
-
-    for _ in range(args.warmup):
-        proc.denoise(params, ycc, None, nlf)
-    torch.cuda.synchronize()
     e0.record(stream)
-    proc.denoise(params, ycc, None, nlf)
+    cal.denoise(params, ycc, None, nlf)
     e1.record(stream)
     torch.cuda.synchronize()
     nlf_fit_ms = max(0.0, calib_ms - e0.elapsed_time(e1))
+    cal.close()
 
-    # Timed region: exactly K steps; per-step events give the median, stage events
-    # (recorded on the same stream by the library) time the block-matching kernel.
-    proc.set_profiling(True)
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    evs[0].record(stream)
-    for k in range(args.steps):
-        proc.denoise(params, ycc, None, nlf)
-        evs[k + 1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    step_ms = [evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)]
-    total_ms = evs[0].elapsed_time(evs[-1])
-    stages = proc.stage_times()
-    proc.set_profiling(False)
+    prim = time_single(gcam, torch, w, h, levels, radius, args.precision, args.bm_rows, ycc, params, nlf,
+                       args.steps, args.warmup, dev)
+    ref32 = None
+    if args.precision != "fp32" and not args.no_compare:
+        ref32 = time_single(gcam, torch, w, h, levels, radius, "fp32", args.bm_rows, ycc, params, nlf, args.steps,
+                            args.warmup, dev)
 
-    ms_per_step = total_ms / args.steps
     mpix = w * h / 1e6
+    ms_per_step = prim["ms_per_step"]
     value = mpix / (ms_per_step / 1e3)
 
     # Dominant kernel: block matching, one launch per level.  Algorithmic bytes
     # per launch = 24 B (3 ch x FP32 in + 3 ch x FP32 out) per level pixel.
     level_px = [(w >> i) * (h >> i) for i in range(levels)]
-    bm_ms = stages["block_match"]
+    bm_ms = prim["stages"]["block_match"]
     bm_avg_ms = sum(bm_ms) / levels
     bm_bytes_per_launch = 24.0 * sum(level_px) / levels
     bm_gbs = bm_bytes_per_launch / (bm_avg_ms / 1e3) / 1e9
     b_alg = algorithmic_bytes_per_base_pixel(levels)
     pipeline_gbs = b_alg * w * h / (ms_per_step / 1e3) / 1e9
-    # VALU view of the same kernel: ~47 issue slots per (pixel, neighbour) incl. 2 quarter-rate exp
-    pairs = sum(level_px) * (2 * radius + 1) ** 2
-    valu_tops = pairs * 47 / (sum(bm_ms) / 1e3) / 1e12
-    valu_peak_tops = 256 * 4 * 16 * 2.4e9 / 1e12
 
-    cpu = None
+    host_img = ycc.cpu().numpy()
+    host_clean = clean.cpu().numpy()
+    host_nlf = np.array([[list(A), list(B)] for A, B in nlf], np.float64)
+    pr = [dict(luma=p.luma, chroma=p.chroma, chromaBoost=p.chromaBoost, gradientBoost=p.gradientBoost,
+               gradientThreshold=p.gradientThreshold, sharpening=p.sharpening) for p in params]
+    cpu, quality = None, None
     if not args.no_cpu_baseline:
         import oracle
         threads = args.cpu_threads or min(cpu_cores(), 16)
         oracle.set_threads(threads)
-        host_img = ycc.cpu().numpy()
-        host_nlf = np.array([[list(A), list(B)] for A, B in nlf], np.float64)
-        pr = [dict(luma=p.luma, chroma=p.chroma, chromaBoost=p.chromaBoost, gradientBoost=p.gradientBoost,
-                   gradientThreshold=p.gradientThreshold, sharpening=p.sharpening) for p in params]
         t = time.perf_counter()
         oracle.pyramid_denoise(host_img, pr, host_nlf, levels=levels, radius=radius)
         cpu_s = time.perf_counter() - t
@@ -146,6 +154,13 @@ def main() -> int:
                "sample": f"one full {w}x{h} frame (the benchmark frame), {levels} levels, "
                          f"{2 * radius + 1}x{2 * radius + 1} window, FP32 C++ oracle, {threads} threads, "
                          f"{cpu_s:.1f} s"}
+        if not args.no_quality:
+            ref64 = oracle.pyramid_denoise(host_img, pr, host_nlf, levels=levels, radius=radius,
+                                           dtype=np.float64).denoised[0]
+            p64, pgpu = psnr(ref64, host_clean), psnr(prim["out"], host_clean)
+            quality = {"psnr_fp64_oracle_db": round(p64, 4), f"psnr_{args.precision}_db": round(pgpu, 4),
+                       "psnr_loss_db": round(p64 - pgpu, 4), "psnr_noisy_db": round(psnr(host_img, host_clean), 4),
+                       "max_abs_vs_fp64": float(np.abs(prim["out"][..., :3] - ref64[..., :3]).max())}
 
     out = {
         "metric": METRIC,
@@ -158,7 +173,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}[args.precision],
+        "dtype": DTYPE[args.precision],
         "data": "synthetic",
         "config": {
             "workload": f"{w}x{h} RGGB 12-bit simplex scene, Poisson-Gaussian a={a:g} b={b:g}, bilinear demosaic "
@@ -178,16 +193,141 @@ def main() -> int:
             "avg_launch_ms": round(bm_avg_ms, 4),
             "bytes_per_launch": int(bm_bytes_per_launch),
         },
-        "valu": {"kernel": f"k_block_match<{radius}>", "achieved_tops": round(valu_tops, 2),
-                 "peak_tops": round(valu_peak_tops, 2), "frac": round(valu_tops / valu_peak_tops, 4)},
         "pipeline": {"B_alg_per_base_px": round(b_alg, 3), "achieved_GBs": round(pipeline_gbs, 2),
                      "hbm_frac": round(pipeline_gbs / HBM_PEAK_GBS, 5)},
-        "median_ms": round(statistics.median(step_ms), 4),
-        "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "median_ms": round(prim["median_ms"], 4),
+        "wall_ms_per_step": round(prim["wall_ms_per_step"], 4),
         "nlf_fit_ms": round(nlf_fit_ms, 3),
-        "stage_ms_last_step": {k: [round(x, 4) for x in v] for k, v in stages.items()},
+        "stage_ms_last_step": {k: [round(x, 4) for x in v] for k, v in prim["stages"].items()},
         "cpu_baseline": cpu,
     }
+    if ref32 is not None:
+        v32 = mpix / (ref32["ms_per_step"] / 1e3)
+        out["fp32_form"] = {"value": round(v32, 3), "ms_per_step": round(ref32["ms_per_step"], 4),
+                            "speedup_of_reported_form": round(value / v32, 4),
+                            "stage_ms_last_step": {k: [round(x, 4) for x in v] for k, v in ref32["stages"].items()}}
+    if quality is not None:
+        out["quality"] = quality
+    return out
+
+
+def burst(args) -> dict:
+    """The batched form: 32 frames, sizes drawn uniformly (seed 42) from four camera sizes, noise gain a
+    log-uniform in [1e-3, 1e-2]; one step = one gcam_burst_denoise call (one graph launch)."""
+    import numpy as np
+    import torch
+
+    import glasscamerapublic_amd as gcam
+
+    levels, radius, b = 5, 4, 2e-3
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(42)
+    n = args.burst_frames
+    sizes = [BURST_SIZES[int(k)] for k in rng.integers(0, len(BURST_SIZES), n)]
+    gains = np.exp(rng.uniform(np.log(1e-3), np.log(1e-2), n))
+    params = [gcam.DenoiseParameters() for _ in range(levels)]
+    frames, nlfs = [], []
+    cal = {}
+    for f, ((w, h), a) in enumerate(zip(sizes, gains)):
+        img = gcam.synth_frame(w, h, float(a), b, scene_seed=1234 + 7 * f, noise_seed=5678 + f, device=dev)
+        # per-frame noise model from the frame itself (MeasureYCbCrNLF per level), outside the timed region
+        if (w, h) not in cal:
+            cal[(w, h)] = gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius, precision=args.precision,
+                                                device=dev)
+        nl = [((1e-8,) * 3, (1e-8,) * 3) for _ in range(levels)]
+        cal[(w, h)].denoise(params, img, None, nl, calibrateFromImage=True)
+        frames.append(img)
+        nlfs.append(nl)
+    for p in cal.values():
+        p.close()
+    torch.cuda.synchronize()
+    bp = gcam.BurstProcessor(sizes, levels=levels, search_radius=radius, precision=args.precision, device=dev)
+    t = time.perf_counter()
+    bp.denoise(params, frames, nlfs)  # autotune + graph capture + first run
+    torch.cuda.synchronize()
+    first_call_s = time.perf_counter() - t
+    for _ in range(args.warmup):
+        bp.denoise(params, frames, nlfs)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for k in range(args.steps):
+        bp.denoise(params, frames, nlfs)
+        evs[k + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = [evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)]
+    ms_per_step = evs[0].elapsed_time(evs[-1]) / args.steps
+    mpix = sum(w * h for w, h in sizes) / 1e6
+    value = mpix / (ms_per_step / 1e3)
+    tun = bp.tuning
+    b_alg = algorithmic_bytes_per_base_pixel(levels)
+    pipeline_gbs = b_alg * mpix * 1e6 / (ms_per_step / 1e3) / 1e9
+    level_px = [sum((w >> i) * (h >> i) for w, h in sizes) for i in range(levels)]
+    bm_gbs = [24.0 * level_px[i] / (tun["level_ms"][i] / 1e3) / 1e9 if tun["level_ms"][i] > 0 else None
+              for i in range(levels)]
+    out = {
+        "metric": "Mpix/s, burst of 32 mixed-size frames, 5-level PCA denoise, one graph launch",
+        "value": round(value, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.precision],
+        "data": "synthetic",
+        "config": {"workload": f"burst of {n} frames, sizes uniform (seed 42) from "
+                               f"{', '.join(f'{w}x{h}' for w, h in BURST_SIZES)}, a log-uniform [1e-3, 1e-2], "
+                               f"b={b:g}, {levels} levels, {2 * radius + 1}x{2 * radius + 1} window",
+                   "frames": n, "total_mpix": round(mpix, 3), "precision": args.precision,
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "kernel": f"k_burst_block_match_lp<{radius}> (level 0)",
+                     "achieved": round(bm_gbs[0], 2) if bm_gbs[0] else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bm_gbs[0] / HBM_PEAK_GBS, 5) if bm_gbs[0] else None, "traffic": None},
+        "pipeline": {"B_alg_per_base_px": round(b_alg, 3), "achieved_GBs": round(pipeline_gbs, 2),
+                     "hbm_frac": round(pipeline_gbs / HBM_PEAK_GBS, 5)},
+        "tuning": tun,
+        "first_call_s": round(first_call_s, 3),
+        "median_ms": round(statistics.median(step_ms), 3),
+        "wall_ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "cpu_baseline": None,
+    }
+    bp.close()
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="12mp", choices=sorted(WORKLOADS) + ["burst32"])
+    ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16"])
+    ap.add_argument("--bm-rows", type=int, default=0, help="block-matching output rows per thread (0 = default)")
+    ap.add_argument("--burst-frames", type=int, default=32)
+    ap.add_argument("--no-compare", action="store_true", help="skip timing the FP32 form beside the reported one")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-quality", action="store_true", help="skip the PSNR check against the FP64 oracle")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: min(cores, 16))")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+    if args.workload == "burst32":
+        if args.precision == "fp32":
+            print("the burst form runs the precision form: --precision fp16 or bf16", file=sys.stderr)
+            return 2
+        out = burst(args)
+    else:
+        out = single_frame(args)
     print(json.dumps(out), flush=True)
     return 0
 

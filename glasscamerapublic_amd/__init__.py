@@ -9,9 +9,10 @@ from .pyramid import (DenoiseParameters, PyramidProcessor, basic_noise_statistic
                       block_matching_denoise_image, block_matching_denoise_image_lp, build_pca_space,
                       collect_patches, denoise_image, downsample_image, measure_ycbcr_nlf, pca_projection,
                       subtract_noise_image, synth_frame)
+from .burst import BurstProcessor  # noqa: F401
 
 __all__ = [
-    "GcamError", "lib", "DenoiseParameters", "PyramidProcessor", "basic_noise_statistics",
+    "GcamError", "lib", "DenoiseParameters", "PyramidProcessor", "BurstProcessor", "basic_noise_statistics",
     "block_matching_denoise_image", "block_matching_denoise_image_lp", "build_pca_space", "collect_patches",
     "denoise_image", "downsample_image", "measure_ycbcr_nlf", "pca_projection", "subtract_noise_image",
     "synth_frame",

@@ -32,7 +32,11 @@ EXPORTED = (
     "gcam_build_pca_space", "gcam_pca_projection", "gcam_block_matching_denoise_image",
     "gcam_block_matching_denoise_image_lp", "gcam_denoise_image",
     "gcam_basic_noise_statistics", "gcam_measure_ycbcr_nlf", "gcam_synth_frame",
+    "gcam_burst_create", "gcam_burst_destroy", "gcam_burst_denoise", "gcam_burst_output", "gcam_burst_tune",
+    "gcam_burst_get_tuning", "gcam_burst_set_tuning",
 )
+GCAM_MAX_LEVELS = 8
+GCAM_MAX_BURST = 64
 
 
 class GcamError(RuntimeError):
@@ -67,6 +71,11 @@ class SynthParamsC(ctypes.Structure):
     _fields_ = [("scene_seed", ctypes.c_uint64), ("noise_seed", ctypes.c_uint64), ("shot_gain", ctypes.c_float),
                 ("read_sigma", ctypes.c_float), ("lo", ctypes.c_float), ("hi", ctypes.c_float),
                 ("period", ctypes.c_float), ("octaves", ctypes.c_int32)]
+
+
+class BurstTuningC(ctypes.Structure):
+    _fields_ = [("rows_per_thread", ctypes.c_int32 * 8), ("grid", ctypes.c_int32 * 8),
+                ("schedule", ctypes.c_int32 * 8), ("level_ms", ctypes.c_float * 8), ("tuned", ctypes.c_int32)]
 
 
 _lib = None
@@ -105,6 +114,15 @@ def _declare(L: ctypes.CDLL) -> None:
     L.gcam_basic_noise_statistics.argtypes = [_IP, _IP, _P]
     L.gcam_measure_ycbcr_nlf.argtypes = [_IP, _IP, _F, ctypes.POINTER(NlfC), _P]
     L.gcam_synth_frame.argtypes = [_I, _I, ctypes.POINTER(SynthParamsC), _P, _IP, _IP, _P]
+    L.gcam_burst_create.argtypes = [_I, ctypes.POINTER(_I), ctypes.POINTER(_I), _I, ctypes.POINTER(ConfigC),
+                                    ctypes.POINTER(_P)]
+    L.gcam_burst_destroy.argtypes = [_P]
+    L.gcam_burst_destroy.restype = None
+    L.gcam_burst_denoise.argtypes = [_P, ctypes.POINTER(DenoiseParamsC), _IP, _IP, ctypes.POINTER(NlfC), _F, _P]
+    L.gcam_burst_tune.argtypes = [_P, ctypes.POINTER(DenoiseParamsC), _IP, _IP, ctypes.POINTER(NlfC), _F, _P]
+    L.gcam_burst_output.argtypes = [_P, _I, _I, _IP]
+    L.gcam_burst_get_tuning.argtypes = [_P, ctypes.POINTER(BurstTuningC)]
+    L.gcam_burst_set_tuning.argtypes = [_P, ctypes.POINTER(BurstTuningC)]
     for name in EXPORTED:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int:  # default restype: gcam_status

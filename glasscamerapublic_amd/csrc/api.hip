@@ -131,7 +131,10 @@ gcam_status gcam_pyramid_create(int32_t width, int32_t height, int32_t levels, c
     for (int i = 1; i < levels; i++) parts.push_back({(void**)&p->layer[i], sizeof(float4) * p->W[i] * p->H[i]});
     for (int i = 0; i < levels - 1; i++) parts.push_back({(void**)&p->sub[i], sizeof(float4) * p->W[i] * p->H[i]});
     for (int i = 0; i < levels; i++) parts.push_back({(void**)&p->den[i], sizeof(float4) * p->W[i] * p->H[i]});
-    parts.push_back({(void**)&p->feat, sizeof(float) * 8 * (size_t)width * height});
+    // Materialized features: only the FP32 PCA form reads them (the precision
+    // form projects inside the block-matching tiles).
+    if (cfg.precision == GCAM_PRECISION_FP32 && !cfg.bilateral)
+        parts.push_back({(void**)&p->feat, sizeof(float) * 8 * (size_t)width * height});
     parts.push_back({(void**)&p->moment_partials, sizeof(double) * kMomentSlots * kMoments});
     parts.push_back({(void**)&p->nlf_partials, sizeof(double) * kNlfBlocks * 13});
     parts.push_back({(void**)&p->nlf_state, sizeof(NlfFitState)});

@@ -24,6 +24,7 @@
 //    FP32 accumulation) -- exactly the squared distance of the rounded
 //    features up to FP32 rounding.  Weights and sums are FP32.
 #include "gcam_internal.hpp"
+#include "burst_tables.hpp"
 
 #include <cmath>
 #include <type_traits>
@@ -89,11 +90,13 @@ struct LpTile {
     static_assert(LN * 4 + FN * 36 <= 64 * 1024, "LDS tile too large");
 };
 
+// One output tile with origin (x0, y0); every thread of the workgroup calls
+// it (the LDS staging has workgroup barriers).
 template <int R, int PV, int TW, int TH, class H>
-__global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in, View<const float2> grad,
-                                                           const float* __restrict__ basis,
-                                                           const NlfDev* __restrict__ nlf, BmParams p,
-                                                           DirTable dir, View<float4> out) {
+__device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const View<const float2>& grad,
+                                           const float* __restrict__ basis, const NlfDev* __restrict__ nlf,
+                                           const BmParams& p, const DirTable& dir, const View<float4>& out, int x0,
+                                           int y0) {
     using T = LpTile<R, PV, TW, TH>;
     using P = Prec<H>;
     using h2 = typename P::h2;
@@ -102,7 +105,6 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
     __shared__ float4 ycn[T::FN];  // Y, Cb, Cr, |f_1..7|^2 (scaled, rounded)
     __shared__ float f0s[T::FN];   // component 0, FP32
     __shared__ h8 fh[T::FN];       // components 1..7 rounded, 0
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH * PV;
     const int tid = threadIdx.y * TW + threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
 
@@ -184,86 +186,122 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
     __syncthreads();
 
     const int x = x0 + threadIdx.x, yb = y0 + threadIdx.y * PV;
-    if (x >= out.w || yb >= out.h) return;
+    if (x < out.w && yb < out.h) {
 
-    h8 fp[PV];
-    float f0p[PV], kl[PV], k2[PV], cp[PV], sb[PV], sr[PV], sbc[PV], src[PV];
-    float fY[PV], fCb[PV], fCr[PV], nY[PV], nC[PV];
-    const float kw = -2.0f * kLog2e, kroot = sqrtf(2.0f * kLog2e);
+        h8 fp[PV];
+        float f0p[PV], kl[PV], k2[PV], cp[PV], sb[PV], sr[PV], sbc[PV], src[PV];
+        float fY[PV], fCb[PV], fCr[PV], nY[PV], nC[PV];
+        const float kw = -2.0f * kLog2e, kroot = sqrtf(2.0f * kLog2e);
 #pragma unroll
-    for (int k = 0; k < PV; k++) {
-        const int y = min(yb + k, out.h - 1);
-        const int c = (threadIdx.y * PV + k + R) * T::FW + threadIdx.x + R;
-        const float4 v = ycn[c];
-        fp[k] = fh[c];
-        f0p[k] = f0s[c];
-        float lens = 1.0f;
-        if (p.lensShading > 0.0f) {
-            const float cx = (float)(in.w / 2), cy = (float)(in.h / 2);
-            const float dx = (float)x - cx, dy = (float)y - cy;
-            const float d = sqrtf(dx * dx + dy * dy) / sqrtf(cx * cx + cy * cy);
-            lens = 1.0f + p.lensShading * d * d;
+        for (int k = 0; k < PV; k++) {
+            const int y = min(yb + k, out.h - 1);
+            const int c = (threadIdx.y * PV + k + R) * T::FW + threadIdx.x + R;
+            const float4 v = ycn[c];
+            fp[k] = fh[c];
+            f0p[k] = f0s[c];
+            float lens = 1.0f;
+            if (p.lensShading > 0.0f) {
+                const float cx = (float)(in.w / 2), cy = (float)(in.h / 2);
+                const float dx = (float)x - cx, dy = (float)y - cy;
+                const float d = sqrtf(dx * dx + dy * dy) / sqrtf(cx * cx + cy * cy);
+                lens = 1.0f + p.lensShading * d * d;
+            }
+            const float s0 = sqrtf(nlf->a[0] + nlf->b[0] * lens * v.x);
+            const float s1 = sqrtf(nlf->a[1] + nlf->b[1] * lens * v.x);
+            const float s2 = sqrtf(nlf->a[2] + nlf->b[2] * lens * v.x);
+            const float dm0 = 1.0f / (p.thr[0] * s0), dm1 = 1.0f / (p.thr[1] * s1), dm2 = 1.0f / (p.thr[2] * s2);
+            float mag = 0.0f;
+            if (grad.ptr) {
+                const float2 gg = grad.at(x, y);
+                mag = sqrtf(gg.x * gg.x + gg.y * gg.y);
+            }
+            const float edge = smoothstepf(2.0f, 16.0f, p.gradientThreshold * mag / s0);
+            const float lm = dm0 / ((1.0f + p.gradientBoost * edge) * scale);
+            kl[k] = -2.0f * kLog2e * lm * lm;  // per unit of scaled squared distance
+            k2[k] = -2.0f * kl[k];
+            cp[k] = kl[k] * v.w;
+            sb[k] = kroot * (dm1 / p.chromaBoost);
+            sr[k] = kroot * (dm2 / p.chromaBoost);
+            sbc[k] = sb[k] * v.y;
+            src[k] = sr[k] * v.z;
+            fY[k] = fCb[k] = fCr[k] = nY[k] = nC[k] = 0.0f;
         }
-        const float s0 = sqrtf(nlf->a[0] + nlf->b[0] * lens * v.x);
-        const float s1 = sqrtf(nlf->a[1] + nlf->b[1] * lens * v.x);
-        const float s2 = sqrtf(nlf->a[2] + nlf->b[2] * lens * v.x);
-        const float dm0 = 1.0f / (p.thr[0] * s0), dm1 = 1.0f / (p.thr[1] * s1), dm2 = 1.0f / (p.thr[2] * s2);
-        float mag = 0.0f;
-        if (grad.ptr) {
-            const float2 gg = grad.at(x, y);
-            mag = sqrtf(gg.x * gg.x + gg.y * gg.y);
-        }
-        const float edge = smoothstepf(2.0f, 16.0f, p.gradientThreshold * mag / s0);
-        const float lm = dm0 / ((1.0f + p.gradientBoost * edge) * scale);
-        kl[k] = -2.0f * kLog2e * lm * lm;  // per unit of scaled squared distance
-        k2[k] = -2.0f * kl[k];
-        cp[k] = kl[k] * v.w;
-        sb[k] = kroot * (dm1 / p.chromaBoost);
-        sr[k] = kroot * (dm2 / p.chromaBoost);
-        sbc[k] = sb[k] * v.y;
-        src[k] = sr[k] * v.z;
-        fY[k] = fCb[k] = fCr[k] = nY[k] = nC[k] = 0.0f;
-    }
 
 #pragma unroll 1
-    for (int dy = -R; dy <= R + PV - 1; dy++) {
-        const int rowq = (threadIdx.y * PV + R + dy) * T::FW + threadIdx.x + R;
+        for (int dy = -R; dy <= R + PV - 1; dy++) {
+            const int rowq = (threadIdx.y * PV + R + dy) * T::FW + threadIdx.x + R;
 #pragma unroll
-        for (int dx = -R; dx <= R; dx++) {
-            const int q = rowq + dx;
-            const h8 fq = fh[q];
-            const float4 qy = ycn[q];
-            const float f0q = f0s[q];
+            for (int dx = -R; dx <= R; dx++) {
+                const int q = rowq + dx;
+                const h8 fq = fh[q];
+                const float4 qy = ycn[q];
+                const float f0q = f0s[q];
 #pragma unroll
-            for (int k = 0; k < PV; k++) {
-                const int ddy = dy - k;
-                if (ddy < -R || ddy > R) continue;  // wave-uniform
-                const int o2 = dx * dx + ddy * ddy;
-                const float wd = dir.w[o2], lwd = dir.lw[o2];
-                float G = P::dot2(h2{fq[0], fq[1]}, h2{fp[k][0], fp[k][1]}, 0.0f);
-                G = P::dot2(h2{fq[2], fq[3]}, h2{fp[k][2], fp[k][3]}, G);
-                G = P::dot2(h2{fq[4], fq[5]}, h2{fp[k][4], fp[k][5]}, G);
-                G = P::dot2(h2{fq[6], fq[7]}, h2{fp[k][6], fp[k][7]}, G);
-                const float t0 = f0q - f0p[k];
-                const float wl =
-                    __builtin_amdgcn_exp2f(fmaf(kl[k] * t0, t0, fmaf(k2[k], G, fmaf(kl[k], qy.w, cp[k]))));
-                const float wY = wd * wl;
-                const float tb = fmaf(sb[k], qy.y, -sbc[k]), tr = fmaf(sr[k], qy.z, -src[k]);
-                float e = fmaf(-tb, tb, lwd);
-                e = fmaf(-tr, tr, e);
-                e = fmaf(kw * wl, wl, e);
-                const float wC = __builtin_amdgcn_exp2f(e);
-                fY[k] = fmaf(wY, qy.x, fY[k]);
-                nY[k] += wY;
-                fCb[k] = fmaf(wC, qy.y, fCb[k]);
-                fCr[k] = fmaf(wC, qy.z, fCr[k]);
-                nC[k] += wC;
+                for (int k = 0; k < PV; k++) {
+                    const int ddy = dy - k;
+                    if (ddy < -R || ddy > R) continue;  // wave-uniform
+                    const int o2 = dx * dx + ddy * ddy;
+                    const float wd = dir.w[o2], lwd = dir.lw[o2];
+                    float G = P::dot2(h2{fq[0], fq[1]}, h2{fp[k][0], fp[k][1]}, 0.0f);
+                    G = P::dot2(h2{fq[2], fq[3]}, h2{fp[k][2], fp[k][3]}, G);
+                    G = P::dot2(h2{fq[4], fq[5]}, h2{fp[k][4], fp[k][5]}, G);
+                    G = P::dot2(h2{fq[6], fq[7]}, h2{fp[k][6], fp[k][7]}, G);
+                    const float t0 = f0q - f0p[k];
+                    const float wl =
+                        __builtin_amdgcn_exp2f(fmaf(kl[k] * t0, t0, fmaf(k2[k], G, fmaf(kl[k], qy.w, cp[k]))));
+                    const float wY = wd * wl;
+                    const float tb = fmaf(sb[k], qy.y, -sbc[k]), tr = fmaf(sr[k], qy.z, -src[k]);
+                    float e = fmaf(-tb, tb, lwd);
+                    e = fmaf(-tr, tr, e);
+                    e = fmaf(kw * wl, wl, e);
+                    const float wC = __builtin_amdgcn_exp2f(e);
+                    fY[k] = fmaf(wY, qy.x, fY[k]);
+                    nY[k] += wY;
+                    fCb[k] = fmaf(wC, qy.y, fCb[k]);
+                    fCr[k] = fmaf(wC, qy.z, fCr[k]);
+                    nC[k] += wC;
+                }
             }
         }
-    }
 #pragma unroll
-    for (int k = 0; k < PV; k++) {
-        if (yb + k < out.h) out.at(x, yb + k) = make_float4(fY[k] / nY[k], fCb[k] / nC[k], fCr[k] / nC[k], nY[k]);
+        for (int k = 0; k < PV; k++) {
+            if (yb + k < out.h) out.at(x, yb + k) = make_float4(fY[k] / nY[k], fCb[k] / nC[k], fCr[k] / nC[k], nY[k]);
+        }
+    }
+}
+
+template <int R, int PV, int TW, int TH, class H>
+__global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in, View<const float2> grad,
+                                                           const float* __restrict__ basis,
+                                                           const NlfDev* __restrict__ nlf, BmParams p,
+                                                           DirTable dir, View<float4> out) {
+    bm_lp_tile<R, PV, TW, TH, H>(in, grad, basis, nlf, p, dir, out, blockIdx.x * TW, blockIdx.y * TH * PV);
+}
+
+// Burst form: a persistent grid walks the tiles of every frame of a level.
+// schedule 0: workgroup b takes tiles b, b + G, b + 2G, ... (G = grid size);
+// schedule 1: workgroups run on XCD b % 8, so XCD x takes the contiguous
+// x-th eighth of the tiles (neighbouring tiles, whose aprons overlap, share
+// that XCD's L2) and its workgroups stride through it.  Every workgroup
+// leaves the loop when its tiles run out.
+template <int R, int PV, int TW, int TH, class H>
+__global__ __launch_bounds__(TW* TH) void k_burst_block_match_lp(const BurstTable<BurstBmFrame>* __restrict__ tab,
+                                                                 DirTable dir, int schedule) {
+    const int total = tab->total, G = gridDim.x;
+    int first = blockIdx.x, step = G, end = total;
+    if (schedule == 1) {
+        constexpr int kXcd = 8;
+        const int x = blockIdx.x % kXcd, chunk = (total + kXcd - 1) / kXcd;
+        first = x * chunk + blockIdx.x / kXcd;
+        step = (G - x + kXcd - 1) / kXcd;
+        end = min(total, (x + 1) * chunk);
+    }
+    for (int item = first; item < end; item += step) {
+        const BurstBmFrame& f = tab->frame[burst_tile_frame(tab, item)];
+        const int lt = item - f.tile0;
+        bm_lp_tile<R, PV, TW, TH, H>(f.in, f.grad, f.basis, f.nlf, f.p, dir, f.out, (lt % f.tiles_x) * TW,
+                                     (lt / f.tiles_x) * (TH * PV));
+        __syncthreads();
     }
 }
 
@@ -317,7 +355,67 @@ gcam_status dispatch(View<const float4> in, View<const float2> grad, const float
     }
 }
 
+template <int R, int PV, int TW, int TH, class H>
+gcam_status launch_burst_shape(const BurstTable<BurstBmFrame>* dev, int total, const DirTable& dir,
+                               const BurstBmConfig& cfg, hipStream_t s) {
+    int grid = cfg.grid > 0 ? std::min(cfg.grid, total) : total;
+    if (cfg.schedule == 1 && grid < 8) grid = std::min(8, total);  // every XCD chunk needs a workgroup
+    if (grid < 1) return GCAM_OK;
+    k_burst_block_match_lp<R, PV, TW, TH, H><<<grid, dim3(TW, TH), 0, s>>>(dev, dir, cfg.schedule);
+    GCAM_LAUNCH_CHECK("burst blockMatchingDenoiseImage");
+    return GCAM_OK;
+}
+
+template <int R, class H>
+gcam_status launch_burst_r(const BurstTable<BurstBmFrame>* dev, int total, const DirTable& dir,
+                           const BurstBmConfig& cfg, hipStream_t s) {
+    if constexpr (R <= 6) {
+        if (cfg.rows == 4) return launch_burst_shape<R, 4, 32, 4, H>(dev, total, dir, cfg, s);
+        if (cfg.rows == 2) return launch_burst_shape<R, 2, 32, 8, H>(dev, total, dir, cfg, s);
+        return launch_burst_shape<R, 1, 32, 8, H>(dev, total, dir, cfg, s);
+    } else {
+        return launch_burst_shape<R, 1, 32, 4, H>(dev, total, dir, cfg, s);
+    }
+}
+
+template <class H>
+gcam_status burst_dispatch(const BurstTable<BurstBmFrame>* dev, int total, int radius, const BurstBmConfig& cfg,
+                           hipStream_t s) {
+    static const DirTable dir = make_dir_table();
+    switch (radius) {
+        case 1: return launch_burst_r<1, H>(dev, total, dir, cfg, s);
+        case 2: return launch_burst_r<2, H>(dev, total, dir, cfg, s);
+        case 3: return launch_burst_r<3, H>(dev, total, dir, cfg, s);
+        case 4: return launch_burst_r<4, H>(dev, total, dir, cfg, s);
+        case 5: return launch_burst_r<5, H>(dev, total, dir, cfg, s);
+        case 6: return launch_burst_r<6, H>(dev, total, dir, cfg, s);
+        case 7: return launch_burst_r<7, H>(dev, total, dir, cfg, s);
+        case 8: return launch_burst_r<8, H>(dev, total, dir, cfg, s);
+        case 9: return launch_burst_r<9, H>(dev, total, dir, cfg, s);
+        case 10: return launch_burst_r<10, H>(dev, total, dir, cfg, s);
+        default: return set_error(GCAM_ERR_INVALID_ARGUMENT, "search radius must be 1..10");
+    }
+}
+
 }  // namespace
+
+void burst_bm_tile(int radius, int rows, int* tw, int* th) {
+    // the shapes of launch_burst_r: 32 x 4 threads (radius > 6, one row each),
+    // 32 x 4 x 4 rows, 32 x 8 x 2 rows, 32 x 8 x 1 row
+    *tw = 32;
+    if (radius > 6) {
+        *th = 4;
+    } else {
+        *th = rows == 4 ? 16 : 8 * rows;
+    }
+}
+
+gcam_status launch_burst_block_match_lp(const BurstTable<BurstBmFrame>* dev, int total_tiles, int radius,
+                                        int precision, const BurstBmConfig& cfg, hipStream_t s) {
+    if (precision == GCAM_PRECISION_FP16) return burst_dispatch<_Float16>(dev, total_tiles, radius, cfg, s);
+    if (precision == GCAM_PRECISION_BF16) return burst_dispatch<__bf16>(dev, total_tiles, radius, cfg, s);
+    return set_error(GCAM_ERR_INVALID_ARGUMENT, "the burst block matching needs FP16 or BF16");
+}
 
 gcam_status launch_block_match_lp(View<const float4> in, View<const float2> grad, const float* basis,
                                   const NlfDev* nlf_level, const BmParams& p, int radius, int rows_per_thread,

@@ -13,6 +13,7 @@
 // parallel-ordered cyclic Jacobi, and the projection reads the basis from
 // device memory, so a whole denoise call stays stream-ordered.
 #include "gcam_internal.hpp"
+#include "burst_tables.hpp"
 
 namespace gcam {
 
@@ -138,10 +139,10 @@ __global__ __launch_bounds__(256) void k_buffer_moments(const float* patches, in
 // 14 x + t%14, and the 18 sums are added in index order (deterministic).
 constexpr int kRedEntries = 14, kRedLanes = 18;
 static_assert(kMoments % kRedEntries == 0, "moment reduction tiling");
-__global__ __launch_bounds__(256) void k_moments_reduce(const double* partials, int nblocks, double* mom) {
+__device__ void moments_reduce_body(int blk, const double* partials, int nblocks, double* mom) {
     __shared__ double part[kRedLanes][kRedEntries];
     const int t = threadIdx.x;
-    const int e = blockIdx.x * kRedEntries + t % kRedEntries, sub = t / kRedEntries;
+    const int e = blk * kRedEntries + t % kRedEntries, sub = t / kRedEntries;
     if (sub < kRedLanes) {
         double acc = 0.0;
         for (int b = sub; b < nblocks; b += kRedLanes) acc += partials[(size_t)b * kMoments + e];
@@ -151,8 +152,12 @@ __global__ __launch_bounds__(256) void k_moments_reduce(const double* partials, 
     if (t < kRedEntries) {
         double acc = 0.0;
         for (int q = 0; q < kRedLanes; q++) acc += part[q][t];
-        mom[blockIdx.x * kRedEntries + t] = acc;
+        mom[blk * kRedEntries + t] = acc;
     }
+}
+
+__global__ __launch_bounds__(256) void k_moments_reduce(const double* partials, int nblocks, double* mom) {
+    moments_reduce_body(blockIdx.x, partials, nblocks, mom);
 }
 
 // collectPatches materialized (per-operation entry point): slot = y*w/8 + x.
@@ -394,6 +399,29 @@ __global__ __launch_bounds__(256) void k_pca_solve(const double* moments, double
     pca_solve_body(moments, n, basis, eigvals, eig_full);
 }
 
+// Burst forms: moments of every frame of a level in one launch (frame f owns
+// workgroups block0 .. block0 + nblocks - 1), the reductions on a
+// (25 x frames) grid, one eigensolve workgroup per frame.
+__global__ __launch_bounds__(256) void k_burst_moments(const BurstTable<BurstMomFrame>* __restrict__ tab) {
+    const BurstMomFrame& f = tab->frame[burst_frame_of(tab, blockIdx.x)];
+    const int lb = blockIdx.x - f.block0;
+    const View<const float4> in = f.in;
+    const int gw = in.w / 8, gh = in.h / 8;
+    const int64_t npatch = (int64_t)gw * gh;
+    moments_body(lb, f.nblocks, npatch, f.partials + (size_t)lb * kMoments,
+                 [&](int64_t k, int c) { return image_patch_value(in, gw, npatch, k, c); });
+}
+
+__global__ __launch_bounds__(256) void k_burst_moments_reduce(const BurstTable<BurstMomFrame>* __restrict__ tab) {
+    const BurstMomFrame& f = tab->frame[blockIdx.y];
+    moments_reduce_body(blockIdx.x, f.partials, f.nblocks, f.partials + (size_t)kMomentBlocks * kMoments);
+}
+
+__global__ __launch_bounds__(256) void k_burst_pca_solve(const BurstTable<BurstMomFrame>* __restrict__ tab) {
+    const BurstMomFrame& f = tab->frame[blockIdx.x];
+    pca_solve_body(f.partials + (size_t)kMomentBlocks * kMoments, f.n, f.basis, f.eigvals, nullptr);
+}
+
 // pcaProjection: f[c] = sum_row basis[row][c] * Y(p + off(row)), rows j-outer /
 // i-inner, clamp-to-edge.  A 64 x 4 tile plus its 2-pixel apron of luma is
 // staged in LDS; the basis is wave-uniform and comes in through scalar loads.
@@ -466,6 +494,17 @@ gcam_status launch_pca_solve(double* partials, int nblocks, int64_t n, float* ba
     GCAM_LAUNCH_CHECK("PCA eigensolve");
     return GCAM_OK;
 }
+
+gcam_status launch_burst_pca(const BurstTable<BurstMomFrame>* dev, int total_blocks, int nframes, hipStream_t s) {
+    if (nframes < 1) return GCAM_OK;
+    k_burst_moments<<<total_blocks, 256, 0, s>>>(dev);
+    k_burst_moments_reduce<<<dim3(kMoments / kRedEntries, nframes), 256, 0, s>>>(dev);
+    k_burst_pca_solve<<<nframes, 256, 0, s>>>(dev);
+    GCAM_LAUNCH_CHECK("burst PCA basis");
+    return GCAM_OK;
+}
+
+int moment_block_count(int64_t npatch) { return moment_blocks(npatch); }
 
 gcam_status launch_pca_project(View<const float4> in, const float* basis, View<float> feat8, hipStream_t s) {
     k_pca_project<<<dim3((in.w + 63) / 64, (in.h + 3) / 4), dim3(64, 4), 0, s>>>(in, basis, feat8.ptr, feat8.stride);

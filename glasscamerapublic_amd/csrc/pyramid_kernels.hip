@@ -9,6 +9,7 @@
 // rows per wavefront so every load and store is a coalesced 16-byte (float4)
 // or 8-byte (float2) access; the neighbouring taps hit L1/L2.
 #include "gcam_internal.hpp"
+#include "burst_tables.hpp"
 
 namespace gcam {
 
@@ -65,9 +66,7 @@ __device__ __forceinline__ float2 bilerp2(const View<const float2>& im, const Ta
 // downsampleImageXYZ: output o averages 4 linear samples at texel positions
 // (o + k) * in/out, k in {0, 1}, per axis (the reference's +-1/2 output-texel
 // offsets around the output centre).
-__global__ __launch_bounds__(256) void k_downsample4(View<const float4> in, View<float4> out) {
-    const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
-    if (ox >= out.w || oy >= out.h) return;
+__device__ __forceinline__ void downsample4_px(const View<const float4>& in, const View<float4>& out, int ox, int oy) {
     const Tap ty0 = lin_tap(oy * in.h, out.h, in.h), ty1 = lin_tap((oy + 1) * in.h, out.h, in.h);
     const Tap tx0 = lin_tap(ox * in.w, out.w, in.w), tx1 = lin_tap((ox + 1) * in.w, out.w, in.w);
     float3 s = bilerp3(in, tx0, ty0);
@@ -81,10 +80,13 @@ __global__ __launch_bounds__(256) void k_downsample4(View<const float4> in, View
     out.at(ox, oy) = make_float4(0.25f * acc.x, 0.25f * acc.y, 0.25f * acc.z, 0.0f);
 }
 
-// downsampleImageXY, the 2-channel gradient version.
-__global__ __launch_bounds__(256) void k_downsample2(View<const float2> in, View<float2> out) {
+__global__ __launch_bounds__(256) void k_downsample4(View<const float4> in, View<float4> out) {
     const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
-    if (ox >= out.w || oy >= out.h) return;
+    if (ox < out.w && oy < out.h) downsample4_px(in, out, ox, oy);
+}
+
+// downsampleImageXY, the 2-channel gradient version.
+__device__ __forceinline__ void downsample2_px(const View<const float2>& in, const View<float2>& out, int ox, int oy) {
     const Tap ty0 = lin_tap(oy * in.h, out.h, in.h), ty1 = lin_tap((oy + 1) * in.h, out.h, in.h);
     const Tap tx0 = lin_tap(ox * in.w, out.w, in.w), tx1 = lin_tap((ox + 1) * in.w, out.w, in.w);
     float2 acc = bilerp2(in, tx0, ty0);
@@ -97,16 +99,19 @@ __global__ __launch_bounds__(256) void k_downsample2(View<const float2> in, View
     out.at(ox, oy) = make_float2(0.25f * acc.x, 0.25f * acc.y);
 }
 
+__global__ __launch_bounds__(256) void k_downsample2(View<const float2> in, View<float2> out) {
+    const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
+    if (ox < out.w && oy < out.h) downsample2_px(in, out, ox, oy);
+}
+
 // subtractNoiseImage: Laplacian-style reconstruct of level i from the noisy
 // and the denoised level i+1 (both upsampled bilinearly), with optional
 // gradient-driven sharpening.
-__global__ __launch_bounds__(256) void k_subtract_noise(View<const float4> in, View<const float4> in1,
-                                                        View<const float4> den1, View<const float2> grad,
-                                                        float luma_weight, float sharpening, float nlf_x,
-                                                        float nlf_y, const NlfDev* nlf_level, float thr_y,
-                                                        View<float4> out) {
-    const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
-    if (ox >= out.w || oy >= out.h) return;
+__device__ __forceinline__ void subtract_noise_px(const View<const float4>& in, const View<const float4>& in1,
+                                                  const View<const float4>& den1, const View<const float2>& grad,
+                                                  float luma_weight, float sharpening, float nlf_x, float nlf_y,
+                                                  const NlfDev* nlf_level, float thr_y, const View<float4>& out,
+                                                  int ox, int oy) {
     const Tap tyy = lin_tap((2 * oy + 1) * in1.h, 2 * out.h, in1.h);
     const Tap txx = lin_tap((2 * ox + 1) * in1.w, 2 * out.w, in1.w);
     const float4 ip = in.at(ox, oy);
@@ -135,6 +140,38 @@ __global__ __launch_bounds__(256) void k_subtract_noise(View<const float4> in, V
     o.z = d1.z + (dp.z - d1.z) * alpha;
     o.w = ip.w;
     out.at(ox, oy) = o;
+}
+
+__global__ __launch_bounds__(256) void k_subtract_noise(View<const float4> in, View<const float4> in1,
+                                                        View<const float4> den1, View<const float2> grad,
+                                                        float luma_weight, float sharpening, float nlf_x,
+                                                        float nlf_y, const NlfDev* nlf_level, float thr_y,
+                                                        View<float4> out) {
+    const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
+    if (ox < out.w && oy < out.h)
+        subtract_noise_px(in, in1, den1, grad, luma_weight, sharpening, nlf_x, nlf_y, nlf_level, thr_y, out, ox, oy);
+}
+
+// Burst forms: one launch for the level of every frame; workgroup b covers
+// the 64 x 4 block b - block0 of its frame (row-major, bx blocks a row).
+__global__ __launch_bounds__(256) void k_burst_downsample(const BurstTable<BurstDownFrame>* __restrict__ tab) {
+    const int b = blockIdx.x;
+    const BurstDownFrame& f = tab->frame[burst_frame_of(tab, b)];
+    const int lb = b - f.block0;
+    const int ox = (lb % f.bx) * 64 + threadIdx.x, oy = (lb / f.bx) * 4 + threadIdx.y;
+    if (ox < f.out.w && oy < f.out.h) {
+        downsample4_px(f.in, f.out, ox, oy);
+        if (f.gin.ptr) downsample2_px(f.gin, f.gout, ox, oy);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_burst_subtract(const BurstTable<BurstSubFrame>* __restrict__ tab) {
+    const int b = blockIdx.x;
+    const BurstSubFrame& f = tab->frame[burst_frame_of(tab, b)];
+    const int lb = b - f.block0;
+    const int ox = (lb % f.bx) * 64 + threadIdx.x, oy = (lb / f.bx) * 4 + threadIdx.y;
+    if (ox < f.out.w && oy < f.out.h)
+        subtract_noise_px(f.in, f.in1, f.den1, f.grad, 1.0f, f.sharpening, 0.0f, 0.0f, f.nlf, f.thr_y, f.out, ox, oy);
 }
 
 // basicNoiseStatistics: 5x5 mean of Y and population variances of Y, Cb, Cr.
@@ -407,6 +444,20 @@ gcam_status launch_set_nlf(const gcam_ycbcr_nlf* host_levels, int levels, NlfDev
         }
     k_set_nlf<<<1, 64, 0, s>>>(v, levels, dev);
     GCAM_LAUNCH_CHECK("set noise model");
+    return GCAM_OK;
+}
+
+gcam_status launch_burst_downsample(const BurstTable<BurstDownFrame>* dev, int total_blocks, hipStream_t s) {
+    if (total_blocks < 1) return GCAM_OK;
+    k_burst_downsample<<<total_blocks, dim3(64, 4), 0, s>>>(dev);
+    GCAM_LAUNCH_CHECK("burst downsampleImageXYZ");
+    return GCAM_OK;
+}
+
+gcam_status launch_burst_subtract(const BurstTable<BurstSubFrame>* dev, int total_blocks, hipStream_t s) {
+    if (total_blocks < 1) return GCAM_OK;
+    k_burst_subtract<<<total_blocks, dim3(64, 4), 0, s>>>(dev);
+    GCAM_LAUNCH_CHECK("burst subtractNoiseImage");
     return GCAM_OK;
 }
 

@@ -24,6 +24,7 @@ extern "C" {
 #define GCAM_ABI_VERSION 1
 #define GCAM_MAX_LEVELS 8
 #define GCAM_MAX_SEARCH_RADIUS 10
+#define GCAM_MAX_BURST 64
 
 typedef enum gcam_status {
     GCAM_OK = 0,
@@ -178,6 +179,57 @@ gcam_status gcam_pyramid_denoise_fwd(gcam_pyramid* pyramid, const gcam_denoise_p
                                      void* stream, gcam_image* out);
 gcam_status gcam_pyramid_denoise_bwd(gcam_pyramid* pyramid, const gcam_image* grad_output, float* grad_thresholds,
                                      void* stream);
+
+/* ---- batched form: a burst of frames of uneven sizes (no counterpart in the reference) ----
+ *
+ * A burst handle holds one pyramid per frame (frame sizes fixed at creation,
+ * in any mix) and denoises all frames with the PyramidProcessor::denoise
+ * algorithm (MetalPipeline/pyramid_processor.cpp:96-172) in one graph launch:
+ * every stage of a level covers all frames in one kernel launch, the block
+ * matching runs as one persistent kernel over the tiles of every frame, and
+ * the whole sequence is a hipGraph captured once per handle and replayed.
+ * Frames, parameters and noise models reach the captured kernels through a
+ * device-side table that each call refreshes (one host-to-device copy).
+ *
+ * The block matching launch (output rows per thread, persistent grid size,
+ * tile order) is picked per level by an autotuner that times the candidates
+ * on the first call's frames (or on gcam_burst_tune); gcam_burst_set_tuning
+ * replaces it with a fixed choice. */
+typedef struct gcam_burst gcam_burst;
+
+typedef struct gcam_burst_tuning {
+    int32_t rows_per_thread[GCAM_MAX_LEVELS]; /* 1, 2 or 4 (radius > 6: 1)              */
+    int32_t grid[GCAM_MAX_LEVELS];            /* persistent workgroups; 0 = one per tile */
+    int32_t schedule[GCAM_MAX_LEVELS];        /* 0: tiles in frame order, strided over the
+                                                 grid; 1: contiguous chunks per XCD     */
+    float level_ms[GCAM_MAX_LEVELS];          /* measured block-matching time (output)  */
+    int32_t tuned;                            /* 1 once chosen (output)                  */
+} gcam_burst_tuning;
+
+/* n_frames in 1..GCAM_MAX_BURST; widths/heights: host arrays of n_frames.
+ * config->precision FP16 or BF16 (the precision form); bilateral must be 0. */
+gcam_status gcam_burst_create(int32_t n_frames, const int32_t* widths, const int32_t* heights, int32_t levels,
+                              const gcam_config* config, gcam_burst** out);
+void gcam_burst_destroy(gcam_burst* burst);
+
+/* params: n_frames * levels (frame-major); frames: n_frames RGBA32F device
+ * images of the creation sizes; gradients: NULL or n_frames images (RG32F;
+ * an entry with data == NULL means no gradient for that frame); nlf:
+ * n_frames * levels noise models (frame-major).  The first call without a
+ * tuning autotunes (and synchronizes the stream); later calls are one
+ * host-to-device table copy plus one graph launch, asynchronous. */
+gcam_status gcam_burst_denoise(gcam_burst* burst, const gcam_denoise_params* params, const gcam_image* frames,
+                               const gcam_image* gradients, const gcam_ycbcr_nlf* nlf, float lens_shading_correction,
+                               void* stream);
+/* Denoised level `level` of frame `frame` (handle-owned, valid until the next call). */
+gcam_status gcam_burst_output(const gcam_burst* burst, int32_t frame, int32_t level, gcam_image* out);
+/* Run the autotuner on these inputs now (outputs are left valid); synchronizes. */
+gcam_status gcam_burst_tune(gcam_burst* burst, const gcam_denoise_params* params, const gcam_image* frames,
+                            const gcam_image* gradients, const gcam_ycbcr_nlf* nlf, float lens_shading_correction,
+                            void* stream);
+gcam_status gcam_burst_get_tuning(const gcam_burst* burst, gcam_burst_tuning* tuning);
+/* Fix the launch choice (validated); the graph is re-captured on the next call. */
+gcam_status gcam_burst_set_tuning(gcam_burst* burst, const gcam_burst_tuning* tuning);
 
 /* ---- per-operation entry points (MetalPipeline/demosaic_kernels.hpp wrappers) ---- */
 

@@ -1,0 +1,135 @@
+"""Batched (burst) form against the single-frame pyramid (GPU).
+
+A burst runs PyramidProcessor::denoise on every frame through batched
+launches, a persistent block-matching kernel and one captured graph.  With the
+block-matching tile fixed to the single-frame default, every output level of
+every frame must equal the single-frame result bit for bit (same kernels, same
+reduction order); with the autotuned tile the luma centring of the fused
+projection moves with the tile, so outputs agree to FP32 rounding.  The graph is
+replayed on new frames, with other GPU work between the replays, and every
+replay is checked.
+"""
+import numpy as np
+import pytest
+import torch
+
+import glasscamerapublic_amd as gcam
+from tests.helpers import nlf_for
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(640, 480), (512, 384), (768, 512), (384, 256)]
+LEVELS, RADIUS = 4, 4
+
+
+def frames_for(seed, gains):
+    return [gcam.synth_frame(w, h, a, 2e-3, scene_seed=seed + 10 * k, noise_seed=seed + 10 * k + 1)
+            for k, ((w, h), a) in enumerate(zip(SIZES, gains))]
+
+
+def frame_params(k):
+    lm = [4.0, 2.0, 1.5, 1.0]
+    return [gcam.DenoiseParameters(luma=lm[i] * (1 + 0.1 * k), chroma=1.0 + 0.2 * i, chromaBoost=[4, 2, 2, 1][i],
+                                   sharpening=1.0 if k % 2 == 0 else 1.3) for i in range(LEVELS)]
+
+
+def frame_nlf(a):
+    return [(tuple(x[0]), tuple(x[1])) for x in nlf_for(LEVELS, a, 2e-3)]
+
+
+def single_frame(frames, params, nlfs, precision="fp16"):
+    outs = []
+    for img, prm, nl, (w, h) in zip(frames, params, nlfs, SIZES):
+        proc = gcam.PyramidProcessor(w, h, levels=LEVELS, search_radius=RADIUS, precision=precision,
+                                     bm_rows_per_thread=2)
+        proc.denoise(prm, img, None, list(nl))
+        outs.append([t.clone() for t in proc.denoisedImagePyramid])
+        proc.close()
+    return outs
+
+
+def test_burst_matches_single_frame_bit_exact():
+    gains = [2e-3, 4e-3, 8e-3, 1e-3]
+    frames = frames_for(100, gains)
+    params = [frame_params(k) for k in range(len(SIZES))]
+    nlfs = [frame_nlf(a) for a in gains]
+    burst = gcam.BurstProcessor(SIZES, levels=LEVELS, search_radius=RADIUS, precision="fp16")
+    burst.set_tuning([2] * LEVELS, [0] * LEVELS, [0] * LEVELS)
+    burst.denoise(params, frames, nlfs)
+    torch.cuda.synchronize()
+    ref = single_frame(frames, params, nlfs)
+    for k in range(len(SIZES)):
+        for i in range(LEVELS):
+            got = burst.output(k, i)
+            assert torch.equal(got, ref[k][i]), f"frame {k} level {i}: max diff {(got - ref[k][i]).abs().max()}"
+
+
+@pytest.mark.parametrize("grid,schedule", [(256, 0), (512, 1), (0, 1)])
+def test_persistent_grid_and_schedule_are_bit_exact(grid, schedule):
+    gains = [3e-3] * 4
+    frames = frames_for(200, gains)
+    params = [frame_params(k) for k in range(len(SIZES))]
+    nlfs = [frame_nlf(a) for a in gains]
+    burst = gcam.BurstProcessor(SIZES, levels=LEVELS, search_radius=RADIUS, precision="fp16")
+    burst.set_tuning([2] * LEVELS, [0] * LEVELS, [0] * LEVELS)
+    burst.denoise(params, frames, nlfs)
+    torch.cuda.synchronize()
+    base = [[burst.output(k, i).clone() for i in range(LEVELS)] for k in range(len(SIZES))]
+    burst.set_tuning([2] * LEVELS, [grid] * LEVELS, [schedule] * LEVELS)
+    burst.denoise(params, frames, nlfs)
+    torch.cuda.synchronize()
+    for k in range(len(SIZES)):
+        for i in range(LEVELS):
+            assert torch.equal(burst.output(k, i), base[k][i]), f"frame {k} level {i}"
+
+
+def test_autotuned_burst_and_graph_replays():
+    gains = [1e-3, 5e-3, 2e-3, 9e-3]
+    params = [frame_params(k) for k in range(len(SIZES))]
+    nlfs = [frame_nlf(a) for a in gains]
+    burst = gcam.BurstProcessor(SIZES, levels=LEVELS, search_radius=RADIUS, precision="fp16")
+    for rep, seed in enumerate([300, 400, 500]):
+        frames = frames_for(seed, gains)
+        burst.denoise(params, frames, nlfs)
+        torch.cuda.synchronize()
+        tun = burst.tuning
+        assert tun["tuned"] and all(r in (1, 2, 4) for r in tun["rows_per_thread"])
+        print("replay", rep, "tuning", tun)
+        ref = single_frame(frames, params, nlfs)  # eager GPU work between the replays
+        for k in range(len(SIZES)):
+            for i in range(LEVELS):
+                got = burst.output(k, i)
+                err = (got[..., :3] - ref[k][i][..., :3]).abs().max().item()
+                assert err < 1e-4, f"replay {rep} frame {k} level {i}: {err}"
+
+
+def test_bf16_burst_matches_single_frame():
+    gains = [4e-3] * 4
+    frames = frames_for(600, gains)
+    params = [frame_params(k) for k in range(len(SIZES))]
+    nlfs = [frame_nlf(a) for a in gains]
+    burst = gcam.BurstProcessor(SIZES, levels=LEVELS, search_radius=RADIUS, precision="bf16")
+    burst.set_tuning([2] * LEVELS, [512] * LEVELS, [1] * LEVELS)
+    burst.denoise(params, frames, nlfs)
+    torch.cuda.synchronize()
+    ref = single_frame(frames, params, nlfs, precision="bf16")
+    for k in range(len(SIZES)):
+        assert torch.equal(burst.output(k, 0), ref[k][0])
+
+
+def test_burst_argument_errors():
+    with pytest.raises(gcam.GcamError):
+        gcam.BurstProcessor(SIZES, levels=LEVELS, search_radius=RADIUS, precision="fp32")
+    burst = gcam.BurstProcessor(SIZES, levels=LEVELS, search_radius=RADIUS, precision="fp16")
+    gains = [4e-3] * 4
+    frames = frames_for(700, gains)
+    params = [frame_params(k) for k in range(len(SIZES))]
+    nlfs = [frame_nlf(a) for a in gains]
+    bad = list(frames)
+    bad[1] = frames[0]  # wrong size for frame 1
+    with pytest.raises(gcam.GcamError, match="frames\\[1\\]"):
+        burst.denoise(params, bad, nlfs)
+    with pytest.raises(gcam.GcamError):
+        burst.set_tuning([3] * LEVELS, [0] * LEVELS, [0] * LEVELS)
+    with pytest.raises(ValueError):
+        burst.denoise(params[:2], frames, nlfs)
