@@ -35,6 +35,13 @@ WORKLOADS = {
     "48mp": (8064, 6048, 5, 4, 4e-3, 2e-3),
 }
 BURST_SIZES = [(3264, 2448), (4032, 3024), (5472, 3648), (8064, 6048)]
+# HBM bytes per level pixel that the block-matching launch of each level moves on the 12 MP workload
+# (9x9 window), levels 0..4: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+# what the counter reads for a plain 256 MiB copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
+MEASURED_BM_TRAFFIC = {
+    ("12mp", "fp16"): [58.1, 52.1, 40.4, 40.9, 45.6],
+    ("12mp", "fp32"): [111.7, 103.8, 81.7, 80.4, 80.1],
+}
 DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}
 
 
@@ -78,7 +85,7 @@ def time_single(gcam, torch, w, h, levels, radius, precision, bm_rows, ycc, para
     step_ms = [evs[k].elapsed_time(evs[k + 1]) for k in range(steps)]
     stages = proc.stage_times()
     proc.set_profiling(False)
-    out = proc.denoisedImagePyramid[0].cpu().numpy()
+    out = proc.denoisedImagePyramid[0].clone()
     proc.close()
     total_ms = evs[0].elapsed_time(evs[-1])
     return {"ms_per_step": total_ms / steps, "median_ms": statistics.median(step_ms),
@@ -134,17 +141,19 @@ def single_frame(args) -> dict:
     bm_avg_ms = sum(bm_ms) / levels
     bm_bytes_per_launch = 24.0 * sum(level_px) / levels
     bm_gbs = bm_bytes_per_launch / (bm_avg_ms / 1e3) / 1e9
+    coef = MEASURED_BM_TRAFFIC.get((args.workload, args.precision))
+    traffic = None if coef is None else round(sum(c * n for c, n in zip(coef, level_px)) / levels)
     b_alg = algorithmic_bytes_per_base_pixel(levels)
     pipeline_gbs = b_alg * w * h / (ms_per_step / 1e3) / 1e9
 
-    host_img = ycc.cpu().numpy()
-    host_clean = clean.cpu().numpy()
-    host_nlf = np.array([[list(A), list(B)] for A, B in nlf], np.float64)
-    pr = [dict(luma=p.luma, chroma=p.chroma, chromaBoost=p.chromaBoost, gradientBoost=p.gradientBoost,
-               gradientThreshold=p.gradientThreshold, sharpening=p.sharpening) for p in params]
     cpu, quality = None, None
     if not args.no_cpu_baseline:
         import oracle
+        host_img = ycc.cpu().numpy()
+        host_clean = clean.cpu().numpy()
+        host_nlf = np.array([[list(A), list(B)] for A, B in nlf], np.float64)
+        pr = [dict(luma=p.luma, chroma=p.chroma, chromaBoost=p.chromaBoost, gradientBoost=p.gradientBoost,
+                   gradientThreshold=p.gradientThreshold, sharpening=p.sharpening) for p in params]
         threads = args.cpu_threads or min(cpu_cores(), 16)
         oracle.set_threads(threads)
         t = time.perf_counter()
@@ -157,10 +166,11 @@ def single_frame(args) -> dict:
         if not args.no_quality:
             ref64 = oracle.pyramid_denoise(host_img, pr, host_nlf, levels=levels, radius=radius,
                                            dtype=np.float64).denoised[0]
-            p64, pgpu = psnr(ref64, host_clean), psnr(prim["out"], host_clean)
+            got = prim["out"].cpu().numpy()
+            p64, pgpu = psnr(ref64, host_clean), psnr(got, host_clean)
             quality = {"psnr_fp64_oracle_db": round(p64, 4), f"psnr_{args.precision}_db": round(pgpu, 4),
                        "psnr_loss_db": round(p64 - pgpu, 4), "psnr_noisy_db": round(psnr(host_img, host_clean), 4),
-                       "max_abs_vs_fp64": float(np.abs(prim["out"][..., :3] - ref64[..., :3]).max())}
+                       "max_abs_vs_fp64": float(np.abs(got[..., :3] - ref64[..., :3]).max())}
 
     out = {
         "metric": METRIC,
@@ -189,7 +199,7 @@ def single_frame(args) -> dict:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(bm_gbs / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": round(bm_avg_ms, 4),
             "bytes_per_launch": int(bm_bytes_per_launch),
         },
