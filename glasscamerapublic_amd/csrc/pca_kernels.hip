@@ -177,19 +177,6 @@ __global__ __launch_bounds__(256) void k_fill_zero(float* p, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) p[i] = 0.0f;
 }
 
-// FP64 reciprocal square root and reciprocal from the hardware estimates
-// (v_rsq_f64, v_rcp_f64) refined by two Newton steps each.
-__device__ __forceinline__ double rsqrt_f64(double x) {
-    double r = __builtin_amdgcn_rsq(x);
-    r = r * (1.5 - 0.5 * x * r * r);
-    return r * (1.5 - 0.5 * x * r * r);
-}
-__device__ __forceinline__ double rcp_f64(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    r = r * (2.0 - x * r);
-    return r * (2.0 - x * r);
-}
-
 // Covariance + cyclic Jacobi (parallel circle ordering) + basis selection.
 //
 // C = (sum x x^T - (sum x)(sum x)^T / n) / (n - 1)   -- PCA.cpp:31-32
@@ -204,10 +191,10 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
     constexpr int N = kPatch;
     constexpr int LD = N + 1;
     __shared__ double mom[kMoments];
-    __shared__ double A[N * LD];
+    __shared__ double A0[N * LD], A1[N * LD];  // current and next A
     __shared__ double V[N * LD];
-    __shared__ double2 rot[13];
     __shared__ double2 energy[4];
+    double *Ac = A0, *An = A1;
     const int tid = threadIdx.x;
 
     for (int e = tid; e < kMoments; e += 256) mom[e] = moments[e];
@@ -217,8 +204,8 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
         int r, s;
         moment_rs(e, r, s);
         const double c = (mom[e] - mom[r] * mom[s] / n) / (n - 1.0);
-        A[r * LD + s] = c;
-        A[s * LD + r] = c;
+        A0[r * LD + s] = c;
+        A0[s * LD + r] = c;
     }
     for (int e = tid; e < N * N; e += 256) {
         const int r = e / N, s = e % N;
@@ -232,14 +219,21 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
     // (identity).  J acts on the 2 x 2 blocks A[{p_u, q_u}][{p_v, q_v}]:
     // Y = J_u^T X J_v with J = [[c, s], [-s, c]] (rows/columns p, q), and the
     // pair entry of a diagonal block is set to zero.  Every entry belongs to
-    // exactly one block, so the update is in place: threads 0..90 own one
-    // block each (both triangles), threads 128..255 the 25 x 13 row pieces
-    // of V; lanes 0..11 of wavefront 0 compute the rotations.
+    // exactly one block: threads 0..90 (wavefronts 0 and 1) own one block each
+    // (both triangles), reading A from one buffer and writing the other, so a
+    // round needs one workgroup barrier.  Threads 128..244 (wavefronts 2 and 3)
+    // own the rows {j0, j0 + 9, j0 + 18} of V at one super-index v each and
+    // update them in place.  Every thread
+    // computes the rotations it needs from the round's A (bit-identical
+    // everywhere).  Rotation: the Golub & Van Loan angle (|phi| <= pi/4,
+    // tan(2 phi) = 2 a_pq / (a_qq - a_pp)) from its half-angle form,
+    // c^2 = (1 + |d| / h) / 2, s = sign(d e) |e| / (2 h c), h = hypot(d, e),
+    // d = a_qq - a_pp, e = 2 a_pq: one rsq for 1/h, one for 1/c.
     {
         constexpr int NU = N * (N + 1) / 2;  // upper triangle, a <= b
         constexpr int IU = (NU + 255) / 256;
         constexpr int NS = 13, NB = NS * (NS + 1) / 2;  // super-indices, blocks
-        constexpr int IV = (N * NS + 127) / 128;
+        constexpr int V0T = 128, NVT = NS * 9;           // V threads: wavefronts 2 and 3
         int ua[IU], ub[IU];
 #pragma unroll
         for (int m = 0; m < IU; m++) {
@@ -251,16 +245,20 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
             ua[m] = a;
             ub[m] = a + t;
         }
-        int bu = 0, bv = 0;  // block of threads < NB
-        {
-            int t = min(tid, NB - 1);
+        const bool is_block = tid < NB, is_v = tid >= V0T && tid < V0T + NVT;
+        int bu = NS - 1, bv = NS - 1;  // block of a block thread; bu = the super-index of a V thread
+        if (is_block) {
+            int t = tid;
+            bu = 0;
             while (t >= NS - bu) {
                 t -= NS - bu;
                 bu++;
             }
             bv = bu + t;
+        } else if (is_v) {
+            bu = (tid - V0T) % NS;
         }
-        if (tid == 0) rot[NS - 1] = make_double2(1.0, 0.0);
+        const int vj0 = is_v ? (tid - V0T) / NS : 0;  // rows vj0, vj0 + 9, vj0 + 18
         for (int sweep = 0; sweep < 40; sweep++) {
             // Off-diagonal and total energy: per-thread entries, wavefront
             // sums, then the 4 wavefront sums in a fixed order (uniform).
@@ -268,7 +266,7 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
 #pragma unroll
             for (int m = 0; m < IU; m++) {
                 if (tid + 256 * m < NU) {
-                    const double x = A[ua[m] * LD + ub[m]];
+                    const double x = Ac[ua[m] * LD + ub[m]];
                     if (ua[m] == ub[m]) {
                         tot += x * x;
                     } else {
@@ -284,6 +282,7 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
             __syncthreads();
             off = energy[0].x + energy[1].x + energy[2].x + energy[3].x;
             tot = energy[0].y + energy[1].y + energy[2].y + energy[3].y + off;
+            __syncthreads();  // energy[] is rewritten by the next sweep
             if (off == 0.0 || off <= kJacobiTol * tot) break;
 
             for (int round = 0; round < N; round++) {
@@ -297,82 +296,97 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
                     p = min(i, j);
                     q = max(i, j);
                 };
-                if (tid < NS - 1) {
-                    int p, q;
-                    pair_of(tid, p, q);
-                    // t = sign(theta) / (|theta| + sqrt(theta^2 + 1)), theta = d / e with
-                    // d = a_qq - a_pp, e = 2 a_pq:  t = sign(d e) |e| / (|d| + hypot(d, e)).
-                    const double e = 2.0 * A[p * LD + q], d = A[q * LD + q] - A[p * LD + p];
-                    double c = 1.0, s = 0.0;
-                    if (e != 0.0) {
-                        const double h2 = d * d + e * e;
-                        const double hyp = h2 * rsqrt_f64(h2);
-                        double t = fabs(e) * rcp_f64(fabs(d) + hyp);
-                        if (d * e < 0.0) t = -t;
-                        c = rsqrt_f64(1.0 + t * t);
-                        s = t * c;
-                    }
-                    rot[tid] = make_double2(c, s);
-                }
-                __syncthreads();
-                if (tid < NB) {
+                // Straight-line code (no select on the expensive operands, which
+                // the compiler would turn into branches).  The identity (p == q,
+                // or a_pq == 0) is blended in arithmetically: c + m (1 - c) = 1 and
+                // s (1 - m) = 0 exactly for m = 1 (1 - c is exact for c in [1/2, 1]).
+                // rotation2 advances two independent rotations step by step, so
+                // their dependent FP64 chains overlap in one wavefront.  Each
+                // reciprocal square root is the hardware estimate (v_rsq_f64)
+                // refined by two Newton steps y (1.5 - 0.5 x y^2), written as
+                // y * fma(x y, -0.5 y, 1.5).
+                auto rotation2 = [&](bool id0, double app0, double aqq0, double apq0, bool id1, double app1,
+                                     double aqq1, double apq1, double2& r0, double2& r1) {
+                    const double e0 = 2.0 * apq0, d0 = aqq0 - app0, e1 = 2.0 * apq1, d1 = aqq1 - app1;
+                    const double m0 = (id0 || e0 == 0.0) ? 1.0 : 0.0, m1 = (id1 || e1 == 0.0) ? 1.0 : 0.0;
+                    double h0 = fma(d0, d0, e0 * e0) + 1e-300, h1 = fma(d1, d1, e1 * e1) + 1e-300;
+                    double y0 = __builtin_amdgcn_rsq(h0), y1 = __builtin_amdgcn_rsq(h1);  // 1 / h
+                    y0 = y0 * fma(h0 * y0, -0.5 * y0, 1.5);
+                    y1 = y1 * fma(h1 * y1, -0.5 * y1, 1.5);
+                    y0 = y0 * fma(h0 * y0, -0.5 * y0, 1.5);
+                    y1 = y1 * fma(h1 * y1, -0.5 * y1, 1.5);
+                    const double c20 = 0.5 * fma(fabs(d0), y0, 1.0), c21 = 0.5 * fma(fabs(d1), y1, 1.0);
+                    double z0 = __builtin_amdgcn_rsq(c20), z1 = __builtin_amdgcn_rsq(c21);  // 1 / c
+                    z0 = z0 * fma(c20 * z0, -0.5 * z0, 1.5);
+                    z1 = z1 * fma(c21 * z1, -0.5 * z1, 1.5);
+                    z0 = z0 * fma(c20 * z0, -0.5 * z0, 1.5);
+                    z1 = z1 * fma(c21 * z1, -0.5 * z1, 1.5);
+                    const double c0 = c20 * z0, c1 = c21 * z1;
+                    const double s0 = copysign((0.5 * fabs(e0) * y0) * z0, d0 * e0);
+                    const double s1 = copysign((0.5 * fabs(e1) * y1) * z1, d1 * e1);
+                    r0 = make_double2(fma(m0, 1.0 - c0, c0), s0 * (1.0 - m0));
+                    r1 = make_double2(fma(m1, 1.0 - c1, c1), s1 * (1.0 - m1));
+                };
+                if (is_block) {
                     int pu, qu, pv, qv;
                     pair_of(bu, pu, qu);
                     pair_of(bv, pv, qv);
-                    const double2 ru = rot[bu], rv = rot[bv];
-                    const double x00 = A[pu * LD + pv], x01 = A[pu * LD + qv];
-                    const double x10 = A[qu * LD + pv], x11 = A[qu * LD + qv];
+                    const double uu0 = Ac[pu * LD + pu], uu1 = Ac[qu * LD + qu], uu2 = Ac[pu * LD + qu];
+                    const double vv0 = Ac[pv * LD + pv], vv1 = Ac[qv * LD + qv], vv2 = Ac[pv * LD + qv];
+                    const double x00 = Ac[pu * LD + pv], x01 = Ac[pu * LD + qv];
+                    const double x10 = Ac[qu * LD + pv], x11 = Ac[qu * LD + qv];
+                    double2 ru, rv;  // rv = ru on a diagonal block
+                    rotation2(pu == qu, uu0, uu1, uu2, pv == qv, vv0, vv1, vv2, ru, rv);
                     // Z = X J_v, Y = J_u^T Z
                     const double z00 = x00 * rv.x - x01 * rv.y, z01 = x00 * rv.y + x01 * rv.x;
                     const double z10 = x10 * rv.x - x11 * rv.y, z11 = x10 * rv.y + x11 * rv.x;
-                    const double y00 = ru.x * z00 - ru.y * z10, y01 = ru.x * z01 - ru.y * z11;
-                    double y10 = ru.y * z00 + ru.x * z10;
-                    const double y11 = ru.y * z01 + ru.x * z11;
-                    double y01w = y01;
-                    if (bu == bv && bu != NS - 1) y01w = y10 = 0.0;  // the annihilated pair
-                    A[pu * LD + pv] = y00;
-                    A[pu * LD + qv] = y01w;
-                    A[qu * LD + pv] = y10;
-                    A[qu * LD + qv] = y11;
-                    A[pv * LD + pu] = y00;
-                    A[qv * LD + pu] = y01w;
-                    A[pv * LD + qu] = y10;
-                    A[qv * LD + qu] = y11;
-                } else if (tid >= 128) {
-                    double2 rr[IV];
-                    double v0[IV], v1[IV];
-                    int ip[IV], iq[IV];
+                    const double y00 = ru.x * z00 - ru.y * z10, y11 = ru.y * z01 + ru.x * z11;
+                    double y01 = ru.x * z01 - ru.y * z11, y10 = ru.y * z00 + ru.x * z10;
+                    if (bu == bv && bu != NS - 1) y01 = y10 = 0.0;  // the annihilated pair
+                    An[pu * LD + pv] = y00;
+                    An[pu * LD + qv] = y01;
+                    An[qu * LD + pv] = y10;
+                    An[qu * LD + qv] = y11;
+                    An[pv * LD + pu] = y00;
+                    An[qv * LD + pu] = y01;
+                    An[pv * LD + qu] = y10;
+                    An[qv * LD + qu] = y11;
+                } else if (is_v) {
+                    int p, q;
+                    pair_of(bu, p, q);
+                    const double a0 = Ac[p * LD + p], a1 = Ac[q * LD + q], a2 = Ac[p * LD + q];
+                    double v0[3], v1[3];
 #pragma unroll
-                    for (int m = 0; m < IV; m++) {
-                        const int it = min(tid - 128 + 128 * m, N * NS - 1);
-                        const int j = it / NS, v = it % NS;
-                        int p, q;
-                        pair_of(v, p, q);
-                        ip[m] = j * LD + p;
-                        iq[m] = j * LD + q;
-                        rr[m] = rot[v];
-                        v0[m] = V[ip[m]];
-                        v1[m] = V[iq[m]];
+                    for (int m = 0; m < 3; m++) {
+                        const int j = min(vj0 + 9 * m, N - 1);
+                        v0[m] = V[j * LD + p];
+                        v1[m] = V[j * LD + q];
                     }
+                    double2 r, r_unused;
+                    rotation2(p == q, a0, a1, a2, p == q, a0, a1, a2, r, r_unused);
 #pragma unroll
-                    for (int m = 0; m < IV; m++) {
-                        if (tid - 128 + 128 * m < N * NS) {
-                            V[ip[m]] = rr[m].x * v0[m] - rr[m].y * v1[m];
-                            V[iq[m]] = rr[m].y * v0[m] + rr[m].x * v1[m];
+                    for (int m = 0; m < 3; m++) {
+                        const int j = vj0 + 9 * m;
+                        if (j < N) {
+                            V[j * LD + p] = r.x * v0[m] - r.y * v1[m];
+                            V[j * LD + q] = r.y * v0[m] + r.x * v1[m];
                         }
                     }
                 }
                 __syncthreads();
+                double* sw = Ac;
+                Ac = An;
+                An = sw;
             }
         }
     }
     __syncthreads();
 
     if (tid < N) {
-        const double di = A[tid * LD + tid];
+        const double di = Ac[tid * LD + tid];
         int rank = 0;
         for (int j = 0; j < N; j++) {
-            const double dj = A[j * LD + j];
+            const double dj = Ac[j * LD + j];
             rank += (dj > di || (dj == di && j < tid)) ? 1 : 0;
         }
         if (eigvals) eigvals[rank] = di;
