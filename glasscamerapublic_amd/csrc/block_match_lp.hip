@@ -7,10 +7,12 @@
 // One kernel per level computes the PCA features of its tile in-workgroup
 // (they never touch HBM) and then the weighted window average:
 //
-// 1. Luma of the tile plus an (R+2)-pixel apron is staged in LDS, centred on
-//    the tile's first pixel and scaled by the level scale s (~1/sigma of the
-//    level).  Feature distances are invariant to the centring; the scale
-//    divides the luma coefficient by s^2.
+// 1. Luma of the tile plus an (R+2)-pixel apron is staged in LDS as 16-bit
+//    values (BF16: high part and rounding remainder), centred on the tile's
+//    first pixel and scaled by the level scale s (~1/sigma of the level);
+//    the same pass stages the YCbCr of the tile plus its R-pixel apron.
+//    Feature distances are invariant to the centring; the scale divides the
+//    luma coefficient by s^2.
 // 2. Projection on MFMA, transposed (F^T = B^T P^T): v_mfma_f32_16x16x32_{f16,
 //    bf16}, (8 components padded to 16) x (25 patch values padded to K = 32)
 //    x 16 pixels, FP32 accumulation, so each lane ends with 4 consecutive
@@ -21,8 +23,10 @@
 //    distances: it stays FP32.  Components 1..7 are rounded to FP16/BF16 and
 //    their distance is taken as a Gram form |f_p|^2 + |f_q|^2 - 2 f_p.f_q, the
 //    dot product as 4 packed dot2 (v_dot2c_f32_{f16,bf16}: 16-bit products,
-//    FP32 accumulation) -- exactly the squared distance of the rounded
-//    features up to FP32 rounding.  Weights and sums are FP32.
+//    FP32 accumulation) started from -|f_q|^2 / 2 (kept in LDS), so the luma
+//    exponent is k2 (f_p.f_q - |f_q|^2/2) + kl |f_p|^2 + kl t0^2 -- exactly the
+//    squared distance of the rounded features up to FP32 rounding.  Weights
+//    and sums are FP32; reciprocals are the hardware v_rcp_f32 (1 ulp).
 #include "gcam_internal.hpp"
 #include "burst_tables.hpp"
 
@@ -101,8 +105,10 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
     using P = Prec<H>;
     using h2 = typename P::h2;
     using h8 = typename P::h8;
-    __shared__ float lum[T::LN];   // centred, scaled luma
-    __shared__ float4 ycn[T::FN];  // Y, Cb, Cr, |f_1..7|^2 (scaled, rounded)
+    constexpr bool kSplit = !std::is_same<H, _Float16>::value;
+    __shared__ H lum_hi[T::LN];                 // centred, scaled luma, rounded
+    __shared__ H lum_lo[kSplit ? T::LN : 1];    // BF16: its rounding remainder
+    __shared__ float4 ycn[T::FN];  // Y, Cb, Cr, -|f_1..7|^2 / 2 (scaled, rounded)
     __shared__ float f0s[T::FN];   // component 0, FP32
     __shared__ h8 fh[T::FN];       // components 1..7 rounded, 0
     const int tid = threadIdx.y * TW + threadIdx.x;
@@ -113,26 +119,31 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
     const float scale = sref > 0.0f ? 1.0f / sref : 1.0f;
     const float mu = in.at(clampi(x0, 0, in.w - 1), clampi(y0, 0, in.h - 1)).x;
 
+    // One pass over the luma region (tile + (R+2)-pixel apron) stages both the
+    // 16-bit luma of the projection and, inside the R-pixel apron, the YCbCr.
     for (int e = tid; e < T::LN; e += T::THREADS) {
         const int ly = e / T::LW, lx = e % T::LW;
-        const float yv = in.at(clampi(x0 - R - 2 + lx, 0, in.w - 1), clampi(y0 - R - 2 + ly, 0, in.h - 1)).x;
-        lum[e] = (yv - mu) * scale;
-    }
-    for (int e = tid; e < T::FN; e += T::THREADS) {
-        const int fy = e / T::FW, fx = e % T::FW;
-        const float4 v = in.at(clampi(x0 - R + fx, 0, in.w - 1), clampi(y0 - R + fy, 0, in.h - 1));
-        ycn[e] = make_float4(v.x, v.y, v.z, 0.0f);
+        const float4 v = in.at(clampi(x0 - R - 2 + lx, 0, in.w - 1), clampi(y0 - R - 2 + ly, 0, in.h - 1));
+        const float yv = (v.x - mu) * scale;
+        const H hi = (H)yv;
+        lum_hi[e] = hi;
+        if (kSplit) lum_lo[e] = (H)(yv - (float)hi);
+        const int fx = lx - 2, fy = ly - 2;
+        if (fx >= 0 && fx < T::FW && fy >= 0 && fy < T::FH) ycn[fy * T::FW + fx] = make_float4(v.x, v.y, v.z, 0.0f);
     }
     // A operand of the projection (constant): the transposed basis, lane holds
     // basis[k][i] for component i = lane % 16, k = 8 (lane / 16) + v; zero
-    // outside 25 x 8.
+    // outside 25 x 8.  koff: LDS offset of patch value k from the patch's
+    // top-left corner (k >= 25 reads any valid value: its basis row is zero).
     h8 aop;
+    int koff[8];
     {
         const int i = lane & 15, kg = lane >> 4;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
-            const int k = 8 * kg + v;
+            const int k = 8 * kg + v, kc = min(k, kPatch - 1);
             aop[v] = (H)((k < kPatch && i < kComponents) ? basis[k * kComponents + i] : 0.0f);
+            koff[v] = (kc / 5) * T::LW + kc % 5;
         }
     }
     __syncthreads();
@@ -142,20 +153,17 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
     // lane % 16 of the group; the patch is centred on the pixel's clamped
     // image position (clamp-to-edge, like the FP32 projection).  BF16 adds the
     // rounding remainder of each luma value as a second product.
-    constexpr bool kSplit = !std::is_same<H, _Float16>::value;
     for (int g = wave; g < T::GROUPS; g += T::WAVES) {
         const int j = lane & 15, kg = lane >> 4;
         const int e = min(g * 16 + j, T::FN - 1);
         const int cx = clampi(x0 - R + e % T::FW, 0, in.w - 1) - (x0 - R);
         const int cy = clampi(y0 - R + e / T::FW, 0, in.h - 1) - (y0 - R);
+        const int base = cy * T::LW + cx;
         h8 bhi, blo;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
-            const int k = 8 * kg + v;
-            const float yv = k < kPatch ? lum[(cy + k / 5) * T::LW + cx + k % 5] : 0.0f;
-            const H hi = (H)yv;
-            bhi[v] = hi;
-            if (kSplit) blo[v] = (H)(yv - (float)hi);
+            bhi[v] = lum_hi[base + koff[v]];
+            if (kSplit) blo[v] = lum_lo[base + koff[v]];
         }
         f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
         c = P::mfma(aop, bhi, c);
@@ -179,7 +187,7 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
             reinterpret_cast<h4*>(&fh[er])[kg] = hv;
             if (kg == 0) {
                 f0s[er] = c[0];
-                ycn[er].w = n;
+                ycn[er].w = -0.5f * n;
             }
         }
     }
@@ -187,11 +195,11 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
 
     const int x = x0 + threadIdx.x, yb = y0 + threadIdx.y * PV;
     if (x < out.w && yb < out.h) {
-
         h8 fp[PV];
         float f0p[PV], kl[PV], k2[PV], cp[PV], sb[PV], sr[PV], sbc[PV], src[PV];
         float fY[PV], fCb[PV], fCr[PV], nY[PV], nC[PV];
         const float kw = -2.0f * kLog2e, kroot = sqrtf(2.0f * kLog2e);
+        const float rcb = __builtin_amdgcn_rcpf(p.chromaBoost), rscale = sref > 0.0f ? sref : 1.0f;
 #pragma unroll
         for (int k = 0; k < PV; k++) {
             const int y = min(yb + k, out.h - 1);
@@ -209,19 +217,20 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
             const float s0 = sqrtf(nlf->a[0] + nlf->b[0] * lens * v.x);
             const float s1 = sqrtf(nlf->a[1] + nlf->b[1] * lens * v.x);
             const float s2 = sqrtf(nlf->a[2] + nlf->b[2] * lens * v.x);
-            const float dm0 = 1.0f / (p.thr[0] * s0), dm1 = 1.0f / (p.thr[1] * s1), dm2 = 1.0f / (p.thr[2] * s2);
+            const float dm0 = __builtin_amdgcn_rcpf(p.thr[0] * s0);
+            const float dm1 = __builtin_amdgcn_rcpf(p.thr[1] * s1), dm2 = __builtin_amdgcn_rcpf(p.thr[2] * s2);
             float mag = 0.0f;
             if (grad.ptr) {
                 const float2 gg = grad.at(x, y);
                 mag = sqrtf(gg.x * gg.x + gg.y * gg.y);
             }
             const float edge = smoothstepf(2.0f, 16.0f, p.gradientThreshold * mag / s0);
-            const float lm = dm0 / ((1.0f + p.gradientBoost * edge) * scale);
+            const float lm = dm0 * rscale * __builtin_amdgcn_rcpf(1.0f + p.gradientBoost * edge);
             kl[k] = -2.0f * kLog2e * lm * lm;  // per unit of scaled squared distance
             k2[k] = -2.0f * kl[k];
-            cp[k] = kl[k] * v.w;
-            sb[k] = kroot * (dm1 / p.chromaBoost);
-            sr[k] = kroot * (dm2 / p.chromaBoost);
+            cp[k] = k2[k] * v.w;               // kl |f_p|^2 (v.w = -|f_p|^2 / 2)
+            sb[k] = kroot * (dm1 * rcb);
+            sr[k] = kroot * (dm2 * rcb);
             sbc[k] = sb[k] * v.y;
             src[k] = sr[k] * v.z;
             fY[k] = fCb[k] = fCr[k] = nY[k] = nC[k] = 0.0f;
@@ -242,13 +251,14 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
                     if (ddy < -R || ddy > R) continue;  // wave-uniform
                     const int o2 = dx * dx + ddy * ddy;
                     const float wd = dir.w[o2], lwd = dir.lw[o2];
-                    float G = P::dot2(h2{fq[0], fq[1]}, h2{fp[k][0], fp[k][1]}, 0.0f);
-                    G = P::dot2(h2{fq[2], fq[3]}, h2{fp[k][2], fp[k][3]}, G);
-                    G = P::dot2(h2{fq[4], fq[5]}, h2{fp[k][4], fp[k][5]}, G);
-                    G = P::dot2(h2{fq[6], fq[7]}, h2{fp[k][6], fp[k][7]}, G);
+                    // acc = f_p . f_q - |f_q|^2 / 2 over components 1..7, so that
+                    // k2 acc + kl |f_p|^2 = kl |f_q - f_p|^2 (Gram form)
+                    float acc = P::dot2(h2{fq[0], fq[1]}, h2{fp[k][0], fp[k][1]}, qy.w);
+                    acc = P::dot2(h2{fq[2], fq[3]}, h2{fp[k][2], fp[k][3]}, acc);
+                    acc = P::dot2(h2{fq[4], fq[5]}, h2{fp[k][4], fp[k][5]}, acc);
+                    acc = P::dot2(h2{fq[6], fq[7]}, h2{fp[k][6], fp[k][7]}, acc);
                     const float t0 = f0q - f0p[k];
-                    const float wl =
-                        __builtin_amdgcn_exp2f(fmaf(kl[k] * t0, t0, fmaf(k2[k], G, fmaf(kl[k], qy.w, cp[k]))));
+                    const float wl = __builtin_amdgcn_exp2f(fmaf(k2[k], acc, fmaf(kl[k] * t0, t0, cp[k])));
                     const float wY = wd * wl;
                     const float tb = fmaf(sb[k], qy.y, -sbc[k]), tr = fmaf(sr[k], qy.z, -src[k]);
                     float e = fmaf(-tb, tb, lwd);
@@ -265,7 +275,8 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
         }
 #pragma unroll
         for (int k = 0; k < PV; k++) {
-            if (yb + k < out.h) out.at(x, yb + k) = make_float4(fY[k] / nY[k], fCb[k] / nC[k], fCr[k] / nC[k], nY[k]);
+            const float rY = __builtin_amdgcn_rcpf(nY[k]), rC = __builtin_amdgcn_rcpf(nC[k]);
+            if (yb + k < out.h) out.at(x, yb + k) = make_float4(fY[k] * rY, fCb[k] * rC, fCr[k] * rC, nY[k]);
         }
     }
 }
