@@ -177,24 +177,195 @@ __global__ __launch_bounds__(256) void k_fill_zero(float* p, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) p[i] = 0.0f;
 }
 
-// Covariance + cyclic Jacobi (parallel circle ordering) + basis selection.
+// FP64 reciprocal square root: the hardware estimate (v_rsq_f64) refined by
+// two Newton steps y (1.5 - 0.5 x y^2), each written as y * fma(x y, -0.5 y, 1.5).
+__device__ __forceinline__ double rsq_refined(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(x * y, -0.5 * y, 1.5);
+    return y * fma(x * y, -0.5 * y, 1.5);
+}
+
+// Cyclic Jacobi sweeps on the symmetric 25 x 25 matrix in Ac (scalar type T),
+// accumulating the rotations into V, until the off-diagonal energy is at most
+// tol of the total or max_sweeps have run.  Every thread of the 256-thread
+// workgroup calls it; on return Ac holds the rotated matrix.
+//
+// A round applies its 12 rotations at once, A <- J^T A J and V <- V J.
+// Round r pairs the indices into super-indices u = 0..11, (p_u, q_u) = the
+// sorted ((r + u + 1) mod 25, (r - u - 1) mod 25), and u = 12 = {r}
+// (identity).  J acts on the 2 x 2 blocks A[{p_u, q_u}][{p_v, q_v}]:
+// Y = J_u^T X J_v with J = [[c, s], [-s, c]] (rows/columns p, q), and the pair
+// entry of a diagonal block is set to zero.  Every entry belongs to exactly one
+// block: threads 0..90 (wavefronts 0 and 1) own one block each (both
+// triangles), reading A from one buffer and writing the other, so a round needs
+// one workgroup barrier.  Threads 128..244 (wavefronts 2 and 3) own the rows
+// {j0, j0 + 9, j0 + 18} of V at one super-index v each and update them in
+// place.  Every thread computes the rotations it needs from the round's A
+// (bit-identical everywhere).  Rotation: the Golub & Van Loan angle
+// (|phi| <= pi/4, tan(2 phi) = 2 a_pq / (a_qq - a_pp)) from its half-angle
+// form, c^2 = (1 + |d| / h) / 2, s = sign(d e) |e| / (2 h c), h = hypot(d, e),
+// d = a_qq - a_pp, e = 2 a_pq: one reciprocal square root for 1/h, one for 1/c.
+// Row pitch of the 25 x 25 matrices in LDS: odd, so the rows of a column fall
+// in different banks, and p * 33 = (p << 5) + p.
+constexpr int kLD = 33;
+
+template <class T>
+__device__ void jacobi_sweeps(T*& Ac, T*& An, T* V, double tol, int max_sweeps, double2* energy) {
+    constexpr int N = kPatch, LD = kLD;
+    constexpr int NU = N * (N + 1) / 2;  // upper triangle, a <= b
+    constexpr int IU = (NU + 255) / 256;
+    constexpr int NS = 13, NB = NS * (NS + 1) / 2;  // super-indices, blocks
+    constexpr int V0T = 128, NVT = NS * 9;           // V threads: wavefronts 2 and 3
+    const T one = T(1), half = T(0.5), guard = sizeof(T) == 8 ? T(1e-300) : T(1e-30f);
+    const int tid = threadIdx.x;
+    int ua[IU], ub[IU];
+#pragma unroll
+    for (int m = 0; m < IU; m++) {
+        int t = min(tid + 256 * m, NU - 1), a = 0;
+        while (t >= N - a) {
+            t -= N - a;
+            a++;
+        }
+        ua[m] = a;
+        ub[m] = a + t;
+    }
+    const bool is_block = tid < NB, is_v = tid >= V0T && tid < V0T + NVT;
+    int bu = NS - 1, bv = NS - 1;  // block of a block thread; bu = the super-index of a V thread
+    if (is_block) {
+        int t = tid;
+        bu = 0;
+        while (t >= NS - bu) {
+            t -= NS - bu;
+            bu++;
+        }
+        bv = bu + t;
+    } else if (is_v) {
+        bu = (tid - V0T) % NS;
+    }
+    const int vj0 = is_v ? (tid - V0T) / NS : 0;  // rows vj0, vj0 + 9, vj0 + 18
+    // The pair of super-index u in round r is the sorted ((r + u + 1) mod 25,
+    // (r - u - 1) mod 25) (u = 12: (r, r)); both indices advance by one per
+    // round, so they are kept and stepped instead of recomputed.
+    auto first = [&](int u, int& i, int& j) {
+        i = u == NS - 1 ? 0 : u + 1;
+        j = u == NS - 1 ? 0 : N - 1 - u;
+    };
+    auto step = [&](int& i) { i = i == N - 1 ? 0 : i + 1; };
+
+    // Straight-line code (no select on the expensive operands, which the
+    // compiler would turn into branches).  The identity (p == q, or a_pq == 0)
+    // is blended in arithmetically: c + m (1 - c) = 1 and s (1 - m) = 0 exactly
+    // for m = 1 (1 - c is exact for c in [1/2, 1]).  rotation2 advances two
+    // independent rotations step by step, so their dependent chains overlap.
+    auto rotation2 = [&](bool id0, T app0, T aqq0, T apq0, bool id1, T app1, T aqq1, T apq1, T& c0, T& s0, T& c1,
+                         T& s1) {
+        const T e0 = T(2) * apq0, d0 = aqq0 - app0, e1 = T(2) * apq1, d1 = aqq1 - app1;
+        const T m0 = (id0 || e0 == T(0)) ? one : T(0), m1 = (id1 || e1 == T(0)) ? one : T(0);
+        const T y0 = rsq_refined(fma(d0, d0, e0 * e0) + guard), y1 = rsq_refined(fma(d1, d1, e1 * e1) + guard);
+        const T c20 = half * fma(fabs(d0), y0, one), c21 = half * fma(fabs(d1), y1, one);  // cos^2 in [1/2, 1]
+        const T z0 = rsq_refined(c20), z1 = rsq_refined(c21);                              // 1 / c
+        const T cc0 = c20 * z0, cc1 = c21 * z1;
+        c0 = fma(m0, one - cc0, cc0);
+        c1 = fma(m1, one - cc1, cc1);
+        s0 = copysign((half * fabs(e0) * y0) * z0, d0 * e0) * (one - m0);
+        s1 = copysign((half * fabs(e1) * y1) * z1, d1 * e1) * (one - m1);
+    };
+
+    for (int sweep = 0; sweep < max_sweeps; sweep++) {
+        // Off-diagonal and total energy (FP64 sums): per-thread entries,
+        // wavefront sums, then the 4 wavefront sums in a fixed order (uniform).
+        double off = 0.0, tot = 0.0;
+#pragma unroll
+        for (int m = 0; m < IU; m++) {
+            if (tid + 256 * m < NU) {
+                const double x = (double)Ac[ua[m] * LD + ub[m]];
+                if (ua[m] == ub[m]) {
+                    tot += x * x;
+                } else {
+                    off += 2.0 * x * x;
+                }
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o);
+            tot += __shfl_xor(tot, o);
+        }
+        if ((tid & 63) == 0) energy[tid >> 6] = make_double2(off, tot);
+        __syncthreads();
+        off = energy[0].x + energy[1].x + energy[2].x + energy[3].x;
+        tot = energy[0].y + energy[1].y + energy[2].y + energy[3].y + off;
+        __syncthreads();  // energy[] is rewritten by the next sweep
+        if (off == 0.0 || off <= tol * tot) break;
+
+        int iu, ju, iv, jv;
+        first(bu, iu, ju);
+        first(bv, iv, jv);
+        for (int round = 0; round < N; round++, step(iu), step(ju), step(iv), step(jv)) {
+            if (is_block) {
+                const int pu = min(iu, ju), qu = max(iu, ju), pv = min(iv, jv), qv = max(iv, jv);
+                const T uu0 = Ac[pu * LD + pu], uu1 = Ac[qu * LD + qu], uu2 = Ac[pu * LD + qu];
+                const T vv0 = Ac[pv * LD + pv], vv1 = Ac[qv * LD + qv], vv2 = Ac[pv * LD + qv];
+                const T x00 = Ac[pu * LD + pv], x01 = Ac[pu * LD + qv];
+                const T x10 = Ac[qu * LD + pv], x11 = Ac[qu * LD + qv];
+                T cu, su, cv, sv;  // (cv, sv) = (cu, su) on a diagonal block
+                rotation2(pu == qu, uu0, uu1, uu2, pv == qv, vv0, vv1, vv2, cu, su, cv, sv);
+                // Z = X J_v, Y = J_u^T Z
+                const T z00 = x00 * cv - x01 * sv, z01 = x00 * sv + x01 * cv;
+                const T z10 = x10 * cv - x11 * sv, z11 = x10 * sv + x11 * cv;
+                const T y00 = cu * z00 - su * z10, y11 = su * z01 + cu * z11;
+                T y01 = cu * z01 - su * z11, y10 = su * z00 + cu * z10;
+                if (bu == bv && bu != NS - 1) y01 = y10 = T(0);  // the annihilated pair
+                An[pu * LD + pv] = y00;
+                An[pu * LD + qv] = y01;
+                An[qu * LD + pv] = y10;
+                An[qu * LD + qv] = y11;
+                An[pv * LD + pu] = y00;
+                An[qv * LD + pu] = y01;
+                An[pv * LD + qu] = y10;
+                An[qv * LD + qu] = y11;
+            } else if (is_v) {
+                const int p = min(iu, ju), q = max(iu, ju);
+                const T a0 = Ac[p * LD + p], a1 = Ac[q * LD + q], a2 = Ac[p * LD + q];
+                T v0[3], v1[3];
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    const int j = min(vj0 + 9 * m, N - 1);
+                    v0[m] = V[j * LD + p];
+                    v1[m] = V[j * LD + q];
+                }
+                T c, s, c_unused, s_unused;
+                rotation2(p == q, a0, a1, a2, p == q, a0, a1, a2, c, s, c_unused, s_unused);
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    const int j = vj0 + 9 * m;
+                    if (j < N) {
+                        V[j * LD + p] = c * v0[m] - s * v1[m];
+                        V[j * LD + q] = s * v0[m] + c * v1[m];
+                    }
+                }
+            }
+            __syncthreads();
+            T* sw = Ac;
+            Ac = An;
+            An = sw;
+        }
+    }
+}
+
+// Covariance + Jacobi eigensolve + basis selection.
 //
 // C = (sum x x^T - (sum x)(sum x)^T / n) / (n - 1)   -- PCA.cpp:31-32
-// Jacobi: each sweep runs 25 rounds; round r rotates the 12 disjoint pairs
-// ((r + k) mod 25, (r - k) mod 25), k = 1..12, so every off-diagonal pair is
-// annihilated once per sweep (index r idles).  Rotation as in Golub & Van Loan
-// Alg. 8.5.2 (c^2 + s^2 = 1 to FP64 precision, so V stays orthogonal).  Sweeps
-// stop when the off-diagonal energy is below kJacobiTol of the total.  The 8
-// largest eigenpairs, largest first (PCA.cpp:49-55), are written as
-// basis[r * 8 + c]; ties in eigenvalue keep index order.
+// FP64 Jacobi sweeps until the off-diagonal energy is kJacobiTol of the total.
+// The 8 largest eigenpairs, largest first (PCA.cpp:49-55), are written as
+// basis[r * 8 + c]; ties in eigenvalue keep index order; each vector is
+// oriented to a non-negative dot product with sign_ref.
 __device__ void pca_solve_body(const double* moments, double n, float* basis, double* eigvals, double* eig_full) {
     constexpr int N = kPatch;
-    constexpr int LD = N + 1;
+    constexpr int LD = kLD;
     __shared__ double mom[kMoments];
     __shared__ double A0[N * LD], A1[N * LD];  // current and next A
     __shared__ double V[N * LD];
     __shared__ double2 energy[4];
-    double *Ac = A0, *An = A1;
     const int tid = threadIdx.x;
 
     for (int e = tid; e < kMoments; e += 256) mom[e] = moments[e];
@@ -212,174 +383,8 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
         V[r * LD + s] = r == s ? 1.0 : 0.0;
     }
     __syncthreads();
-
-    // A round applies its 12 rotations at once, A <- J^T A J and V <- V J.
-    // Round r pairs the indices into super-indices u = 0..11, (p_u, q_u) =
-    // the sorted ((r + u + 1) mod 25, (r - u - 1) mod 25), and u = 12 = {r}
-    // (identity).  J acts on the 2 x 2 blocks A[{p_u, q_u}][{p_v, q_v}]:
-    // Y = J_u^T X J_v with J = [[c, s], [-s, c]] (rows/columns p, q), and the
-    // pair entry of a diagonal block is set to zero.  Every entry belongs to
-    // exactly one block: threads 0..90 (wavefronts 0 and 1) own one block each
-    // (both triangles), reading A from one buffer and writing the other, so a
-    // round needs one workgroup barrier.  Threads 128..244 (wavefronts 2 and 3)
-    // own the rows {j0, j0 + 9, j0 + 18} of V at one super-index v each and
-    // update them in place.  Every thread
-    // computes the rotations it needs from the round's A (bit-identical
-    // everywhere).  Rotation: the Golub & Van Loan angle (|phi| <= pi/4,
-    // tan(2 phi) = 2 a_pq / (a_qq - a_pp)) from its half-angle form,
-    // c^2 = (1 + |d| / h) / 2, s = sign(d e) |e| / (2 h c), h = hypot(d, e),
-    // d = a_qq - a_pp, e = 2 a_pq: one rsq for 1/h, one for 1/c.
-    {
-        constexpr int NU = N * (N + 1) / 2;  // upper triangle, a <= b
-        constexpr int IU = (NU + 255) / 256;
-        constexpr int NS = 13, NB = NS * (NS + 1) / 2;  // super-indices, blocks
-        constexpr int V0T = 128, NVT = NS * 9;           // V threads: wavefronts 2 and 3
-        int ua[IU], ub[IU];
-#pragma unroll
-        for (int m = 0; m < IU; m++) {
-            int t = min(tid + 256 * m, NU - 1), a = 0;
-            while (t >= N - a) {
-                t -= N - a;
-                a++;
-            }
-            ua[m] = a;
-            ub[m] = a + t;
-        }
-        const bool is_block = tid < NB, is_v = tid >= V0T && tid < V0T + NVT;
-        int bu = NS - 1, bv = NS - 1;  // block of a block thread; bu = the super-index of a V thread
-        if (is_block) {
-            int t = tid;
-            bu = 0;
-            while (t >= NS - bu) {
-                t -= NS - bu;
-                bu++;
-            }
-            bv = bu + t;
-        } else if (is_v) {
-            bu = (tid - V0T) % NS;
-        }
-        const int vj0 = is_v ? (tid - V0T) / NS : 0;  // rows vj0, vj0 + 9, vj0 + 18
-        for (int sweep = 0; sweep < 40; sweep++) {
-            // Off-diagonal and total energy: per-thread entries, wavefront
-            // sums, then the 4 wavefront sums in a fixed order (uniform).
-            double off = 0.0, tot = 0.0;
-#pragma unroll
-            for (int m = 0; m < IU; m++) {
-                if (tid + 256 * m < NU) {
-                    const double x = Ac[ua[m] * LD + ub[m]];
-                    if (ua[m] == ub[m]) {
-                        tot += x * x;
-                    } else {
-                        off += 2.0 * x * x;
-                    }
-                }
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                off += __shfl_xor(off, o);
-                tot += __shfl_xor(tot, o);
-            }
-            if ((tid & 63) == 0) energy[tid >> 6] = make_double2(off, tot);
-            __syncthreads();
-            off = energy[0].x + energy[1].x + energy[2].x + energy[3].x;
-            tot = energy[0].y + energy[1].y + energy[2].y + energy[3].y + off;
-            __syncthreads();  // energy[] is rewritten by the next sweep
-            if (off == 0.0 || off <= kJacobiTol * tot) break;
-
-            for (int round = 0; round < N; round++) {
-                // pair of super-index u in this round (u = NS - 1: the idle index)
-                auto pair_of = [&](int u, int& p, int& q) {
-                    if (u == NS - 1) {
-                        p = q = round;
-                        return;
-                    }
-                    const int i = (round + u + 1) % N, j = (round - u - 1 + N) % N;
-                    p = min(i, j);
-                    q = max(i, j);
-                };
-                // Straight-line code (no select on the expensive operands, which
-                // the compiler would turn into branches).  The identity (p == q,
-                // or a_pq == 0) is blended in arithmetically: c + m (1 - c) = 1 and
-                // s (1 - m) = 0 exactly for m = 1 (1 - c is exact for c in [1/2, 1]).
-                // rotation2 advances two independent rotations step by step, so
-                // their dependent FP64 chains overlap in one wavefront.  Each
-                // reciprocal square root is the hardware estimate (v_rsq_f64)
-                // refined by two Newton steps y (1.5 - 0.5 x y^2), written as
-                // y * fma(x y, -0.5 y, 1.5).
-                auto rotation2 = [&](bool id0, double app0, double aqq0, double apq0, bool id1, double app1,
-                                     double aqq1, double apq1, double2& r0, double2& r1) {
-                    const double e0 = 2.0 * apq0, d0 = aqq0 - app0, e1 = 2.0 * apq1, d1 = aqq1 - app1;
-                    const double m0 = (id0 || e0 == 0.0) ? 1.0 : 0.0, m1 = (id1 || e1 == 0.0) ? 1.0 : 0.0;
-                    double h0 = fma(d0, d0, e0 * e0) + 1e-300, h1 = fma(d1, d1, e1 * e1) + 1e-300;
-                    double y0 = __builtin_amdgcn_rsq(h0), y1 = __builtin_amdgcn_rsq(h1);  // 1 / h
-                    y0 = y0 * fma(h0 * y0, -0.5 * y0, 1.5);
-                    y1 = y1 * fma(h1 * y1, -0.5 * y1, 1.5);
-                    y0 = y0 * fma(h0 * y0, -0.5 * y0, 1.5);
-                    y1 = y1 * fma(h1 * y1, -0.5 * y1, 1.5);
-                    const double c20 = 0.5 * fma(fabs(d0), y0, 1.0), c21 = 0.5 * fma(fabs(d1), y1, 1.0);
-                    double z0 = __builtin_amdgcn_rsq(c20), z1 = __builtin_amdgcn_rsq(c21);  // 1 / c
-                    z0 = z0 * fma(c20 * z0, -0.5 * z0, 1.5);
-                    z1 = z1 * fma(c21 * z1, -0.5 * z1, 1.5);
-                    z0 = z0 * fma(c20 * z0, -0.5 * z0, 1.5);
-                    z1 = z1 * fma(c21 * z1, -0.5 * z1, 1.5);
-                    const double c0 = c20 * z0, c1 = c21 * z1;
-                    const double s0 = copysign((0.5 * fabs(e0) * y0) * z0, d0 * e0);
-                    const double s1 = copysign((0.5 * fabs(e1) * y1) * z1, d1 * e1);
-                    r0 = make_double2(fma(m0, 1.0 - c0, c0), s0 * (1.0 - m0));
-                    r1 = make_double2(fma(m1, 1.0 - c1, c1), s1 * (1.0 - m1));
-                };
-                if (is_block) {
-                    int pu, qu, pv, qv;
-                    pair_of(bu, pu, qu);
-                    pair_of(bv, pv, qv);
-                    const double uu0 = Ac[pu * LD + pu], uu1 = Ac[qu * LD + qu], uu2 = Ac[pu * LD + qu];
-                    const double vv0 = Ac[pv * LD + pv], vv1 = Ac[qv * LD + qv], vv2 = Ac[pv * LD + qv];
-                    const double x00 = Ac[pu * LD + pv], x01 = Ac[pu * LD + qv];
-                    const double x10 = Ac[qu * LD + pv], x11 = Ac[qu * LD + qv];
-                    double2 ru, rv;  // rv = ru on a diagonal block
-                    rotation2(pu == qu, uu0, uu1, uu2, pv == qv, vv0, vv1, vv2, ru, rv);
-                    // Z = X J_v, Y = J_u^T Z
-                    const double z00 = x00 * rv.x - x01 * rv.y, z01 = x00 * rv.y + x01 * rv.x;
-                    const double z10 = x10 * rv.x - x11 * rv.y, z11 = x10 * rv.y + x11 * rv.x;
-                    const double y00 = ru.x * z00 - ru.y * z10, y11 = ru.y * z01 + ru.x * z11;
-                    double y01 = ru.x * z01 - ru.y * z11, y10 = ru.y * z00 + ru.x * z10;
-                    if (bu == bv && bu != NS - 1) y01 = y10 = 0.0;  // the annihilated pair
-                    An[pu * LD + pv] = y00;
-                    An[pu * LD + qv] = y01;
-                    An[qu * LD + pv] = y10;
-                    An[qu * LD + qv] = y11;
-                    An[pv * LD + pu] = y00;
-                    An[qv * LD + pu] = y01;
-                    An[pv * LD + qu] = y10;
-                    An[qv * LD + qu] = y11;
-                } else if (is_v) {
-                    int p, q;
-                    pair_of(bu, p, q);
-                    const double a0 = Ac[p * LD + p], a1 = Ac[q * LD + q], a2 = Ac[p * LD + q];
-                    double v0[3], v1[3];
-#pragma unroll
-                    for (int m = 0; m < 3; m++) {
-                        const int j = min(vj0 + 9 * m, N - 1);
-                        v0[m] = V[j * LD + p];
-                        v1[m] = V[j * LD + q];
-                    }
-                    double2 r, r_unused;
-                    rotation2(p == q, a0, a1, a2, p == q, a0, a1, a2, r, r_unused);
-#pragma unroll
-                    for (int m = 0; m < 3; m++) {
-                        const int j = vj0 + 9 * m;
-                        if (j < N) {
-                            V[j * LD + p] = r.x * v0[m] - r.y * v1[m];
-                            V[j * LD + q] = r.y * v0[m] + r.x * v1[m];
-                        }
-                    }
-                }
-                __syncthreads();
-                double* sw = Ac;
-                Ac = An;
-                An = sw;
-            }
-        }
-    }
+    double *Ac = A0, *An = A1;
+    jacobi_sweeps<double>(Ac, An, V, kJacobiTol, 40, energy);
     __syncthreads();
 
     if (tid < N) {
