@@ -454,8 +454,8 @@ gcam_status launch_level_backward(const AdjointLevel& L, hipStream_t s) {
     GCAM_LAUNCH_CHECK("block-matching backward");
     if (!L.gD1) return GCAM_OK;  // coarsest level: nothing upstream depends on thresholds
     k_proj_bwd<<<g, b, 0, s>>>(L.S, L.gF, L.basis, L.gS);
-    k_basis_grad<<<kMomentBlocks, 256, 0, s>>>(L.S, L.gF, L.gB_part);
-    k_eig_bwd<<<1, 256, 0, s>>>(L.gB_part, kMomentBlocks, L.eig, L.gC);
+    k_basis_grad<<<kBasisGradBlocks, 256, 0, s>>>(L.S, L.gF, L.gB_part);
+    k_eig_bwd<<<1, 256, 0, s>>>(L.gB_part, kBasisGradBlocks, L.eig, L.gC);
     const int gw = L.S.w / 8, gh = L.S.h / 8;
     if (gw * gh > 0) k_patch_bwd<<<(gw * gh + 255) / 256, 256, 0, s>>>(L.S, L.gC, L.eig, L.n, L.gS);
     k_sub_bwd<<<grid64x4(L.w1, L.h1), b, 0, s>>>(L.S, L.gS, L.w1, L.h1, L.gD1);

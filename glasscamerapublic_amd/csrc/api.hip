@@ -350,7 +350,7 @@ static gcam_status alloc_adjoint(gcam_pyramid* p) {
         {(void**)&p->adj_gD[1], sizeof(float4) * n1},
         {(void**)&p->adj_gthr_part, sizeof(float) * 3 * adjoint_thr_blocks(p->W[0], p->H[0])},
         {(void**)&p->adj_gthr, sizeof(float) * 3 * p->levels},
-        {(void**)&p->adj_gB_part, sizeof(double) * kMomentBlocks * kPatch * kComponents},
+        {(void**)&p->adj_gB_part, sizeof(double) * kBasisGradBlocks * kPatch * kComponents},
         {(void**)&p->adj_gC, sizeof(double) * 625},
     };
     size_t total = 0;

@@ -134,10 +134,10 @@ __global__ __launch_bounds__(256) void k_buffer_moments(const float* patches, in
     });
 }
 
-// Sum of the per-workgroup partials, entry by entry: workgroup x owns 14
-// entries; thread t adds the partials b = t/14, t/14 + 18, ... of entry
-// 14 x + t%14, and the 18 sums are added in index order (deterministic).
-constexpr int kRedEntries = 14, kRedLanes = 18;
+// Sum of the per-workgroup partials, entry by entry: workgroup x owns 7
+// entries; thread t adds the partials b = t/7, t/7 + 36, ... of entry
+// 7 x + t%7, and the 36 sums are added in index order (deterministic).
+constexpr int kRedEntries = 7, kRedLanes = 36;
 static_assert(kMoments % kRedEntries == 0, "moment reduction tiling");
 __device__ void moments_reduce_body(int blk, const double* partials, int nblocks, double* mom) {
     __shared__ double part[kRedLanes][kRedEntries];
@@ -486,9 +486,11 @@ gcam_status launch_collect_patches(View<const float4> in, float* patches, int64_
     return GCAM_OK;
 }
 
-// Workgroups of the moment reduction: one per 256 patches, at most kMomentBlocks.
+// Workgroups of the moment reduction: one per 64 patches (each of its four
+// wavefronts then takes four groups of four patches: a short latency chain),
+// at most kMomentBlocks.
 static int moment_blocks(int64_t npatch) {
-    return (int)std::max<int64_t>(1, std::min<int64_t>(kMomentBlocks, (npatch + 255) / 256));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(kMomentBlocks, (npatch + 63) / 64));
 }
 
 gcam_status launch_image_moments(View<const float4> in, double* partials, int* nblocks, hipStream_t s) {
