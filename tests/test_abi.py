@@ -105,3 +105,22 @@ def test_cpp_facade_compiles_and_links(tmp_path):
     exe = tmp_path / "facade"
     subprocess.run(["g++", "-std=c++17", f"-I{repo}/include", "-D__HIP_PLATFORM_AMD__", str(src), "-o", str(exe),
                     f"-L{libdir}", "-lgcam_denoise", f"-Wl,-rpath,{libdir}"], check=True)
+
+
+def test_burst_create_rejects_bad_arguments_without_a_device():
+    """Argument checks of gcam_burst_create that run before any device call."""
+    import ctypes
+    from glasscamerapublic_amd._lib import ConfigC, lib
+    L = lib()
+    h = ctypes.c_void_p()
+    W = (ctypes.c_int32 * 2)(64, 64)
+    H = (ctypes.c_int32 * 2)(64, 64)
+    assert L.gcam_burst_create(0, W, H, 2, None, ctypes.byref(h)) == 1  # n_frames out of range
+    assert L.gcam_burst_create(65, W, H, 2, None, ctypes.byref(h)) == 1
+    assert L.gcam_burst_create(2, None, H, 2, None, ctypes.byref(h)) == 1
+    cfg = ConfigC(4, 0, 0, 0)  # FP32: the burst runs the precision form
+    assert L.gcam_burst_create(2, W, H, 2, ctypes.byref(cfg), ctypes.byref(h)) == 4
+    cfg = ConfigC(4, 1, 0, 1)  # bilateral
+    assert L.gcam_burst_create(2, W, H, 2, ctypes.byref(cfg), ctypes.byref(h)) == 4
+    assert L.gcam_burst_create(2, W, H, 2, None, None) == 1
+    assert not h.value

@@ -133,3 +133,45 @@ def test_burst_argument_errors():
         burst.set_tuning([3] * LEVELS, [0] * LEVELS, [0] * LEVELS)
     with pytest.raises(ValueError):
         burst.denoise(params[:2], frames, nlfs)
+
+
+def test_burst_gradient_sharpening_lens_odd_sizes():
+    """Odd frame sizes (no multiple of the tiles), gradients on some frames,
+    sharpening > 1 and lens shading: bit-equal to the single-frame path."""
+    sizes = [(517, 389), (333, 250), (640, 481)]
+    levels, radius = 3, 3
+    gains = [3e-3, 6e-3, 2e-3]
+    frames = [gcam.synth_frame(w, h, a, 2e-3, scene_seed=900 + k, noise_seed=950 + k)
+              for k, ((w, h), a) in enumerate(zip(sizes, gains))]
+    grads = [torch.rand((h, w, 2), device="cuda") * 0.05 if k != 1 else None for k, (w, h) in enumerate(sizes)]
+    params = [[gcam.DenoiseParameters(luma=2.0, chroma=1.5, chromaBoost=2.0, gradientBoost=2.0 if k != 1 else 0.0,
+                                      gradientThreshold=1.5, sharpening=1.4) for _ in range(levels)]
+              for k in range(len(sizes))]
+    nlfs = [[(tuple(x[0]), tuple(x[1])) for x in nlf_for(levels, a, 2e-3)] for a in gains]
+    burst = gcam.BurstProcessor(sizes, levels=levels, search_radius=radius, precision="fp16")
+    burst.set_tuning([2] * levels, [0] * levels, [0] * levels)
+    burst.denoise(params, frames, nlfs, gradients=grads, lensShadingCorrection=0.3)
+    torch.cuda.synchronize()
+    for k, (w, h) in enumerate(sizes):
+        proc = gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius, precision="fp16",
+                                     bm_rows_per_thread=2)
+        proc.denoise(params[k], frames[k], grads[k], list(nlfs[k]), lensShadingCorrection=0.3)
+        for i in range(levels):
+            assert torch.equal(burst.output(k, i), proc.denoisedImagePyramid[i]), f"frame {k} level {i}"
+        proc.close()
+
+
+def test_single_level_burst():
+    sizes = [(256, 192), (320, 200)]
+    frames = [gcam.synth_frame(w, h, 4e-3, 2e-3, scene_seed=31 + k, noise_seed=41 + k) for k, (w, h) in enumerate(sizes)]
+    params = [gcam.DenoiseParameters(luma=3.0)]
+    nlf = [(tuple(x[0]), tuple(x[1])) for x in nlf_for(1)]
+    burst = gcam.BurstProcessor(sizes, levels=1, search_radius=2, precision="fp16")
+    burst.set_tuning([1], [0], [0])
+    burst.denoise(params, frames, nlf)
+    torch.cuda.synchronize()
+    for k, (w, h) in enumerate(sizes):
+        proc = gcam.PyramidProcessor(w, h, levels=1, search_radius=2, precision="fp16", bm_rows_per_thread=1)
+        ref = proc.denoise(params, frames[k], None, list(nlf))
+        assert torch.equal(burst.output(k, 0), ref)
+        proc.close()
