@@ -96,11 +96,14 @@ def test_autotuned_burst_and_graph_replays():
         assert tun["tuned"] and all(r in (1, 2, 4) for r in tun["rows_per_thread"])
         print("replay", rep, "tuning", tun)
         ref = single_frame(frames, params, nlfs)  # eager GPU work between the replays
+        # The tuned tile may differ from the single-frame one; the FP16 features
+        # are centred on each tile's first pixel, so the two differ by FP16
+        # rounding of the features (the precision form's bar against FP64 is 5e-3).
         for k in range(len(SIZES)):
             for i in range(LEVELS):
                 got = burst.output(k, i)
                 err = (got[..., :3] - ref[k][i][..., :3]).abs().max().item()
-                assert err < 1e-4, f"replay {rep} frame {k} level {i}: {err}"
+                assert err < 1e-3, f"replay {rep} frame {k} level {i}: {err}"
 
 
 def test_bf16_burst_matches_single_frame():
