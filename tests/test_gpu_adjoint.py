@@ -120,3 +120,50 @@ def test_unsupported_settings_are_rejected():
         proc.denoise_fwd(prm, torch.ones(3, 3), ycc, nlf)
     with pytest.raises(gcam.GcamError, match="no gcam_pyramid_denoise_fwd"):
         proc.denoise_bwd(torch.zeros((256, 256, 4), device="cuda"))
+
+
+def test_differentiation_workload_16_crops(oracle_threads):
+    """The differentiation workload: 16 crops of 512x512, 5 levels, 9x9 window,
+    a = 5e-3, b = 2e-3, L = 1/2 |out - clean|^2 summed over the batch; the 15
+    threshold gradients from torch.autograd through the HIP adjoint, against
+    central differences (eps = 1e-3) of the GPU batch loss and against the sum
+    of the FP64 CPU adjoints.  Components above 5 % of the largest are also held
+    to 1e-3 relative against the FP64 gradient."""
+    w, h, levels, radius, n = 512, 512, 5, 4, 16
+    frames = [gcam.synth_frame(w, h, 5e-3, 2e-3, scene_seed=300 + k, noise_seed=400 + k, want_clean=True)
+              for k in range(n)]
+    nlf = [(tuple(x[0]), tuple(x[1])) for x in nlf_for(levels, 5e-3, 2e-3)]
+    procs = [gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius) for _ in range(n)]
+    clean = torch.stack([f[1][..., :3] for f in frames])
+
+    def batch_loss(t):
+        out = denoise_thresholds(t, [f[0] for f in frames], procs, params_for(levels), [nlf] * n)
+        return 0.5 * ((out - clean).double() ** 2).sum()
+
+    thr = torch.tensor(THR5, requires_grad=True)
+    batch_loss(thr).backward()
+    g = thr.grad.numpy().astype(np.float64)
+
+    eps = 1e-3
+    fd = np.zeros((levels, 3))
+    with torch.no_grad():
+        for i in range(levels):
+            for c in range(3):
+                tp, tm = THR5.copy(), THR5.copy()
+                tp[i, c] += eps
+                tm[i, c] -= eps
+                fd[i, c] = (float(batch_loss(torch.from_numpy(tp))) - float(batch_loss(torch.from_numpy(tm)))) / (2 * eps)
+
+    ref = np.zeros((levels, 3))
+    for k in range(n):
+        ref += oracle.pyramid_loss_grad(host(frames[k][0]).astype(np.float64), host(frames[k][1]).astype(np.float64),
+                                        THR5.astype(np.float64), CB5.astype(np.float64), np.array(nlf),
+                                        levels=levels, radius=radius)[1]
+    print(np.c_[g.ravel(), fd.ravel(), ref.ravel()])
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(g, ref, rtol=0, atol=1e-3 * scale)
+    np.testing.assert_allclose(g, fd, rtol=0, atol=1e-3 * scale)
+    big = np.abs(ref) > 0.05 * scale
+    rel = np.abs(g - ref)[big] / np.abs(ref)[big]
+    print("relative error of the significant components vs FP64:", rel.max())
+    assert rel.max() < 1e-3
