@@ -40,10 +40,21 @@ BURST_SIZES = [(3264, 2448), (4032, 3024), (5472, 3648), (8064, 6048)]
 # (9x9 window), levels 0..4: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
 # what the counter reads for a plain 256 MiB copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
 MEASURED_BM_TRAFFIC = {
-    ("12mp", "fp16"): [58.1, 52.1, 40.4, 40.9, 45.6],
-    ("12mp", "fp32"): [111.7, 103.8, 81.7, 80.4, 80.1],
+    ("12mp", "fp16"): [58.1, 52.1, 40.3, 40.8, 45.0],
+    ("12mp", "fp32"): [111.6, 103.7, 81.8, 80.1, 80.1],
+}
+# Wave-level VALU instructions per block-matching dispatch on the same workload, levels 0..4:
+# rocprofv3 --pmc SQ_INSTS_VALU, in millions.
+MEASURED_BM_VALU = {
+    ("12mp", "fp16"): [482.805, 120.839, 30.759, 7.722, 1.946],
+    ("12mp", "fp32"): [534.553, 133.651, 33.950, 8.532, 2.155],
 }
 DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}
+# Vector ALU: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz (a wave64 instruction takes 4 cycles).
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
+# Issue slots per (pixel, window position) of the reference's inner loop: ~39 VALU operations and
+# two exponentials at quarter rate.
+MODEL_SLOTS_PER_PAIR = 47
 
 
 def algorithmic_bytes_per_base_pixel(levels: int) -> float:
@@ -152,6 +163,18 @@ def single_frame(args) -> dict:
     bm_gbs = bm_bytes_per_launch / (bm_avg_ms / 1e3) / 1e9
     coef = MEASURED_BM_TRAFFIC.get((args.workload, args.precision))
     traffic = None if coef is None else round(sum(c * n for c, n in zip(coef, level_px)) / levels)
+    pairs_per_launch = (2 * radius + 1) ** 2 * sum(level_px) / levels
+    model_rate = MODEL_SLOTS_PER_PAIR * pairs_per_launch / (bm_avg_ms / 1e3)
+    valu = {"kernel": f"k_block_match{'' if args.precision == 'fp32' else '_lp'}<{radius}>",
+            "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+            "model_slots_per_pair": MODEL_SLOTS_PER_PAIR,
+            "model_frac": round(model_rate / VALU_PEAK_LANE_OPS, 4),
+            "measured_lane_ops_per_pair": None, "issue_frac": None}
+    insts = MEASURED_BM_VALU.get((args.workload, args.precision))
+    if insts is not None:
+        lane_ops = 64e6 * sum(insts) / levels
+        valu["measured_lane_ops_per_pair"] = round(lane_ops / pairs_per_launch, 2)
+        valu["issue_frac"] = round(lane_ops / (bm_avg_ms / 1e3) / VALU_PEAK_LANE_OPS, 4)
     b_alg = algorithmic_bytes_per_base_pixel(levels)
     pipeline_gbs = b_alg * w * h / (ms_per_step / 1e3) / 1e9
 
@@ -212,6 +235,7 @@ def single_frame(args) -> dict:
             "avg_launch_ms": round(bm_avg_ms, 4),
             "bytes_per_launch": int(bm_bytes_per_launch),
         },
+        "valu": valu,
         "pipeline": {"B_alg_per_base_px": round(b_alg, 3), "achieved_GBs": round(pipeline_gbs, 2),
                      "hbm_frac": round(pipeline_gbs / HBM_PEAK_GBS, 5)},
         "median_ms": round(prim["median_ms"], 4),
