@@ -1,4 +1,5 @@
-"""ctypes binding of the C ABI declared in ``include/gcam/denoise.h``.
+"""ctypes binding of the C ABI declared in ``include/gcam/denoise.h`` and
+``include/gcam/front_end.h``.
 
 The product path is the HIP library ``libgcam_denoise.so`` built in-tree by
 ``glasscamerapublic_amd.build``.  There is no fallback: if the library is
@@ -18,6 +19,7 @@ GCAM_PIXEL_RG32F = 1
 GCAM_PIXEL_R32F = 2
 GCAM_PIXEL_PCA8_F32 = 3
 GCAM_PIXEL_PCA8_F16 = 4
+GCAM_PIXEL_R16U = 5
 
 GCAM_PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 GCAM_LEVEL_DENOISED, GCAM_LEVEL_SUBTRACTED, GCAM_LEVEL_NOISY, GCAM_LEVEL_FEATURES = 0, 1, 2, 3
@@ -34,6 +36,10 @@ EXPORTED = (
     "gcam_basic_noise_statistics", "gcam_measure_ycbcr_nlf", "gcam_synth_frame",
     "gcam_burst_create", "gcam_burst_destroy", "gcam_burst_denoise", "gcam_burst_output", "gcam_burst_tune",
     "gcam_burst_get_tuning", "gcam_burst_set_tuning",
+    "gcam_scale_raw_data", "gcam_raw_image_sobel", "gcam_gaussian_kernel_bilinear_weights",
+    "gcam_gaussian_blur_sobel_image", "gcam_raw_variance", "gcam_demosaic_image", "gcam_blend_highlights_image",
+    "gcam_transform_image", "gcam_despeckle_image", "gcam_cam_ycbcr", "gcam_matrix3_inverse",
+    "gcam_front_end_create", "gcam_front_end_destroy", "gcam_front_end_run",
 )
 GCAM_MAX_LEVELS = 8
 GCAM_MAX_BURST = 64
@@ -76,6 +82,13 @@ class SynthParamsC(ctypes.Structure):
 class BurstTuningC(ctypes.Structure):
     _fields_ = [("rows_per_thread", ctypes.c_int32 * 8), ("grid", ctypes.c_int32 * 8),
                 ("schedule", ctypes.c_int32 * 8), ("level_ms", ctypes.c_float * 8), ("tuned", ctypes.c_int32)]
+
+
+class FrontEndParamsC(ctypes.Structure):
+    _fields_ = [("bayer_pattern", ctypes.c_int32), ("black_level", ctypes.c_float), ("lens_shading", ctypes.c_float),
+                ("scale_mul", ctypes.c_float * 4), ("raw_nlf_a", ctypes.c_float * 4),
+                ("raw_nlf_b", ctypes.c_float * 4), ("rgb_cam", ctypes.c_float * 9),
+                ("highlight_clip", ctypes.c_float), ("despeckle", ctypes.c_int32), ("despeckle_nlf", NlfC)]
 
 
 _lib = None
@@ -123,6 +136,24 @@ def _declare(L: ctypes.CDLL) -> None:
     L.gcam_burst_output.argtypes = [_P, _I, _I, _IP]
     L.gcam_burst_get_tuning.argtypes = [_P, ctypes.POINTER(BurstTuningC)]
     L.gcam_burst_set_tuning.argtypes = [_P, ctypes.POINTER(BurstTuningC)]
+    _FP = ctypes.POINTER(ctypes.c_float)
+    L.gcam_scale_raw_data.argtypes = [_IP, _IP, _I, _FP, _F, _F, _P]
+    L.gcam_raw_image_sobel.argtypes = [_IP, _IP, _P]
+    L.gcam_gaussian_kernel_bilinear_weights.argtypes = [_F, _FP, _I]
+    L.gcam_gaussian_blur_sobel_image.argtypes = [_IP, _IP, _F, _F, _FP, _IP, _P]
+    L.gcam_raw_variance.argtypes = [_FP, _FP, _FP]
+    L.gcam_raw_variance.restype = None
+    L.gcam_demosaic_image.argtypes = [_IP, _IP, _IP, _IP, _IP, _I, _FP, _P]
+    L.gcam_blend_highlights_image.argtypes = [_IP, _F, _IP, _P]
+    L.gcam_transform_image.argtypes = [_IP, _FP, _IP, _P]
+    L.gcam_despeckle_image.argtypes = [_IP, _FP, _FP, _IP, _P]
+    L.gcam_cam_ycbcr.argtypes = [_FP, _FP]
+    L.gcam_cam_ycbcr.restype = None
+    L.gcam_matrix3_inverse.argtypes = [_FP, _FP]
+    L.gcam_front_end_create.argtypes = [_I, _I, ctypes.POINTER(_P)]
+    L.gcam_front_end_destroy.argtypes = [_P]
+    L.gcam_front_end_destroy.restype = None
+    L.gcam_front_end_run.argtypes = [_P, _IP, ctypes.POINTER(FrontEndParamsC), _IP, _IP, _P]
     for name in EXPORTED:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int:  # default restype: gcam_status

@@ -1,0 +1,788 @@
+// Raw front end for gfx950: the steps of RawConverter::demosaic that produce
+// the denoise's inputs, and the despeckle that RawConverter::denoise runs
+// before the pyramid.
+//
+//   scaleRawData                  MetalPipeline/demosaic.metal:95-117
+//   rawImageSobel                 MetalPipeline/demosaic.metal:120-139
+//   sampledConvolutionSobel       MetalPipeline/demosaic.metal:170-210
+//   interpolateGreen              MetalPipeline/demosaic.metal:214-300
+//   interpolateRedBlue            MetalPipeline/demosaic.metal:309-416
+//   interpolateRedBlueAtGreen     MetalPipeline/demosaic.metal:420-521
+//   blendHighlightsImage          MetalPipeline/demosaic.metal:523-564
+//   transformImage                MetalPipeline/demosaic.metal:649-662
+//   despeckleLumaMedianChromaImage MetalPipeline/demosaic.metal:1160-1283
+//   gaussianKernelBilinearWeights MetalPipeline/demosaic_utils.cpp:939-1006
+//   cam_ycbcr                     MetalPipeline/demosaic_utils.cpp:703-720
+//   orchestration                 MetalPipeline/raw_converter.cpp:148-214, 70-78
+//
+// Every kernel is a stencil over FP32 images and HBM-bound; the Sobel filter
+// is fused into the gradient blur (the Sobel tile of a workgroup lives in LDS
+// and never reaches HBM), and the last demosaic pass, the highlight blend and
+// the YCbCr transform run as one pass.  Arithmetic is FP32 evaluated in source
+// order (no contraction): out-of-bounds reads clamp to the edge, the linear
+// sampler interpolates at exact sub-texel positions, so every stage except the
+// gradient direction (atan2f) reproduces a plain FP32 evaluation of the
+// formulas bit for bit.
+#pragma clang fp contract(off)
+
+#include "gcam_internal.hpp"
+#include "gcam/front_end.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace gcam {
+
+namespace {
+
+// MetalPipeline/demosaic.metal:24-38: offsets of the red, green, blue and
+// second green sample of a 2x2 quad, per BayerPattern (grbg, gbrg, rggb, bggr).
+const int kBayerOffsets[4][4][2] = {
+    {{1, 0}, {0, 0}, {0, 1}, {1, 1}},
+    {{0, 1}, {0, 0}, {1, 0}, {1, 1}},
+    {{0, 0}, {0, 1}, {1, 1}, {1, 0}},
+    {{1, 1}, {0, 1}, {0, 0}, {1, 0}},
+};
+enum { kRed = 0, kGreen = 1, kBlue = 2, kGreen2 = 3 };
+
+struct Quad {
+    int2 o[4];  // red, green, blue, green2
+};
+
+Quad quad_of(int pattern) {
+    Quad q;
+    for (int c = 0; c < 4; c++) q.o[c] = make_int2(kBayerOffsets[pattern][c][0], kBayerOffsets[pattern][c][1]);
+    return q;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <class P>
+__device__ __forceinline__ P fetch(const View<const P>& v, int x, int y) {
+    return v.at(clampi(x, 0, v.w - 1), clampi(y, 0, v.h - 1));
+}
+
+// Metal's smoothstep / mix / clamp.
+__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
+    float t = (x - e0) / (e1 - e0);
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a + (b - a) * t; }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+constexpr float kPi = 3.14159265358979323846f;
+
+// ---- scaleRawData: one thread per 2x2 quad -------------------------------
+
+__global__ __launch_bounds__(256) void k_scale_raw(View<const uint16_t> raw, View<float> out, Quad q, float4 sm,
+                                                   float black, float lens_corr) {
+    const int qx = blockIdx.x * blockDim.x + threadIdx.x, qy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (qx >= out.w / 2 || qy >= out.h / 2) return;
+    const int x = 2 * qx, y = 2 * qy;
+    float lens = 1.0f;
+    if (lens_corr > 0.0f) {
+        const float cx = (float)(out.w / 2), cy = (float)(out.h / 2);
+        const float dx = (float)x - cx, dy = (float)y - cy;
+        const float d = sqrtf(dx * dx + dy * dy) / sqrtf(cx * cx + cy * cy);
+        lens = 1.0f + lens_corr * d * d;
+    }
+    const float s[4] = {sm.x, sm.y, sm.z, sm.w};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int px = x + q.o[c].x, py = y + q.o[c].y;
+        const float v = (float)raw.at(px, py) / 65535.0f;
+        out.at(px, py) = fmaxf(lens * s[c] * (v - black) * 0.9f + 0.1f, 0.0f);
+    }
+}
+
+// ---- rawImageSobel -------------------------------------------------------
+
+// (gx, gy, |gx|, |gy|) at (x, y) from a clamped reader of the scaled raw:
+// gx = left - right and gy = top - bottom, 1-2-1 weighted, / sqrt(4.5),
+// accumulated row by row, left to right (demosaic.metal:120-129).
+template <class Raw>
+__device__ __forceinline__ float4 sobel_at(const Raw& r, int x, int y) {
+    float gx = 0.0f, gy = 0.0f;
+#pragma unroll
+    for (int j = -1; j <= 1; j++) {
+#pragma unroll
+        for (int i = -1; i <= 1; i++) {
+            const float kx = (float)(-i * (j == 0 ? 2 : 1));
+            const float ky = (float)(-j * (i == 0 ? 2 : 1));
+            const float s = r(x + i, y + j);
+            gx = gx + kx * s;
+            gy = gy + ky * s;
+        }
+    }
+    const float norm = sqrtf(4.5f);
+    gx = gx / norm;
+    gy = gy / norm;
+    return make_float4(gx, gy, fabsf(gx), fabsf(gy));
+}
+
+__global__ __launch_bounds__(256) void k_raw_sobel(View<const float> in, View<float4> out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    out.at(x, y) = sobel_at([&](int u, int v) { return fetch(in, u, v); }, x, y);
+}
+
+// ---- gaussianBlurSobelImage (sampledConvolutionSobel) --------------------
+
+// One bilinear tap: weight, integer offset and exact fractional part of the
+// sample position relative to the output pixel.
+struct Tap {
+    float w, fx, fy;
+    int ix, iy;
+};
+constexpr int kMaxTaps1 = 4, kMaxTaps2 = 36;  // radius <= 1.5 and <= 5.5
+struct BlurTaps {
+    int n1, n2;
+    Tap t1[kMaxTaps1];
+    Tap t2[kMaxTaps2];
+};
+
+// Output tile of a workgroup and its apron: taps reach texels x - 5 .. x + 6
+// for offsets in [-5, 5.5].
+constexpr int kBTW = 64, kBTH = 16, kBApL = 5, kBApR = 6;
+constexpr int kBAW = kBTW + kBApL + kBApR, kBAH = kBTH + kBApL + kBApR;
+
+// Weighted mean of the bilinear samples of the LDS Sobel tile at the taps
+// (demosaic.metal:170-186): s = (T00 (1 - fx) + T10 fx) (1 - fy) +
+// (T01 (1 - fx) + T11 fx) fy per channel, summed in tap order.
+__device__ __forceinline__ float4 blur_taps(const float4 (*sb)[kBAW], int lx, int ly, const Tap* taps, int n) {
+    float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float norm = 0.0f;
+    for (int i = 0; i < n; i++) {
+        const Tap t = taps[i];
+        const int ax = lx + t.ix, ay = ly + t.iy;
+        const float4 a = sb[ay][ax], b = sb[ay][ax + 1], c = sb[ay + 1][ax], d = sb[ay + 1][ax + 1];
+        const float gx = 1.0f - t.fx, gy = 1.0f - t.fy;
+        float4 s;
+        s.x = (a.x * gx + b.x * t.fx) * gy + (c.x * gx + d.x * t.fx) * t.fy;
+        s.y = (a.y * gx + b.y * t.fx) * gy + (c.y * gx + d.y * t.fx) * t.fy;
+        s.z = (a.z * gx + b.z * t.fx) * gy + (c.z * gx + d.z * t.fx) * t.fy;
+        s.w = (a.w * gx + b.w * t.fx) * gy + (c.w * gx + d.w * t.fx) * t.fy;
+        sum.x = sum.x + t.w * s.x;
+        sum.y = sum.y + t.w * s.y;
+        sum.z = sum.z + t.w * s.z;
+        sum.w = sum.w + t.w * s.w;
+        norm = norm + t.w;
+    }
+    return make_float4(sum.x / norm, sum.y / norm, sum.z / norm, sum.w / norm);
+}
+
+// kFromRaw: the Sobel tile is computed here from the scaled raw (the fused
+// front end); otherwise it is read from `sobel` (the per-operation entry
+// point, like the reference's two-texture kernel).  A tile entry outside the
+// image holds the Sobel value of the nearest image pixel, which is what the
+// clamp-to-edge sampler reads there.
+template <bool kFromRaw>
+__global__ __launch_bounds__(256) void k_blur_sobel(View<const float> raw, View<const float4> sobel, BlurTaps taps,
+                                                    float var_a, float var_b, View<float2> out) {
+    __shared__ float4 sb[kBAH][kBAW];
+    constexpr int RW = kBAW + 2, RH = kBAH + 2;
+    __shared__ float rw[kFromRaw ? RH : 1][kFromRaw ? RW : 1];
+    const int x0 = blockIdx.x * kBTW, y0 = blockIdx.y * kBTH;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x, nth = blockDim.x * blockDim.y;
+    const int W = out.w, H = out.h;
+    if (kFromRaw) {
+        // raw over [x0 - 6, x0 + TW + 6] (clamped positions)
+        for (int e = tid; e < RW * RH; e += nth) {
+            const int ry = e / RW, rx = e % RW;
+            rw[ry][rx] = fetch(raw, x0 - kBApL - 1 + rx, y0 - kBApL - 1 + ry);
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < kBAW * kBAH; e += nth) {
+        const int ay = e / kBAW, ax = e % kBAW;
+        const int cx = clampi(x0 - kBApL + ax, 0, W - 1), cy = clampi(y0 - kBApL + ay, 0, H - 1);
+        if (kFromRaw) {
+            // raw at the clamped neighbours of (cx, cy), which lie in rw's region
+            sb[ay][ax] = sobel_at(
+                [&](int u, int v) {
+                    return rw[clampi(v, 0, H - 1) - (y0 - kBApL - 1)][clampi(u, 0, W - 1) - (x0 - kBApL - 1)];
+                },
+                cx, cy);
+        } else {
+            sb[ay][ax] = sobel.at(cx, cy);
+        }
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x;
+    if (x >= W) return;
+    for (int r = threadIdx.y; r < kBTH; r += blockDim.y) {
+        const int y = y0 + r;
+        if (y >= H) break;
+        const int lx = threadIdx.x + kBApL, ly = r + kBApL;
+        float4 res = blur_taps(sb, lx, ly, taps.t1, taps.n1);
+        const float sigma = sqrtf(var_a + var_b * raw.at(x, y));
+        if (sqrtf(res.x * res.x + res.y * res.y) < 4.0f * sigma) res = blur_taps(sb, lx, ly, taps.t2, taps.n2);
+        out.at(x, y) = make_float2(copysignf(res.z, res.x), copysignf(res.w, res.y));
+    }
+}
+
+// ---- demosaic ------------------------------------------------------------
+
+struct Var2 {
+    float a, b;
+};
+
+// interpolateGreen at one pixel (demosaic.metal:214-300).
+__device__ float green_at(const View<const float>& R, const View<const float2>& Gr, Quad q, Var2 gv, int x, int y) {
+    const bool red = (q.o[kRed].x & 1) == (x & 1) && (q.o[kRed].y & 1) == (y & 1);
+    const bool blue = (q.o[kBlue].x & 1) == (x & 1) && (q.o[kBlue].y & 1) == (y & 1);
+    if (!(red || blue)) return R.at(x, y);
+    const float low_noise = 1.0f - smoothstepf(3.5e-4f, 2e-3f, gv.b);
+    const float gl = fetch(R, x - 1, y), gr = fetch(R, x + 1, y), gu = fetch(R, x, y - 1), gd = fetch(R, x, y + 1);
+    const float c = R.at(x, y);
+    const float cl = fetch(R, x - 2, y), cr = fetch(R, x + 2, y), cu = fetch(R, x, y - 2), cd = fetch(R, x, y + 2);
+    const float d_ave = (fetch(R, x - 1, y - 1) + fetch(R, x + 1, y - 1) + fetch(R, x - 1, y + 1) + fetch(R, x + 1, y + 1)) / 4;
+    const float g_ave = (gl + gr + gu + gd) / 4;
+    const float2 g2 = Gr.at(x, y);
+    const float ax = fabsf(g2.x), ay = fabsf(g2.y);
+    const float lf_h = (gl + gr) / 2, lf_v = (gu + gd) / 2;
+    const float hf_h = ((cl + cr) - 2 * c) / 4, hf_v = ((cu + cd) - 2 * c) / 4;
+    const float sd = sqrtf(gv.a + gv.b * g_ave);
+    const float glen = sqrtf(ax * ax + ay * ay);
+    const float thr = smoothstepf(sd, 4 * sd, glen);
+    const float low_thr = 1.0f - smoothstepf(2 * sd, 8 * sd, glen);
+    const float sharpening = (0.5f + 0.5f * low_noise) * thr * (1 + low_noise * low_thr);
+    const float highlights = 1.0f - smoothstepf(0.25f, 1.0f, fmaxf(c, fmaxf(fmaxf(cl, cr), fmaxf(cu, cd))));
+    float dir = 2 * atan2f(ay, ax) / kPi;
+    if (gv.b < 1e-3f) {
+        dir = dir < 0.5f ? mixf(dir, 0.0f, 1.0f - smoothstepf(0.3f, 0.45f, dir))
+                         : mixf(dir, 1.0f, smoothstepf(1.0f - 0.45f, 1.0f - 0.3f, dir));
+    }
+    const float cmin = fminf(c, fminf(g_ave, d_ave)), cmax = fmaxf(c, fmaxf(g_ave, d_ave));
+    const float white = cmin / cmax;
+    const float k = highlights * white * sharpening;
+    const float est_h = lf_h - k * hf_h, est_v = lf_v - k * hf_v;
+    const float g = mixf(est_v, est_h, dir);
+    const float gmax = fmaxf(fmaxf(gl, gr), fmaxf(gu, gd)), gmin = fminf(fminf(gl, gr), fminf(gu, gd));
+    return clampf(g, mixf(1.0f, 0.5f, white) * gmin, mixf(1.0f, 1.5f, white) * gmax);
+}
+
+__global__ __launch_bounds__(256) void k_interpolate_green(View<const float> raw, View<const float2> grad,
+                                                           View<float> green, Quad q, Var2 gv) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= green.w || y >= green.h) return;
+    green.at(x, y) = green_at(raw, grad, q, gv, x, y);
+}
+
+// interpolateRedBluePixel (demosaic.metal:309-380): the missing colour at a
+// red (red_site) or blue site from the colour differences of its diagonal
+// neighbours, steered by the gradient direction.
+__device__ float4 red_blue_site(const View<const float>& R, const View<const float>& Gn, const View<const float2>& Gr,
+                                int x, int y, bool red_site, Var2 rv, Var2 bv) {
+    const float g = Gn.at(x, y), c1 = R.at(x, y);
+    const float g_tl = fetch(Gn, x - 1, y - 1), g_tr = fetch(Gn, x + 1, y - 1);
+    const float g_bl = fetch(Gn, x - 1, y + 1), g_br = fetch(Gn, x + 1, y + 1);
+    const float c_tl = fetch(R, x - 1, y - 1), c_tr = fetch(R, x + 1, y - 1);
+    const float c_bl = fetch(R, x - 1, y + 1), c_br = fetch(R, x + 1, y + 1);
+    const float c2_ave = (c_tl + c_tr + c_bl + c_br) / 4;
+    const float d_tl = g_tl - c_tl, d_tr = g_tr - c_tr, d_bl = g_bl - c_bl, d_br = g_br - c_br;
+    const float g2_tl = fetch(Gn, x - 2, y - 2), g2_tr = fetch(Gn, x + 2, y - 2);
+    const float g2_bl = fetch(Gn, x - 2, y + 2), g2_br = fetch(Gn, x + 2, y + 2);
+    const float d2_tl = g2_tl - fetch(R, x - 2, y - 2), d2_tr = g2_tr - fetch(R, x + 2, y - 2);
+    const float d2_bl = g2_bl - fetch(R, x - 2, y + 2), d2_br = g2_br - fetch(R, x + 2, y + 2);
+    const Var2 v = red_site ? rv : bv;
+    const float sd = sqrtf(v.a + v.b * c2_ave);
+    const float2 gg = Gr.at(x, y);
+    const float ax = fabsf(gg.x), ay = fabsf(gg.y);
+    const float dir = 1.0f - 2 * atan2f(ay, ax) / kPi;
+    const float thr = smoothstepf(sd, 4 * sd, sqrtf(ax * ax + ay * ay));
+    const float alpha = mixf(0.5f, dir, thr);
+    const float highlights = 1.0f - smoothstepf(0.25f, 1.0f, fmaxf(g, fmaxf(fmaxf(g2_tr, g2_bl), fmaxf(g2_tl, g2_br))));
+    const float centre = 2 * (g - c1);
+    const float anti = (d_tr + d_bl) / 2 + highlights * (d2_tr + d2_bl - centre) / 8;
+    const float diag = (d_tl + d_br) / 2 + highlights * (d2_tl + d2_br - centre) / 8;
+    float c2 = g - mixf(anti, diag, alpha);
+    const float c2max = fmaxf(fmaxf(c_tl, c_tr), fmaxf(c_bl, c_br));
+    const float c2min = fminf(fminf(c_tl, c_tr), fminf(c_bl, c_br));
+    c2 = clampf(c2, c2min, c2max);
+    return red_site ? make_float4(c1, g, c2, 0.0f) : make_float4(c2, g, c1, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_interpolate_red_blue(View<const float> raw, View<const float> green,
+                                                              View<const float2> grad, View<float4> rgb, Quad q,
+                                                              Var2 rv, Var2 bv) {
+    const int qx = blockIdx.x * blockDim.x + threadIdx.x, qy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (qx >= rgb.w / 2 || qy >= rgb.h / 2) return;
+    const int x = 2 * qx, y = 2 * qy;
+    rgb.at(x + q.o[kRed].x, y + q.o[kRed].y) =
+        red_blue_site(raw, green, grad, x + q.o[kRed].x, y + q.o[kRed].y, true, rv, bv);
+    rgb.at(x + q.o[kBlue].x, y + q.o[kBlue].y) =
+        red_blue_site(raw, green, grad, x + q.o[kBlue].x, y + q.o[kBlue].y, false, rv, bv);
+    for (int c : {kGreen, kGreen2}) {
+        const int px = x + q.o[c].x, py = y + q.o[c].y;
+        rgb.at(px, py) = make_float4(0.0f, green.at(px, py), 0.0f, 0.0f);
+    }
+}
+
+// interpolateRedBlueAtGreenPixel (demosaic.metal:420-502).
+__device__ float4 red_blue_at_green_site(const View<const float4>& C, const View<const float2>& Gr, int x, int y,
+                                         Var2 rv, Var2 bv) {
+    const float gc = C.at(x, y).y;
+    const float4 l = fetch(C, x - 1, y), r = fetch(C, x + 1, y), u = fetch(C, x, y - 1), d = fetch(C, x, y + 1);
+    const float4 l3 = fetch(C, x - 3, y), r3 = fetch(C, x + 3, y), u3 = fetch(C, x, y - 3), d3 = fetch(C, x, y + 3);
+    const float red_ave = (l.x + r.x + u.x + d.x) / 4, blue_ave = (l.z + r.z + u.z + d.z) / 4;
+    const float dr_l = l.y - l.x, dr_r = r.y - r.x, dr_u = u.y - u.x, dr_d = d.y - d.x;
+    const float db_l = l.y - l.z, db_r = r.y - r.z, db_u = u.y - u.z, db_d = d.y - d.z;
+    const float dr_l3 = l3.y - l3.x, dr_r3 = r3.y - r3.x, dr_u3 = u3.y - u3.x, dr_d3 = d3.y - d3.x;
+    const float db_l3 = l3.y - l3.z, db_r3 = r3.y - r3.z, db_u3 = u3.y - u3.z, db_d3 = d3.y - d3.z;
+    const float2 gg = Gr.at(x, y);
+    const float ax = fabsf(gg.x), ay = fabsf(gg.y);
+    const float dir = 2 * atan2f(ay, ax) / kPi;
+    const float glen = sqrtf(ax * ax + ay * ay);
+    const float rsd = sqrtf(rv.a + rv.b * red_ave), bsd = sqrtf(bv.a + bv.b * blue_ave);
+    const float ralpha = mixf(0.5f, dir, smoothstepf(rsd, 4 * rsd, glen));
+    const float balpha = mixf(0.5f, dir, smoothstepf(bsd, 4 * bsd, glen));
+    const float highlights = 1.0f - smoothstepf(0.25f, 1.0f, fmaxf(gc, fmaxf(fmaxf(r.y, l.y), fmaxf(d.y, u.y))));
+    const float red = gc - mixf((dr_u + dr_d) / 2 - highlights * ((dr_d3 - dr_u) - (dr_d - dr_u3)) / 8,
+                                (dr_l + dr_r) / 2 - highlights * ((dr_r3 - dr_l) - (dr_r - dr_l3)) / 8, ralpha);
+    const float blue = gc - mixf((db_u + db_d) / 2 - highlights * ((db_d3 - db_u) - (db_d - db_u3)) / 8,
+                                 (db_l + db_r) / 2 - highlights * ((db_r3 - db_l) - (db_r - db_l3)) / 8, balpha);
+    return make_float4(clampf(red, fminf(fminf(l.x, r.x), fminf(u.x, d.x)), fmaxf(fmaxf(l.x, r.x), fmaxf(u.x, d.x))),
+                       gc,
+                       clampf(blue, fminf(fminf(l.z, r.z), fminf(u.z, d.z)), fmaxf(fmaxf(l.z, r.z), fmaxf(u.z, d.z))),
+                       0.0f);
+}
+
+// blendHighlightsImage at one pixel (demosaic.metal:523-564): above `clip`,
+// keep the sum of channels and scale the opponent chroma to the length of
+// the clipped pixel's.
+__device__ __forceinline__ float4 blend_highlight(float4 p, float clip) {
+    if (!(p.x > clip || p.y > clip || p.z > clip)) return make_float4(p.x, p.y, p.z, 0.0f);
+    const float s3 = 1.7320508f;
+    const float cx = fminf(p.x, clip), cy = fminf(p.y, clip), cz = fminf(p.z, clip);
+    // trans rows: (1, 1, 1), (s3, -s3, 0), (-1, -1, 2)
+    const float a0 = 1 * p.x + 1 * p.y + 1 * p.z;
+    float a1 = s3 * p.x + -s3 * p.y + 0 * p.z;
+    float a2 = -1 * p.x + -1 * p.y + 2 * p.z;
+    const float b1 = s3 * cx + -s3 * cy + 0 * cz;
+    const float b2 = -1 * cx + -1 * cy + 2 * cz;
+    const float sum0 = a1 * a1 + a2 * a2, sum1 = b1 * b1 + b2 * b2;
+    const float ratio = sum0 > 0 ? sqrtf(sum1 / sum0) : 1.0f;
+    a1 *= ratio;
+    a2 *= ratio;
+    // itrans rows: (1, s3/2, -0.5), (1, -s3/2, -0.5), (1, 0, 1)
+    return make_float4((1 * a0 + s3 / 2 * a1 + -0.5f * a2) / 3, (1 * a0 + -s3 / 2 * a1 + -0.5f * a2) / 3,
+                       (1 * a0 + 0 * a1 + 1 * a2) / 3, 0.0f);
+}
+
+struct Mat3 {
+    float m[9];
+};
+
+__device__ __forceinline__ float4 transform3(const Mat3& t, float4 p) {
+    return make_float4(t.m[0] * p.x + t.m[1] * p.y + t.m[2] * p.z, t.m[3] * p.x + t.m[4] * p.y + t.m[5] * p.z,
+                       t.m[6] * p.x + t.m[7] * p.y + t.m[8] * p.z, 0.0f);
+}
+
+// interpolateRedBlueAtGreen, one quad per thread; with kTail the quad's four
+// pixels then go through blendHighlightsImage and transformImage (the fused
+// front end) instead of being written as linear RGB.
+template <bool kTail>
+__global__ __launch_bounds__(256) void k_interpolate_red_blue_at_green(View<const float4> in, View<const float2> grad,
+                                                                       View<float4> out, Quad q, Var2 rv, Var2 bv,
+                                                                       float clip, Mat3 t) {
+    const int qx = blockIdx.x * blockDim.x + threadIdx.x, qy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (qx >= out.w / 2 || qy >= out.h / 2) return;
+    const int x = 2 * qx, y = 2 * qy;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int px = x + q.o[c].x, py = y + q.o[c].y;
+        float4 v = (c == kGreen || c == kGreen2) ? red_blue_at_green_site(in, grad, px, py, rv, bv) : in.at(px, py);
+        if (kTail) v = transform3(t, blend_highlight(v, clip));
+        out.at(px, py) = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_blend_highlights(View<const float4> in, View<float4> out, float clip) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    out.at(x, y) = blend_highlight(in.at(x, y), clip);
+}
+
+__global__ __launch_bounds__(256) void k_transform(View<const float4> in, View<float4> out, Mat3 t) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    out.at(x, y) = transform3(t, in.at(x, y));
+}
+
+// ---- despeckleLumaMedianChromaImage --------------------------------------
+
+__device__ __forceinline__ void sort2(float& a, float& b) {
+    const float lo = fminf(a, b), hi = fmaxf(a, b);
+    a = lo;
+    b = hi;
+}
+
+// Median of nine by the forgetful selection of demosaic.metal:1162-1192
+// (each pass drops the minimum and maximum of what is left).
+__device__ __forceinline__ float median9(float v[9]) {
+    float a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3], a4 = v[4], a5 = v[5];
+    sort2(a0, a1); sort2(a2, a3); sort2(a4, a5); sort2(a0, a2); sort2(a1, a3); sort2(a0, a4); sort2(a3, a5);
+    a0 = v[6];
+    sort2(a0, a1); sort2(a2, a3); sort2(a0, a2); sort2(a1, a3); sort2(a0, a4); sort2(a3, a4);
+    a0 = v[7];
+    sort2(a0, a1); sort2(a2, a3); sort2(a0, a2); sort2(a1, a3);
+    a0 = v[8];
+    sort2(a0, a1); sort2(a0, a2); sort2(a1, a2);
+    return a1;
+}
+
+__global__ __launch_bounds__(256) void k_despeckle(View<const float4> in, View<float4> out, Var2 yv) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    // The reference's reading order: the right two columns top to bottom,
+    // then (-1, 1), (-1, 0), (-1, -1).
+    const int ox[9] = {0, 1, 0, 1, 0, 1, -1, -1, -1}, oy[9] = {-1, -1, 0, 0, 1, 1, 1, 0, -1};
+    float cb[9], cr[9];
+    float max1 = 0.0f, max2 = 0.0f, min1 = 65535.0f, min2 = 65535.0f;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const float4 p = fetch(in, x + ox[k], y + oy[k]);
+        cb[k] = p.y;
+        cr[k] = p.z;
+        const float v = p.x;
+        max2 = (v <= max1 && v > max2) ? v : max2;
+        max2 = v > max1 ? max1 : max2;
+        max1 = v > max1 ? v : max1;
+        min2 = (v >= min1 && v < min2) ? v : min2;
+        min2 = v < min1 ? min1 : min2;
+        min1 = v < min1 ? v : min1;
+    }
+    const float centre = in.at(x, y).x;
+    const float sigma = sqrtf(yv.a + yv.b * centre);
+    const float lo = mixf(min2, min1, smoothstepf(sigma, 4 * sigma, min2 - min1));
+    const float hi = mixf(max2, max1, smoothstepf(sigma, 4 * sigma, max1 - max2));
+    out.at(x, y) = make_float4(clampf(centre, lo, hi), median9(cb), median9(cr), 0.0f);
+}
+
+// ---- host side -----------------------------------------------------------
+
+// gaussianKernelBilinearWeights (demosaic_utils.cpp:987-1006) with
+// KernelOptimizeBilinear2d (:939-985): the k x k Gaussian (k = max(ceil(2r),
+// 2) made odd) folded into ((k / 2) + 1)^2 bilinear taps {weight, x, y}; the
+// corner tap sits at (k / 2.0, k / 2.0) as the reference places it.
+std::vector<float> bilinear_gaussian(float radius) {
+    int k = std::max((int)std::ceil(2 * radius), 2);
+    if (k % 2 == 0) k++;
+    const int half = k / 2, ow = half + 1;
+    std::vector<float> g((size_t)k * k), taps((size_t)ow * ow * 3);
+    for (int y = 0; y < k; y++)
+        for (int x = 0; x < k; x++) {
+            const int dx = x - half, dy = y - half;
+            g[(size_t)y * k + x] = std::exp(-((float)(dx * dx + dy * dy) / (2 * radius * radius)));
+        }
+    auto G = [&](int r, int c) { return g[(size_t)r * k + c]; };
+    auto set = [&](int r, int c, float wt, float ox, float oy) {
+        float* t = &taps[((size_t)(r / 2) * ow + c / 2) * 3];
+        t[0] = wt;
+        t[1] = ox;
+        t[2] = oy;
+    };
+    // The last row / column index after the pair loops is k - 1.
+    for (int r = 0; r < k; r += 2) {
+        for (int c = 0; c < k; c += 2) {
+            const bool rp = r + 1 < k, cp = c + 1 < k;  // a pair of rows / columns
+            const float x1 = (float)(c - half), y1 = (float)(r - half);
+            if (rp && cp) {
+                const float w1 = G(r, c), w2 = G(r, c + 1), w3 = G(r + 1, c), w4 = G(r + 1, c + 1);
+                set(r, c, w1 + w2 + w3 + w4, (x1 * w1 + (x1 + 1.0f) * w2) / (w1 + w2),
+                    (y1 * w1 + (y1 + 1.0f) * w3) / (w1 + w3));
+            } else if (rp) {
+                const float w1 = G(r, c), w2 = G(r + 1, c);
+                set(r, c, w1 + w2, x1, (y1 * w1 + (y1 + 1.0f) * w2) / (w1 + w2));
+            } else if (cp) {
+                const float w1 = G(r, c), w2 = G(r, c + 1);
+                set(r, c, w1 + w2, (x1 * w1 + (x1 + 1.0f) * w2) / (w1 + w2), y1);
+            } else {
+                set(r, c, G(r, c), k / 2.0f, k / 2.0f);
+            }
+        }
+    }
+    return taps;
+}
+
+// Taps of one kernel as integer offset + exact fraction; false if they do not
+// fit the device table or reach beyond the tile apron.
+bool make_taps(float radius, Tap* out, int cap, int* n) {
+    if (!(radius > 0.0f) || !std::isfinite(radius)) return false;
+    const std::vector<float> t = bilinear_gaussian(radius);
+    const int m = (int)t.size() / 3;
+    if (m > cap) return false;
+    for (int i = 0; i < m; i++) {
+        const float fx0 = std::floor(t[3 * i + 1]), fy0 = std::floor(t[3 * i + 2]);
+        Tap& d = out[i];
+        d.w = t[3 * i];
+        d.ix = (int)fx0;
+        d.iy = (int)fy0;
+        d.fx = t[3 * i + 1] - fx0;
+        d.fy = t[3 * i + 2] - fy0;
+        if (d.ix < -kBApL || d.ix + 1 > kBApR || d.iy < -kBApL || d.iy + 1 > kBApR) return false;
+    }
+    *n = m;
+    return true;
+}
+
+dim3 grid_for(int w, int h, dim3 b) { return dim3((w + b.x - 1) / b.x, (h + b.y - 1) / b.y); }
+
+bool valid_image(const gcam_image* im, int fmt) {
+    return im && im->data && im->width > 0 && im->height > 0 && im->stride >= im->width && im->format == fmt;
+}
+bool same_size(const gcam_image* a, const gcam_image* b) { return a->width == b->width && a->height == b->height; }
+
+gcam_status bad(const char* what) { return set_error(GCAM_ERR_INVALID_ARGUMENT, what); }
+
+gcam_status launch_status(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(GCAM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return GCAM_OK;
+}
+
+}  // namespace
+
+}  // namespace gcam
+
+using namespace gcam;
+
+namespace {
+const dim3 kBlock(64, 4);
+}
+
+extern "C" {
+
+int32_t gcam_gaussian_kernel_bilinear_weights(float radius, float* taps, int32_t capacity) {
+    if (!(radius > 0.0f) || !std::isfinite(radius) || radius > 64.0f) return -1;
+    const std::vector<float> t = bilinear_gaussian(radius);
+    const int n = (int)t.size() / 3;
+    if (taps && capacity >= n) std::copy(t.begin(), t.end(), taps);
+    return n;
+}
+
+void gcam_raw_variance(const float raw_nlf_a[4], const float raw_nlf_b[4], float out[6]) {
+    out[0] = raw_nlf_a[0];
+    out[1] = raw_nlf_b[0];
+    out[2] = (raw_nlf_a[1] + raw_nlf_a[3]) / 2;
+    out[3] = (raw_nlf_b[1] + raw_nlf_b[3]) / 2;
+    out[4] = raw_nlf_a[2];
+    out[5] = raw_nlf_b[2];
+}
+
+void gcam_cam_ycbcr(const float rgb_cam[9], float out[9]) {
+    const float yr[3] = {0.2126f, 0.7152f, 0.0722f};  // row 1 of xyz_sRGB (demosaic.hpp:138-142)
+    float K[3];
+    for (int j = 0; j < 3; j++) K[j] = yr[0] * rgb_cam[j] + yr[1] * rgb_cam[3 + j] + yr[2] * rgb_cam[6 + j];
+    const float kr = K[0], kg = K[1], kb = K[2];
+    const float m[9] = {kr, kg, kb, -0.5f * kr / (1.0f - kb), -0.5f * kg / (1 - kb), 0.5f,
+                        0.5f, -0.5f * kg / (1 - kr), -0.5f * kb / (1 - kr)};
+    std::copy(m, m + 9, out);
+}
+
+gcam_status gcam_matrix3_inverse(const float m[9], float out[9]) {
+    if (!m || !out) return bad("gcam_matrix3_inverse: null matrix");
+    double a[9];
+    for (int i = 0; i < 9; i++) a[i] = m[i];
+    const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    if (det == 0.0 || !std::isfinite(det)) return bad("gcam_matrix3_inverse: singular matrix");
+    const double r = 1.0 / det;
+    const double inv[9] = {c00 * r, (a[2] * a[7] - a[1] * a[8]) * r, (a[1] * a[5] - a[2] * a[4]) * r,
+                           c01 * r, (a[0] * a[8] - a[2] * a[6]) * r, (a[2] * a[3] - a[0] * a[5]) * r,
+                           c02 * r, (a[1] * a[6] - a[0] * a[7]) * r, (a[0] * a[4] - a[1] * a[3]) * r};
+    for (int i = 0; i < 9; i++) out[i] = (float)inv[i];
+    return GCAM_OK;
+}
+
+gcam_status gcam_scale_raw_data(const gcam_image* raw, gcam_image* scaled, int32_t bayer, const float scale_mul[4],
+                                float black_level, float lens_shading, void* stream) {
+    if (!valid_image(raw, GCAM_PIXEL_R16U) || !valid_image(scaled, GCAM_PIXEL_R32F) || !same_size(raw, scaled))
+        return bad("gcam_scale_raw_data: raw must be R16U and scaled R32F of the same size");
+    if ((raw->width | raw->height) & 1) return bad("gcam_scale_raw_data: width and height must be even");
+    if (bayer < 0 || bayer > 3 || !scale_mul) return bad("gcam_scale_raw_data: bad Bayer pattern or scale_mul");
+    const dim3 g = grid_for(raw->width / 2, raw->height / 2, kBlock);
+    k_scale_raw<<<g, kBlock, 0, (hipStream_t)stream>>>(view<const uint16_t>(*raw), view<float>(*scaled),
+                                                        quad_of(bayer),
+                                                        make_float4(scale_mul[0], scale_mul[1], scale_mul[2], scale_mul[3]),
+                                                        black_level, lens_shading);
+    return launch_status("scaleRawData");
+}
+
+gcam_status gcam_raw_image_sobel(const gcam_image* scaled, gcam_image* sobel, void* stream) {
+    if (!valid_image(scaled, GCAM_PIXEL_R32F) || !valid_image(sobel, GCAM_PIXEL_RGBA32F) || !same_size(scaled, sobel))
+        return bad("gcam_raw_image_sobel: scaled must be R32F and sobel RGBA32F of the same size");
+    k_raw_sobel<<<grid_for(sobel->width, sobel->height, kBlock), kBlock, 0, (hipStream_t)stream>>>(
+        view<const float>(*scaled), view<float4>(*sobel));
+    return launch_status("rawImageSobel");
+}
+
+static gcam_status blur_sobel(const gcam_image* scaled, const gcam_image* sobel, float radius1, float radius2,
+                              const float raw_variance[2], gcam_image* gradient, hipStream_t s) {
+    BlurTaps taps{};
+    if (!make_taps(radius1, taps.t1, kMaxTaps1, &taps.n1) || !make_taps(radius2, taps.t2, kMaxTaps2, &taps.n2))
+        return bad("gaussianBlurSobelImage: radius1 must be in (0, 1.5] and radius2 in (0, 5.5]");
+    const dim3 g((gradient->width + kBTW - 1) / kBTW, (gradient->height + kBTH - 1) / kBTH);
+    const dim3 b(kBTW, 4);
+    if (sobel) {
+        k_blur_sobel<false><<<g, b, 0, s>>>(view<const float>(*scaled), view<const float4>(*sobel), taps,
+                                            raw_variance[0], raw_variance[1], view<float2>(*gradient));
+    } else {
+        k_blur_sobel<true><<<g, b, 0, s>>>(view<const float>(*scaled), View<const float4>{nullptr, 0, 0, 0}, taps,
+                                           raw_variance[0], raw_variance[1], view<float2>(*gradient));
+    }
+    return launch_status("gaussianBlurSobelImage");
+}
+
+gcam_status gcam_gaussian_blur_sobel_image(const gcam_image* scaled, const gcam_image* sobel, float radius1,
+                                           float radius2, const float raw_variance[2], gcam_image* gradient,
+                                           void* stream) {
+    if (!valid_image(scaled, GCAM_PIXEL_R32F) || !valid_image(gradient, GCAM_PIXEL_RG32F) ||
+        !same_size(scaled, gradient) || !raw_variance)
+        return bad("gcam_gaussian_blur_sobel_image: scaled R32F, gradient RG32F of the same size, raw_variance");
+    if (sobel && (!valid_image(sobel, GCAM_PIXEL_RGBA32F) || !same_size(sobel, scaled)))
+        return bad("gcam_gaussian_blur_sobel_image: sobel must be RGBA32F of the image size (or NULL)");
+    return blur_sobel(scaled, sobel, radius1, radius2, raw_variance, gradient, (hipStream_t)stream);
+}
+
+static gcam_status demosaic(const gcam_image* scaled, const gcam_image* gradient, gcam_image* green,
+                            gcam_image* rgb_tmp, gcam_image* rgb_out, int32_t bayer, const float rv[6], bool tail,
+                            float clip, const Mat3& t, hipStream_t s) {
+    const Quad q = quad_of(bayer);
+    const Var2 red{rv[0], rv[1]}, gv{rv[2], rv[3]}, blue{rv[4], rv[5]};
+    const int w = scaled->width, h = scaled->height;
+    k_interpolate_green<<<grid_for(w, h, kBlock), kBlock, 0, s>>>(view<const float>(*scaled),
+                                                                  view<const float2>(*gradient), view<float>(*green),
+                                                                  q, gv);
+    gcam_status st = launch_status("interpolateGreen");
+    if (st != GCAM_OK) return st;
+    const dim3 gq = grid_for(w / 2, h / 2, kBlock);
+    k_interpolate_red_blue<<<gq, kBlock, 0, s>>>(view<const float>(*scaled), view<const float>(*green),
+                                                 view<const float2>(*gradient), view<float4>(*rgb_tmp), q, red, blue);
+    if ((st = launch_status("interpolateRedBlue")) != GCAM_OK) return st;
+    if (tail) {
+        k_interpolate_red_blue_at_green<true><<<gq, kBlock, 0, s>>>(view<const float4>(*rgb_tmp),
+                                                                    view<const float2>(*gradient),
+                                                                    view<float4>(*rgb_out), q, red, blue, clip, t);
+    } else {
+        k_interpolate_red_blue_at_green<false><<<gq, kBlock, 0, s>>>(view<const float4>(*rgb_tmp),
+                                                                     view<const float2>(*gradient),
+                                                                     view<float4>(*rgb_out), q, red, blue, clip, t);
+    }
+    return launch_status("interpolateRedBlueAtGreen");
+}
+
+gcam_status gcam_demosaic_image(const gcam_image* scaled, const gcam_image* gradient, gcam_image* green,
+                                gcam_image* rgb_tmp, gcam_image* rgb_out, int32_t bayer, const float raw_variance[6],
+                                void* stream) {
+    if (!valid_image(scaled, GCAM_PIXEL_R32F) || !valid_image(gradient, GCAM_PIXEL_RG32F) ||
+        !valid_image(green, GCAM_PIXEL_R32F) || !valid_image(rgb_tmp, GCAM_PIXEL_RGBA32F) ||
+        !valid_image(rgb_out, GCAM_PIXEL_RGBA32F) || !same_size(scaled, gradient) || !same_size(scaled, green) ||
+        !same_size(scaled, rgb_tmp) || !same_size(scaled, rgb_out) || !raw_variance)
+        return bad("gcam_demosaic_image: scaled R32F, gradient RG32F, green R32F, rgb_tmp / rgb_out RGBA32F, one size");
+    if (rgb_tmp->data == rgb_out->data) return bad("gcam_demosaic_image: rgb_tmp and rgb_out must differ");
+    if ((scaled->width | scaled->height) & 1) return bad("gcam_demosaic_image: width and height must be even");
+    if (bayer < 0 || bayer > 3) return bad("gcam_demosaic_image: bad Bayer pattern");
+    return demosaic(scaled, gradient, green, rgb_tmp, rgb_out, bayer, raw_variance, false, 1.0f, Mat3{},
+                    (hipStream_t)stream);
+}
+
+gcam_status gcam_blend_highlights_image(const gcam_image* in, float clip, gcam_image* out, void* stream) {
+    if (!valid_image(in, GCAM_PIXEL_RGBA32F) || !valid_image(out, GCAM_PIXEL_RGBA32F) || !same_size(in, out))
+        return bad("gcam_blend_highlights_image: RGBA32F images of one size");
+    k_blend_highlights<<<grid_for(out->width, out->height, kBlock), kBlock, 0, (hipStream_t)stream>>>(
+        view<const float4>(*in), view<float4>(*out), clip);
+    return launch_status("blendHighlightsImage");
+}
+
+gcam_status gcam_transform_image(const gcam_image* in, const float m[9], gcam_image* out, void* stream) {
+    if (!valid_image(in, GCAM_PIXEL_RGBA32F) || !valid_image(out, GCAM_PIXEL_RGBA32F) || !same_size(in, out) || !m)
+        return bad("gcam_transform_image: RGBA32F images of one size and a 3x3 matrix");
+    Mat3 t;
+    std::copy(m, m + 9, t.m);
+    k_transform<<<grid_for(out->width, out->height, kBlock), kBlock, 0, (hipStream_t)stream>>>(
+        view<const float4>(*in), view<float4>(*out), t);
+    return launch_status("transformImage");
+}
+
+gcam_status gcam_despeckle_image(const gcam_image* in, const float var_a[3], const float var_b[3], gcam_image* out,
+                                 void* stream) {
+    if (!valid_image(in, GCAM_PIXEL_RGBA32F) || !valid_image(out, GCAM_PIXEL_RGBA32F) || !same_size(in, out) ||
+        !var_a || !var_b)
+        return bad("gcam_despeckle_image: RGBA32F images of one size and the noise model");
+    if (in->data == out->data) return bad("gcam_despeckle_image: in place is not supported");
+    k_despeckle<<<grid_for(out->width, out->height, kBlock), kBlock, 0, (hipStream_t)stream>>>(
+        view<const float4>(*in), view<float4>(*out), Var2{var_a[0], var_b[0]});
+    return launch_status("despeckleLumaMedianChromaImage");
+}
+
+// ---- the front end as one call -------------------------------------------
+
+struct gcam_front_end {
+    int w, h, device;
+    float* scaled;
+    float* green;
+    float4* rgb;  // interpolateRedBlue output
+    float4* ycc;  // before the despeckle
+};
+
+gcam_status gcam_front_end_create(int32_t width, int32_t height, gcam_front_end** out) {
+    if (!out) return bad("gcam_front_end_create: null output");
+    *out = nullptr;
+    if (width < 2 || height < 2 || ((width | height) & 1) || (int64_t)width * height > (int64_t)1 << 30)
+        return bad("gcam_front_end_create: width and height must be even, >= 2");
+    auto* f = new gcam_front_end{};
+    f->w = width;
+    f->h = height;
+    hipGetDevice(&f->device);
+    const size_t n = (size_t)width * height;
+    if (hipMalloc(&f->scaled, n * sizeof(float)) != hipSuccess || hipMalloc(&f->green, n * sizeof(float)) != hipSuccess ||
+        hipMalloc(&f->rgb, n * sizeof(float4)) != hipSuccess || hipMalloc(&f->ycc, n * sizeof(float4)) != hipSuccess) {
+        gcam_front_end_destroy(f);
+        return set_error(GCAM_ERR_OUT_OF_MEMORY, "gcam_front_end_create: device allocation failed");
+    }
+    *out = f;
+    return GCAM_OK;
+}
+
+void gcam_front_end_destroy(gcam_front_end* f) {
+    if (!f) return;
+    hipFree(f->scaled);
+    hipFree(f->green);
+    hipFree(f->rgb);
+    hipFree(f->ycc);
+    delete f;
+}
+
+gcam_status gcam_front_end_run(gcam_front_end* f, const gcam_image* raw, const gcam_front_end_params* p,
+                               gcam_image* ycbcr, gcam_image* gradient, void* stream) {
+    if (!f || !p) return bad("gcam_front_end_run: null handle or parameters");
+    if (!valid_image(raw, GCAM_PIXEL_R16U) || raw->width != f->w || raw->height != f->h)
+        return bad("gcam_front_end_run: raw must be R16U of the handle's size");
+    if (!valid_image(ycbcr, GCAM_PIXEL_RGBA32F) || !same_size(raw, ycbcr) || !valid_image(gradient, GCAM_PIXEL_RG32F) ||
+        !same_size(raw, gradient))
+        return bad("gcam_front_end_run: ycbcr (RGBA32F) and gradient (RG32F) must have the raw's size");
+    if (p->bayer_pattern < 0 || p->bayer_pattern > 3) return bad("gcam_front_end_run: bad Bayer pattern");
+    const hipStream_t s = (hipStream_t)stream;
+    const int w = f->w, h = f->h;
+    gcam_image sc{f->scaled, w, h, w, GCAM_PIXEL_R32F}, gr{f->green, w, h, w, GCAM_PIXEL_R32F};
+    gcam_image rgb{f->rgb, w, h, w, GCAM_PIXEL_RGBA32F}, pre{f->ycc, w, h, w, GCAM_PIXEL_RGBA32F};
+    gcam_status st = gcam_scale_raw_data(raw, &sc, p->bayer_pattern, p->scale_mul, p->black_level, p->lens_shading, s);
+    if (st != GCAM_OK) return st;
+    float rv[6];
+    gcam_raw_variance(p->raw_nlf_a, p->raw_nlf_b, rv);
+    if ((st = blur_sobel(&sc, nullptr, 1.5f, 4.5f, rv + 2, gradient, s)) != GCAM_OK) return st;
+    Mat3 t;
+    gcam_cam_ycbcr(p->rgb_cam, t.m);
+    gcam_image* tail_out = p->despeckle ? &pre : ycbcr;
+    if ((st = demosaic(&sc, gradient, &gr, &rgb, tail_out, p->bayer_pattern, rv, true, p->highlight_clip, t, s)) !=
+        GCAM_OK)
+        return st;
+    if (p->despeckle) return gcam_despeckle_image(&pre, p->despeckle_nlf.a, p->despeckle_nlf.b, ycbcr, s);
+    return GCAM_OK;
+}
+
+}  // extern "C"

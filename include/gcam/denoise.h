@@ -42,8 +42,9 @@ typedef enum gcam_pixel_format {
     GCAM_PIXEL_RG32F = 1,    /* gls::pixel_float2: gradient                    */
     GCAM_PIXEL_R32F = 2,     /* gls::pixel_float                               */
     GCAM_PIXEL_PCA8_F32 = 3, /* 8 x float PCA features (32 B per pixel)        */
-    GCAM_PIXEL_PCA8_F16 = 4  /* 8 x half in a uint4 texel, the reference's
+    GCAM_PIXEL_PCA8_F16 = 4, /* 8 x half in a uint4 texel, the reference's
                                 pcaImage format (demosaic.metal:721-741)       */
+    GCAM_PIXEL_R16U = 5      /* gls::luma_pixel_16: 16-bit raw Bayer data      */
 } gcam_pixel_format;
 
 /* A device image: gls::mtl_image_2d<T> (MetalPipeline/gls_mtl_image.hpp:116-208).

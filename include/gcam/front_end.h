@@ -1,0 +1,129 @@
+/*
+ * gcam front end -- C ABI of the raw front end that produces the inputs of
+ * the pyramid denoise (gcam/denoise.h) on MI355X (gfx950): the steps of
+ * RawConverter::demosaic up to the YCbCr image and the gradient image
+ * (MetalPipeline/raw_converter.cpp:148-214), and the despeckle that
+ * RawConverter::denoise runs before PyramidProcessor::denoise
+ * (raw_converter.cpp:70-78).
+ *
+ * Same conventions as gcam/denoise.h: device images, stream-ordered
+ * asynchronous calls, status codes, gcam_last_error() for the message.
+ * Images of one call share one size; raw frames have even width and height.
+ * Paths are relative to the GlassCameraPublic repository.
+ */
+#ifndef GCAM_FRONT_END_H
+#define GCAM_FRONT_END_H
+
+#include "gcam/denoise.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* BayerPattern (MetalPipeline/demosaic.hpp, demosaic.metal:24-29). */
+typedef enum gcam_bayer_pattern {
+    GCAM_BAYER_GRBG = 0,
+    GCAM_BAYER_GBRG = 1,
+    GCAM_BAYER_RGGB = 2,
+    GCAM_BAYER_BGGR = 3
+} gcam_bayer_pattern;
+
+/* scaleRawDataKernel (MetalPipeline/demosaic_kernels.hpp:30-49, demosaic.metal:95-117):
+ * scaled = max(lens * scale_mul[c] * (raw / 65535 - black_level) * 0.9 + 0.1, 0)
+ * per quad colour c (red, green, blue, green2), lens = 1 + lens_shading * d^2
+ * at the quad's distance d from the centre.  black_level is normalised
+ * (DemosaicParameters.black_level / 0xffff, raw_converter.cpp:182).
+ * raw: GCAM_PIXEL_R16U; scaled: GCAM_PIXEL_R32F. */
+gcam_status gcam_scale_raw_data(const gcam_image* raw, gcam_image* scaled, int32_t bayer_pattern,
+                                const float scale_mul[4], float black_level, float lens_shading, void* stream);
+
+/* rawImageSobelKernel (demosaic_kernels.hpp:51-64, demosaic.metal:120-139):
+ * sobel = (gx, gy, |gx|, |gy|) of the scaled raw (RGBA32F). */
+gcam_status gcam_raw_image_sobel(const gcam_image* scaled, gcam_image* sobel, void* stream);
+
+/* gaussianKernelBilinearWeights (MetalPipeline/demosaic_utils.cpp:987-1006):
+ * writes the bilinear taps {weight, x offset, y offset} (3 floats each) when
+ * capacity >= the count; returns the count (-1 for a bad radius).  Host only. */
+int32_t gcam_gaussian_kernel_bilinear_weights(float radius, float* taps, int32_t capacity);
+
+/* gaussianBlurSobelImageKernel (demosaic_kernels.hpp:66-97, sampledConvolutionSobel
+ * demosaic.metal:188-210): blur the Sobel image with the radius1 kernel; where
+ * its gradient is below 4 sigma of the raw noise (raw_variance = {a, b},
+ * var = a + b * scaled), with the radius2 kernel.  gradient (RG32F) = the
+ * blurred (|gx|, |gy|) with the signs of the blurred (gx, gy).  sobel may be
+ * NULL: the Sobel filter of `scaled` is then computed inside the blur and
+ * never stored.  radius1 <= 1.5, radius2 <= 5.5 (the reference uses 1.5 and
+ * 4.5, raw_converter.hpp:134). */
+gcam_status gcam_gaussian_blur_sobel_image(const gcam_image* scaled, const gcam_image* sobel, float radius1,
+                                           float radius2, const float raw_variance[2], gcam_image* gradient,
+                                           void* stream);
+
+/* getRawVariance (MetalPipeline/demosaic.hpp:186-193): RawNLF (a[4], b[4]
+ * per quad colour) -> {red a, b, green a, b, blue a, b}.  Host only. */
+void gcam_raw_variance(const float raw_nlf_a[4], const float raw_nlf_b[4], float out[6]);
+
+/* demosaicImageKernel (demosaic_kernels.hpp:124-181): interpolateGreen ->
+ * green (R32F), interpolateRedBlue -> rgb_tmp, interpolateRedBlueAtGreen ->
+ * rgb_out (RGBA32F, linear camera RGB, w = 0) (demosaic.metal:214-521).
+ * raw_variance as returned by gcam_raw_variance. */
+gcam_status gcam_demosaic_image(const gcam_image* scaled, const gcam_image* gradient, gcam_image* green,
+                                gcam_image* rgb_tmp, gcam_image* rgb_out, int32_t bayer_pattern,
+                                const float raw_variance[6], void* stream);
+
+/* blendHighlightsImageKernel (demosaic_kernels.hpp:255-269, demosaic.metal:536-564). */
+gcam_status gcam_blend_highlights_image(const gcam_image* in, float clip, gcam_image* out, void* stream);
+
+/* transformImageKernel (demosaic_kernels.hpp:271-295, demosaic.metal:649-662):
+ * out.xyz = m (row-major 3x3) * in.xyz, out.w = 0. */
+gcam_status gcam_transform_image(const gcam_image* in, const float m[9], gcam_image* out, void* stream);
+
+/* despeckleImageKernel (demosaic_kernels.hpp:297-315, despeckleLumaMedianChromaImage
+ * demosaic.metal:1242-1283): chroma = 3x3 median, luma clamped between its
+ * neighbourhood's extremes by the noise model var = var_a + var_b * Y
+ * (channel 0 used).  in and out must differ. */
+gcam_status gcam_despeckle_image(const gcam_image* in, const float var_a[3], const float var_b[3], gcam_image* out,
+                                 void* stream);
+
+/* cam_ycbcr(rgb_cam, xyz_sRGB) (MetalPipeline/demosaic_utils.cpp:703-720,
+ * raw_converter.cpp:166): the camera-RGB -> YCbCr matrix, row-major.  Host only. */
+void gcam_cam_ycbcr(const float rgb_cam[9], float out[9]);
+
+/* inverse(cam_to_ycbcr) (raw_converter.cpp:169): 3x3 inverse, row-major.  Host only. */
+gcam_status gcam_matrix3_inverse(const float m[9], float out[9]);
+
+/* The front end as one call: RawConverter::demosaic from the raw frame to the
+ * YCbCr image (raw_converter.cpp:179-214; the high-noise raw denoise of
+ * :195-202 is not part of it), then RawConverter::denoise's despeckle
+ * (:70-78).  The Sobel filter is fused into the gradient blur, and
+ * interpolateRedBlueAtGreen, blendHighlightsImage and transformImage run as
+ * one pass. */
+typedef struct gcam_front_end_params {
+    int32_t bayer_pattern;       /* gcam_bayer_pattern                               */
+    float black_level;           /* normalised, as for gcam_scale_raw_data           */
+    float lens_shading;          /* DemosaicParameters.lensShadingCorrection         */
+    float scale_mul[4];          /* DemosaicParameters.scale_mul                     */
+    float raw_nlf_a[4];          /* NoiseModel.rawNlf.first                          */
+    float raw_nlf_b[4];          /* NoiseModel.rawNlf.second                         */
+    float rgb_cam[9];            /* DemosaicParameters.rgb_cam, row-major            */
+    float highlight_clip;        /* blendHighlightsImage clip; 1.0 in the reference  */
+    int32_t despeckle;           /* 1: run the despeckle with despeckle_nlf          */
+    gcam_ycbcr_nlf despeckle_nlf; /* NoiseModel.pyramidNlf[0]                        */
+} gcam_front_end_params;
+
+typedef struct gcam_front_end gcam_front_end;
+
+/* Allocates the intermediates (scaled raw, green, two RGBA images) for one
+ * frame size on the current device. */
+gcam_status gcam_front_end_create(int32_t width, int32_t height, gcam_front_end** out);
+void gcam_front_end_destroy(gcam_front_end* front_end);
+
+/* raw (R16U) -> ycbcr (RGBA32F, the image for gcam_pyramid_denoise) and
+ * gradient (RG32F, its gradient image).  One handle per stream. */
+gcam_status gcam_front_end_run(gcam_front_end* front_end, const gcam_image* raw, const gcam_front_end_params* params,
+                               gcam_image* ycbcr, gcam_image* gradient, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GCAM_FRONT_END_H */

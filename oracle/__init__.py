@@ -68,6 +68,18 @@ def _declare(L):
     L.oracle_eigh_jacobi.argtypes = [_fp, _i, _fp, _fp]
     L.oracle_fit_ycbcr_nlf.argtypes = [_fp, _i, _i, _d, _fp, _fp, _fp]
     L.oracle_fit_ycbcr_nlf.restype = _i
+    L.oracle_scale_raw.argtypes = [_fp, _i, _i, _i, _fp, _f, _f, _fp]
+    L.oracle_raw_sobel.argtypes = [_fp, _i, _i, _fp]
+    L.oracle_gaussian_bilinear_weights.argtypes = [_f, _fp, _i]
+    L.oracle_gaussian_bilinear_weights.restype = _i
+    L.oracle_blur_sobel.argtypes = [_fp, _fp, _i, _i, _fp, _i, _fp, _i, _f, _f, _fp]
+    L.oracle_interpolate_green.argtypes = [_fp, _fp, _i, _i, _i, _f, _f, _fp]
+    L.oracle_interpolate_red_blue.argtypes = [_fp, _fp, _fp, _i, _i, _i, _fp, _fp, _fp]
+    L.oracle_interpolate_red_blue_at_green.argtypes = [_fp, _fp, _i, _i, _i, _fp, _fp, _fp]
+    L.oracle_blend_highlights.argtypes = [_fp, _i, _i, _f, _fp]
+    L.oracle_transform.argtypes = [_fp, _i, _i, _fp, _fp]
+    L.oracle_despeckle.argtypes = [_fp, _i, _i, _fp, _fp, _fp]
+    L.oracle_cam_ycbcr.argtypes = [_fp, _fp]
     L.oracle_set_threads.argtypes = [_i]
     L.oracle_get_threads.restype = _i
 
@@ -295,3 +307,134 @@ def pyramid_loss_grad(image, clean, thr, chroma_boost, nlf, levels=5, radius=4, 
     loss = L.oracle_pyramid_loss_grad_f64(_p(image), _p(clean), w, h, levels, radius, _p(thr), _p(cb), _p(nl),
                                           _p(g), _p(den))
     return loss, g, den
+
+
+# ---------------------------------------------------------------------------
+# Raw front end (FP32): the steps of RawConverter::demosaic that produce the
+# denoise's YCbCr image and gradient, and the despeckle of
+# RawConverter::denoise (MetalPipeline/raw_converter.cpp:148-213, 70-78).
+# Images are (H, W) for one channel, (H, W, C) otherwise; raw is uint16.
+
+BAYER = {"grbg": 0, "gbrg": 1, "rggb": 2, "bggr": 3}
+
+
+def _f32(a):
+    return _c(a, np.float32)
+
+
+def _vec(v, n):
+    a = np.ascontiguousarray(np.asarray(v, np.float32).reshape(-1))
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def scale_raw(raw, bayer=2, scale_mul=(1.0, 1.0, 1.0, 1.0), black_level=0.0, lens_correction=0.0):
+    """scaleRawData, MetalPipeline/demosaic.metal:95-117."""
+    raw = np.ascontiguousarray(raw, np.uint16)
+    h, w = raw.shape
+    out = np.zeros((h, w), np.float32)
+    sm = _vec(scale_mul, 4)
+    lib().oracle_scale_raw(_p(raw), w, h, int(bayer), _p(sm), float(black_level), float(lens_correction), _p(out))
+    return out
+
+
+def raw_sobel(scaled):
+    """rawImageSobel, MetalPipeline/demosaic.metal:120-139: (gx, gy, |gx|, |gy|)."""
+    scaled = _f32(scaled)
+    h, w = scaled.shape
+    out = np.empty((h, w, 4), np.float32)
+    lib().oracle_raw_sobel(_p(scaled), w, h, _p(out))
+    return out
+
+
+def gaussian_bilinear_weights(radius):
+    """gaussianKernelBilinearWeights, MetalPipeline/demosaic_utils.cpp:987-1006: (n, 3) taps."""
+    n = lib().oracle_gaussian_bilinear_weights(float(radius), None, 0)
+    out = np.empty((n, 3), np.float32)
+    lib().oracle_gaussian_bilinear_weights(float(radius), _p(out), n)
+    return out
+
+
+def blur_sobel(scaled, sobel, raw_variance, radius1=1.5, radius2=4.5):
+    """sampledConvolutionSobel with the two kernels of gaussianBlurSobelImageKernel
+    (MetalPipeline/demosaic.metal:188-210, demosaic_kernels.hpp:66-97)."""
+    scaled, sobel = _f32(scaled), _f32(sobel)
+    h, w = scaled.shape
+    w1, w2 = gaussian_bilinear_weights(radius1), gaussian_bilinear_weights(radius2)
+    out = np.empty((h, w, 2), np.float32)
+    va, vb = (float(x) for x in raw_variance)
+    lib().oracle_blur_sobel(_p(scaled), _p(sobel), w, h, _p(w1), len(w1), _p(w2), len(w2), va, vb, _p(out))
+    return out
+
+
+def raw_variance(raw_nlf_a, raw_nlf_b):
+    """getRawVariance, MetalPipeline/demosaic.hpp:186-193: (red, green, blue) x (a, b)."""
+    a, b = np.asarray(raw_nlf_a, np.float32), np.asarray(raw_nlf_b, np.float32)
+    return np.array([[a[0], b[0]], [(a[1] + a[3]) / 2, (b[1] + b[3]) / 2], [a[2], b[2]]], np.float32)
+
+
+def demosaic(scaled, gradient, bayer, rvar):
+    """demosaicImageKernel, MetalPipeline/demosaic_kernels.hpp:124-181: interpolateGreen,
+    interpolateRedBlue, interpolateRedBlueAtGreen (demosaic.metal:214-521).
+    rvar = raw_variance(...).  Returns (green, rgb after red/blue, final rgb)."""
+    scaled, gradient = _f32(scaled), _f32(gradient)
+    h, w = scaled.shape
+    rv = np.ascontiguousarray(rvar, np.float32)
+    green = np.empty((h, w), np.float32)
+    lib().oracle_interpolate_green(_p(scaled), _p(gradient), w, h, int(bayer), float(rv[1, 0]), float(rv[1, 1]),
+                                   _p(green))
+    tmp = np.empty((h, w, 4), np.float32)
+    r, b = np.ascontiguousarray(rv[0]), np.ascontiguousarray(rv[2])
+    lib().oracle_interpolate_red_blue(_p(scaled), _p(green), _p(gradient), w, h, int(bayer), _p(r), _p(b), _p(tmp))
+    rgb = np.empty((h, w, 4), np.float32)
+    lib().oracle_interpolate_red_blue_at_green(_p(tmp), _p(gradient), w, h, int(bayer), _p(r), _p(b), _p(rgb))
+    return green, tmp, rgb
+
+
+def blend_highlights(img, clip=1.0):
+    """blendHighlightsImage, MetalPipeline/demosaic.metal:536-564."""
+    img = _f32(img)
+    h, w = img.shape[:2]
+    out = np.empty((h, w, 4), np.float32)
+    lib().oracle_blend_highlights(_p(img), w, h, float(clip), _p(out))
+    return out
+
+
+def transform(img, m):
+    """transformImage, MetalPipeline/demosaic.metal:649-662."""
+    img = _f32(img)
+    h, w = img.shape[:2]
+    out = np.empty((h, w, 4), np.float32)
+    lib().oracle_transform(_p(img), w, h, _p(_vec(m, 9)), _p(out))
+    return out
+
+
+def despeckle(img, var_a, var_b):
+    """despeckleLumaMedianChromaImage, MetalPipeline/demosaic.metal:1242-1283."""
+    img = _f32(img)
+    h, w = img.shape[:2]
+    out = np.empty((h, w, 4), np.float32)
+    lib().oracle_despeckle(_p(img), w, h, _p(_vec(var_a, 3)), _p(_vec(var_b, 3)), _p(out))
+    return out
+
+
+def cam_ycbcr(rgb_cam=np.eye(3)):
+    """cam_ycbcr(rgb_cam, xyz_sRGB), MetalPipeline/demosaic_utils.cpp:703-720."""
+    out = np.empty(9, np.float32)
+    lib().oracle_cam_ycbcr(_p(_vec(rgb_cam, 9)), _p(out))
+    return out.reshape(3, 3)
+
+
+def raw_front_end(raw, bayer=2, scale_mul=(1.0, 1.0, 1.0, 1.0), black_level=0.0, lens_correction=0.0,
+                  raw_nlf_a=(1e-5,) * 4, raw_nlf_b=(1e-3,) * 4, rgb_cam=np.eye(3), clip=1.0, despeckle_nlf=None):
+    """The denoise inputs from a raw frame (MetalPipeline/raw_converter.cpp:179-214 and 70-78):
+    returns (ycbcr image after the despeckle, gradient, ycbcr before it)."""
+    scaled = scale_raw(raw, bayer, scale_mul, black_level, lens_correction)
+    rv = raw_variance(raw_nlf_a, raw_nlf_b)
+    grad = blur_sobel(scaled, raw_sobel(scaled), rv[1])
+    _, _, rgb = demosaic(scaled, grad, bayer, rv)
+    ycc = transform(blend_highlights(rgb, clip), cam_ycbcr(rgb_cam))
+    if despeckle_nlf is None:
+        return ycc, grad, ycc
+    return despeckle(ycc, despeckle_nlf[0], despeckle_nlf[1]), grad, ycc

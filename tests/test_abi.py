@@ -9,13 +9,16 @@ import pytest
 import glasscamerapublic_amd as gcam
 from glasscamerapublic_amd import _lib
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "gcam", "denoise.h")
+HEADER_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "gcam")
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(gcam_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for f in sorted(os.listdir(HEADER_DIR)):
+        if f.endswith(".h"):
+            text = re.sub(r"/\*.*?\*/", "", open(os.path.join(HEADER_DIR, f)).read(), flags=re.S)
+            names.update(re.findall(r"\b(gcam_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_what_the_binding_exports():

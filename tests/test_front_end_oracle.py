@@ -1,0 +1,275 @@
+"""The raw front-end oracle pinned by the reference's own tables and by what
+the mathematics fixes (CPU only).
+
+The reference holds no test vectors for the front end; what it does hold are
+the sRGB YCbCr matrices (MetalPipeline/demosaic_utils.cpp:728-740), which
+cam_ycbcr with an identity camera matrix and its inverse must reproduce.  The
+rest is pinned by independent computations: a direct Gaussian convolution
+against the folded bilinear taps, closed-form gradients of ramps, exact
+demosaicing of grey ramps, solid colours and straight edges, invariants of the
+highlight blend, and sort-based medians and extremes for the despeckle.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import glasscamerapublic_amd as gcam
+import oracle
+
+# MetalPipeline/demosaic_utils.cpp:728-740
+SRGB_YCBCR = np.array([[0.2126, 0.7152, 0.0722], [-0.1146, -0.3854, 0.5], [0.5, -0.4542, -0.0458]])
+YCBCR_SRGB = np.array([[1, 0, 1.5748], [1, -0.1873, -0.4681], [1, 1.8556, 0]])
+
+
+def test_cam_ycbcr_reproduces_the_srgb_tables():
+    m = oracle.cam_ycbcr(np.eye(3))
+    np.testing.assert_allclose(m, SRGB_YCBCR, atol=6e-5)
+    np.testing.assert_allclose(np.linalg.inv(m.astype(np.float64)), YCBCR_SRGB, atol=6e-5)
+    # the library's host-side versions: same matrix, and its inverse
+    np.testing.assert_array_equal(gcam.cam_ycbcr(np.eye(3)), m)
+    np.testing.assert_allclose(gcam.inverse3(m), YCBCR_SRGB, atol=6e-5)
+    rgb_cam = np.array([[1.6, -0.4, -0.2], [-0.2, 1.4, -0.2], [0.0, -0.5, 1.5]])
+    np.testing.assert_array_equal(gcam.cam_ycbcr(rgb_cam), oracle.cam_ycbcr(rgb_cam))
+    np.testing.assert_allclose(gcam.inverse3(rgb_cam) @ rgb_cam, np.eye(3), atol=1e-6)
+
+
+def test_raw_variance_layout():
+    a, b = (1.0, 2.0, 3.0, 4.0), (5.0, 6.0, 7.0, 8.0)
+    want = np.array([[1, 5], [3, 7], [3, 7]], np.float32)
+    np.testing.assert_array_equal(oracle.raw_variance(a, b), want)
+    np.testing.assert_array_equal(gcam.getRawVariance((a, b)), want)
+
+
+@pytest.mark.parametrize("radius", [0.5, 1.5, 2.5, 4.5, 5.5])
+def test_bilinear_taps_reproduce_the_gaussian(radius):
+    taps = oracle.gaussian_bilinear_weights(radius)
+    np.testing.assert_array_equal(taps, gcam.gaussianKernelBilinearWeights(radius))  # library host code
+    k = max(int(np.ceil(2 * radius)), 2)
+    k += 1 - k % 2
+    half = k // 2
+    assert taps.shape == ((half + 1) ** 2, 3)
+    d = np.arange(-half, half + 1, dtype=np.float64)
+    g = np.exp(-(d[None, :] ** 2 + d[:, None] ** 2) / (2 * radius * radius))
+    np.testing.assert_allclose(taps[:, 0].sum(), g.sum(), rtol=1e-5)
+    # Blur a random 4-channel image with the oracle (wide kernel forced by a
+    # huge noise floor) and directly: every weight at its own texel, except the
+    # corner weight, which the reference places half a texel beyond the corner
+    # (offset k/2 + 1/2): there it averages the four texels around that point.
+    rng = np.random.default_rng(3)
+    h, w = 40, 48
+    img = rng.standard_normal((h, w, 4)).astype(np.float32)
+    out = oracle.blur_sobel(np.zeros((h, w), np.float32), img, (1e30, 0.0), radius1=0.5, radius2=radius)
+    pad = np.pad(img.astype(np.float64), ((half + 2, half + 2), (half + 2, half + 2), (0, 0)), mode="edge")
+    direct = np.zeros((h, w, 4))
+    for dy in range(-half, half + 1):
+        for dx in range(-half, half + 1):
+            wt = g[dy + half, dx + half]
+            if dx == half and dy == half:
+                s = 0.25 * sum(pad[half + 2 + dy + j:half + 2 + dy + j + h, half + 2 + dx + i:half + 2 + dx + i + w]
+                               for j in (0, 1) for i in (0, 1))
+            else:
+                s = pad[half + 2 + dy:half + 2 + dy + h, half + 2 + dx:half + 2 + dx + w]
+            direct += wt * s
+    direct /= g.sum()
+    want = np.stack([np.copysign(np.abs(direct[..., 2]), direct[..., 0]),
+                     np.copysign(np.abs(direct[..., 3]), direct[..., 1])], -1)
+    # channels z, w of a Sobel image are >= 0; here they are signed, so compare |.|
+    np.testing.assert_allclose(np.abs(out), np.abs(want), atol=2e-5)
+    np.testing.assert_array_equal(np.sign(out), np.sign(want))
+
+
+def test_blur_switches_to_the_wide_kernel_below_four_sigma():
+    rng = np.random.default_rng(4)
+    h, w = 24, 24
+    sob = np.abs(rng.standard_normal((h, w, 4))).astype(np.float32)
+    sob[..., 2:] = sob[..., :2]  # like a Sobel image of non-negative gradients: (x, y, |x|, |y|)
+    raw = np.full((h, w), 0.5, np.float32)
+    narrow = oracle.blur_sobel(raw, sob, (0.0, 0.0))   # sigma 0: never below
+    wide = oracle.blur_sobel(raw, sob, (1e30, 0.0))     # sigma huge: always below
+    assert not np.allclose(narrow, wide)
+    res1 = oracle.blur_sobel(raw, sob, (0.0, 0.0), radius1=1.5, radius2=1.5)
+    np.testing.assert_array_equal(res1, narrow)
+    # a threshold between: pixels take the narrow result where its gradient is >= 4 sigma
+    nx = oracle.blur_sobel(raw, sob, (0.0, 0.0), radius2=1.5)
+    mag = np.hypot(*[np.abs(nx[..., c]) for c in (0, 1)])
+    sigma2 = (np.median(mag) / 4) ** 2
+    mixed = oracle.blur_sobel(raw, sob, (sigma2, 0.0))
+    # the narrow blur's (x, y) magnitudes are those of channels z, w here (inputs >= 0)
+    take_narrow = mag >= 4 * np.sqrt(np.float32(sigma2))
+    np.testing.assert_array_equal(mixed[take_narrow], narrow[take_narrow])
+    np.testing.assert_array_equal(mixed[~take_narrow], wide[~take_narrow])
+
+
+def test_sobel_of_a_ramp_and_against_a_direct_convolution():
+    h, w = 20, 24
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    s = oracle.raw_sobel(0.01 * x + 0.02 * y)
+    inner = (slice(1, -1), slice(1, -1))
+    np.testing.assert_allclose(s[inner][..., 0], -8 * 0.01 / np.sqrt(4.5), rtol=1e-4)
+    np.testing.assert_allclose(s[inner][..., 1], -8 * 0.02 / np.sqrt(4.5), rtol=1e-4)
+    rng = np.random.default_rng(5)
+    img = rng.random((h, w)).astype(np.float32)
+    p = np.pad(img.astype(np.float64), 1, mode="edge")
+    sh = lambda dx, dy: p[1 + dy:1 + dy + h, 1 + dx:1 + dx + w]  # noqa: E731
+    gx = (sh(-1, -1) + 2 * sh(-1, 0) + sh(-1, 1) - sh(1, -1) - 2 * sh(1, 0) - sh(1, 1)) / np.sqrt(4.5)
+    gy = (sh(-1, -1) + 2 * sh(0, -1) + sh(1, -1) - sh(-1, 1) - 2 * sh(0, 1) - sh(1, 1)) / np.sqrt(4.5)
+    out = oracle.raw_sobel(img)
+    np.testing.assert_allclose(out[..., 0], gx, atol=1e-6)
+    np.testing.assert_allclose(out[..., 1], gy, atol=1e-6)
+    np.testing.assert_array_equal(out[..., 2], np.abs(out[..., 0]))
+    np.testing.assert_array_equal(out[..., 3], np.abs(out[..., 1]))
+
+
+@pytest.mark.parametrize("bayer", [0, 1, 2, 3])
+def test_scale_raw_closed_form(bayer):
+    rng = np.random.default_rng(6 + bayer)
+    h, w = 12, 16
+    raw = rng.integers(0, 4096, (h, w)).astype(np.uint16)
+    sm = np.array([2.1, 1.0, 1.7, 1.05], np.float32) * 16
+    out = oracle.scale_raw(raw, bayer, sm, black_level=0.002, lens_correction=0.3)
+    offs = [((1, 0), (0, 0), (0, 1), (1, 1)), ((0, 1), (0, 0), (1, 0), (1, 1)),
+            ((0, 0), (0, 1), (1, 1), (1, 0)), ((1, 1), (0, 1), (0, 0), (1, 0))][bayer]
+    cx, cy = w // 2, h // 2
+    for yy in range(h):
+        for xx in range(w):
+            c = [i for i, (ox, oy) in enumerate(offs) if ox == xx % 2 and oy == yy % 2][0]
+            qx, qy = xx - xx % 2, yy - yy % 2
+            d2 = ((qx - cx) ** 2 + (qy - cy) ** 2) / (cx * cx + cy * cy)
+            v = (1 + 0.3 * d2) * sm[c] * (raw[yy, xx] / 65535 - 0.002) * 0.9 + 0.1
+            assert abs(out[yy, xx] - max(v, 0.0)) < 2e-6
+
+
+def mosaic(rgb, bayer=2):
+    """Bayer mosaic of an (H, W, 3) image."""
+    offs = [((1, 0), (0, 0), (0, 1), (1, 1)), ((0, 1), (0, 0), (1, 0), (1, 1)),
+            ((0, 0), (0, 1), (1, 1), (1, 0)), ((1, 1), (0, 1), (0, 0), (1, 0))][bayer]
+    out = np.empty(rgb.shape[:2], np.float32)
+    for c, (ox, oy) in enumerate(offs):
+        out[oy::2, ox::2] = rgb[oy::2, ox::2, [0, 1, 2, 1][c]]
+    return out
+
+
+RV = np.array([[1e-5, 2e-4], [1e-5, 2e-4], [1e-5, 2e-4]], np.float32)
+
+
+@pytest.mark.parametrize("bayer", [0, 2, 3])
+def test_demosaic_is_exact_on_grey_ramps_and_solid_colours(bayer):
+    h, w = 32, 40
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    # Clamp-to-edge breaks the mosaic's parity at the border; the passes read
+    # +-2, +-2 and +-3 pixels, so border effects reach 7 pixels in.
+    inner = (slice(8, -8), slice(8, -8))
+    ramp = 0.2 + 0.01 * x + 0.005 * y
+    grad = np.zeros((h, w, 2), np.float32)
+    _, _, rgb = oracle.demosaic(mosaic(np.stack([ramp] * 3, -1), bayer), grad, bayer, RV)
+    for c in range(3):
+        np.testing.assert_allclose(rgb[inner][..., c], ramp[inner], atol=2e-6)
+    solid = np.empty((h, w, 3), np.float32)
+    solid[...] = (0.6, 0.35, 0.2)
+    _, _, rgb = oracle.demosaic(mosaic(solid, bayer), grad, bayer, RV)
+    np.testing.assert_allclose(rgb[inner][..., :3], solid[inner], atol=2e-6)
+
+
+def test_green_follows_the_edge_direction():
+    """A grey vertical step: at red and blue sites beside it the gradient is
+    horizontal, so green is interpolated vertically (along the edge) and is
+    exact; a horizontal step likewise uses the horizontal neighbours."""
+    h, w = 32, 32
+    for vertical in (True, False):
+        img = np.full((h, w), 0.2, np.float32)
+        if vertical:
+            img[:, 16:] = 0.7
+        else:
+            img[16:, :] = 0.7
+        raw = mosaic(np.stack([img] * 3, -1), 2)
+        grad = oracle.blur_sobel(raw, oracle.raw_sobel(raw), (1e-6, 1e-4))
+        green, _, _ = oracle.demosaic(raw, grad, 2, RV)
+        band = (slice(6, -6), slice(14, 18)) if vertical else (slice(14, 18), slice(6, -6))
+        np.testing.assert_allclose(green[band], img[band], atol=1e-6)
+
+
+def test_highlight_blend_invariants():
+    rng = np.random.default_rng(8)
+    px = np.concatenate([rng.uniform(0, 1, (50, 3)), rng.uniform(0.5, 2.0, (200, 3)),
+                         np.full((5, 3), 1.4)]).astype(np.float32)
+    img = np.concatenate([px, np.zeros((len(px), 1), np.float32)], 1).reshape(1, -1, 4)
+    out = oracle.blend_highlights(img, 1.0)[0, :, :3].astype(np.float64)
+    p = px.astype(np.float64)
+    low = (p <= 1.0).all(1)
+    np.testing.assert_array_equal(out[low], p[low])
+    T = np.array([[1, 1, 1], [np.sqrt(3), -np.sqrt(3), 0], [-1, -1, 2]])
+    hi = ~low
+    np.testing.assert_allclose(out[hi].sum(1), p[hi].sum(1), rtol=1e-5)      # luma kept
+    a, b, c = (T @ out[hi].T)[1:], (T @ p[hi].T)[1:], (T @ np.minimum(p[hi], 1.0).T)[1:]
+    cross = a[0] * b[1] - a[1] * b[0]
+    np.testing.assert_allclose(cross, 0, atol=1e-4)                           # hue kept
+    np.testing.assert_allclose(np.hypot(*a), np.hypot(*c), rtol=1e-4, atol=1e-5)  # clipped chroma length
+    np.testing.assert_allclose(out[-5:], 1.4, rtol=1e-6)                      # grey stays grey
+
+
+def test_transform_is_a_matrix_product():
+    rng = np.random.default_rng(9)
+    img = rng.random((6, 7, 4)).astype(np.float32)
+    m = rng.standard_normal((3, 3)).astype(np.float32)
+    out = oracle.transform(img, m)
+    np.testing.assert_allclose(out[..., :3], img[..., :3] @ m.T, atol=1e-6)
+    assert (out[..., 3] == 0).all()
+
+
+def test_despeckle_against_sort_based_statistics():
+    rng = np.random.default_rng(10)
+    h, w = 16, 18
+    img = rng.uniform(0.1, 0.9, (h, w, 4)).astype(np.float32)
+    img[4:7, 6:9, 0] = 0.5
+    img[5, 7, 0] = 0.56  # a speckle 1.7 sigma above a flat neighbourhood (sigma ~ 0.035)
+    va, vb = (1e-4, 1e-4, 1e-4), (2e-3, 2e-3, 2e-3)
+    out = oracle.despeckle(img, va, vb)
+    p = np.pad(img, ((1, 1), (1, 1), (0, 0)), mode="edge")
+    for y in range(h):
+        for x in range(w):
+            nb = p[y:y + 3, x:x + 3].reshape(9, 4)
+            assert out[y, x, 1] == np.median(nb[:, 1]) and out[y, x, 2] == np.median(nb[:, 2])
+            lum = np.sort(np.concatenate([nb[:, 0], [0.0, 0.0]]))      # maxima start at 0
+            lmin = np.sort(np.concatenate([nb[:, 0], [65535.0, 65535.0]]))
+            max1, max2, min1, min2 = lum[-1], lum[-2], lmin[0], lmin[1]
+            c = img[y, x, 0]
+            sig = np.sqrt(np.float32(va[0]) + np.float32(vb[0]) * c)
+
+            def ss(e0, e1, v):
+                t = np.clip((v - e0) / (e1 - e0), 0, 1)
+                return t * t * (3 - 2 * t)
+            lo = min2 + (min1 - min2) * ss(sig, 4 * sig, min2 - min1)
+            hi = max2 + (max1 - max2) * ss(sig, 4 * sig, max1 - max2)
+            assert abs(out[y, x, 0] - min(max(c, lo), hi)) < 1e-6
+    assert 0.5 < out[5, 7, 0] < 0.52  # the speckle is pulled most of the way in
+    big = img.copy()
+    big[5, 7, 0] = 0.95  # far above 4 sigma: a feature, kept
+    assert oracle.despeckle(big, va, vb)[5, 7, 0] == np.float32(0.95)
+
+
+def test_front_end_entry_points_validate_arguments_without_a_device():
+    L = gcam.lib()
+    f = ctypes.c_void_p()
+    assert L.gcam_front_end_create(7, 8, ctypes.byref(f)) == 1   # odd width
+    assert L.gcam_front_end_create(8, 8, None) == 1
+    assert L.gcam_front_end_run(None, None, None, None, None, None) == 1
+    sm = (ctypes.c_float * 4)(1, 1, 1, 1)
+    assert L.gcam_scale_raw_data(None, None, 2, sm, 0.0, 0.0, None) == 1
+    assert L.gcam_raw_image_sobel(None, None, None) == 1
+    assert L.gcam_transform_image(None, None, None, None) == 1
+    assert L.gcam_gaussian_kernel_bilinear_weights(-1.0, None, 0) == -1
+    bad = (ctypes.c_float * 9)(*([0.0] * 9))
+    out = (ctypes.c_float * 9)()
+    assert L.gcam_matrix3_inverse(bad, out) == 1
+
+
+def test_despeckle_with_a_negative_variance_clamps_to_the_second_extremes():
+    """var_a + var_b * Y < 0 makes sigma a NaN; min, max and clamp follow fmin /
+    fmax (Metal's definition of clamp), so both blends take the second extremes."""
+    img = np.zeros((3, 3, 4), np.float32)
+    img[..., 0] = [[-0.30, -0.10, -0.20], [-0.25, -0.05, -0.15], [-0.12, -0.22, -0.28]]
+    out = oracle.despeckle(img, (1e-4,) * 3, (1e-2,) * 3)
+    lum = np.sort(np.concatenate([img[..., 0].ravel(), [0.0, 0.0]]))
+    lmin = np.sort(img[..., 0].ravel())
+    assert out[1, 1, 0] == np.float32(min(max(-0.05, lmin[1]), lum[-2]))

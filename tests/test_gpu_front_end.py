@@ -1,0 +1,173 @@
+"""Raw front end on the GPU against the FP32 oracle.
+
+Every stage except the demosaic's direction weights evaluates the same FP32
+operations in the same order as the oracle (no contraction, correctly rounded
+division and square root), so scale, Sobel, both blur paths (with the Sobel
+tile fused or read from an image), the highlight blend, the transform and the
+despeckle are compared bit for bit.  The demosaic takes the gradient direction
+from atan2f, whose device and host implementations differ in the last bits;
+the direction enters continuously (through mix weights), so the demosaic is
+held to 1e-5 absolute on values in [0.1, 1].
+"""
+import numpy as np
+import pytest
+import torch
+
+import glasscamerapublic_amd as gcam
+import oracle
+from tests.helpers import nlf_for
+
+pytestmark = pytest.mark.gpu
+
+DEMOSAIC_ATOL = 1e-5
+SCALE_12BIT = 65535.0 / 4095.0
+
+
+def raw_frame(w, h, seed, a=4e-3, b=2e-3):
+    """A 12-bit RGGB synthetic frame (device int16 storage) and its uint16 host copy."""
+    _, raw = gcam.synth_frame(w, h, a, b, scene_seed=seed, noise_seed=seed + 1, want_raw=True)
+    return raw, raw.cpu().numpy().view(np.uint16)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("bayer,lens", [(0, 0.0), (1, 0.4), (2, 0.0), (3, 0.25)])
+def test_scale_raw_is_bit_exact(bayer, lens):
+    rng = np.random.default_rng(bayer)
+    h, w = 130, 250
+    raw = rng.integers(0, 65536, (h, w)).astype(np.uint16)
+    sm = np.array([2.0, 1.0, 1.6, 1.02], np.float32) * SCALE_12BIT
+    got = gcam.scaleRawData(dev(raw.view(np.int16)), bayer, sm, 0.003, lens).cpu().numpy()
+    want = oracle.scale_raw(raw, bayer, sm, 0.003, lens)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("w,h", [(256, 192), (250, 130), (66, 18)])
+def test_sobel_and_gradient_blur_are_bit_exact(w, h):
+    raw_d, raw = raw_frame(w, h, 11)
+    scaled = oracle.scale_raw(raw, 2, [SCALE_12BIT] * 4)
+    sob = oracle.raw_sobel(scaled)
+    got_sob = gcam.rawImageSobel(dev(scaled)).cpu().numpy()
+    np.testing.assert_array_equal(got_sob, sob)
+    rv = oracle.raw_variance((2e-6,) * 4, (4e-4,) * 4)
+    want = oracle.blur_sobel(scaled, sob, rv[1])
+    given = gcam.gaussianBlurSobelImage(dev(scaled), dev(sob), rv[1]).cpu().numpy()
+    fused = gcam.gaussianBlurSobelImage(dev(scaled), None, rv[1]).cpu().numpy()
+    np.testing.assert_array_equal(given, want)
+    np.testing.assert_array_equal(fused, want)
+    narrow = oracle.blur_sobel(scaled, sob, (0.0, 0.0))
+    wide_frac = float(np.mean(np.any(want != narrow, axis=-1)))
+    print(f"{w}x{h}: wide-kernel pixels {wide_frac:.3f}")
+    assert 0.05 < wide_frac < 0.995  # both branches of the blur are exercised
+
+
+@pytest.mark.parametrize("bayer", [0, 1, 2, 3])
+def test_demosaic_matches_the_oracle(bayer):
+    h, w = 192, 256
+    _, raw = raw_frame(w, h, 21 + bayer)
+    scaled = oracle.scale_raw(raw, bayer, [SCALE_12BIT * g for g in (1.9, 1.0, 1.5, 1.0)])
+    rv = oracle.raw_variance((2e-6,) * 4, (4e-4,) * 4)
+    grad = oracle.blur_sobel(scaled, oracle.raw_sobel(scaled), rv[1])
+    g_o, t_o, rgb_o = oracle.demosaic(scaled, grad, bayer, rv)
+    g_d, t_d, rgb_d = (x.cpu().numpy() for x in gcam.demosaicImage(dev(scaled), dev(grad), bayer, rv))
+    for name, got, want in (("green", g_d, g_o), ("red/blue", t_d, t_o), ("rgb", rgb_d, rgb_o)):
+        err = float(np.abs(got - want).max())
+        print(f"bayer {bayer} {name}: max abs {err:.3g}")
+        assert err <= DEMOSAIC_ATOL, name
+
+
+def test_highlights_transform_despeckle_are_bit_exact():
+    rng = np.random.default_rng(31)
+    h, w = 96, 130
+    img = np.zeros((h, w, 4), np.float32)
+    img[..., :3] = rng.uniform(0.05, 1.8, (h, w, 3))
+    got = gcam.blendHighlightsImage(dev(img), 1.0).cpu().numpy()
+    np.testing.assert_array_equal(got, oracle.blend_highlights(img, 1.0))
+    m = oracle.cam_ycbcr(np.array([[1.6, -0.4, -0.2], [-0.2, 1.4, -0.2], [0.0, -0.5, 1.5]]))
+    ycc = oracle.transform(img, m)
+    np.testing.assert_array_equal(gcam.transformImage(dev(img), m).cpu().numpy(), ycc)
+    va, vb = (2e-5, 1e-5, 1e-5), (1.5e-3, 5e-4, 5e-4)
+    np.testing.assert_array_equal(gcam.despeckleImage(dev(ycc), va, vb).cpu().numpy(),
+                                  oracle.despeckle(ycc, va, vb))
+
+
+def params_for(bayer=2, despeckle=True):
+    rgb_cam = [[1.7, -0.5, -0.2], [-0.25, 1.45, -0.2], [0.05, -0.6, 1.55]]
+    nl = nlf_for(1)[0]
+    return gcam.FrontEndParameters(
+        bayerPattern=bayer, black_level=0.001, lensShadingCorrection=0.2,
+        scale_mul=[SCALE_12BIT * g for g in (1.8, 1.0, 1.4, 1.0)],
+        rawNlf=((2e-6, 1e-6, 3e-6, 1e-6), (5e-4, 3e-4, 6e-4, 3e-4)), rgb_cam=rgb_cam, highlightClip=1.0,
+        despeckleNlf=(tuple(nl[0]), tuple(nl[1])) if despeckle else None)
+
+
+def oracle_front(raw, p):
+    return oracle.raw_front_end(raw, p.bayerPattern, p.scale_mul, p.black_level, p.lensShadingCorrection,
+                                p.rawNlf[0], p.rawNlf[1], np.array(p.rgb_cam), p.highlightClip, p.despeckleNlf)
+
+
+@pytest.mark.parametrize("w,h,despeckle", [(512, 384, True), (258, 130, False)])
+def test_front_end_matches_the_oracle(w, h, despeckle):
+    raw_d, raw = raw_frame(w, h, 41)
+    p = params_for(despeckle=despeckle)
+    fe = gcam.RawFrontEnd(w, h)
+    ycc, grad = fe.run(raw_d, p)
+    torch.cuda.synchronize()
+    ycc_o, grad_o, _ = oracle_front(raw, p)
+    np.testing.assert_array_equal(grad.cpu().numpy(), grad_o)
+    err = float(np.abs(ycc.cpu().numpy() - ycc_o).max())
+    print(f"{w}x{h} despeckle={despeckle}: YCbCr max abs {err:.3g}")
+    assert err <= 2 * DEMOSAIC_ATOL  # the transform's rows sum to at most ~1.5 in magnitude
+    # the per-operation entry points compose to the same result
+    scaled = gcam.scaleRawData(raw_d, p.bayerPattern, p.scale_mul, p.black_level, p.lensShadingCorrection)
+    rv = gcam.getRawVariance(p.rawNlf)
+    g2 = gcam.gaussianBlurSobelImage(scaled, gcam.rawImageSobel(scaled), rv[1])
+    assert torch.equal(g2, grad)
+    _, _, rgb = gcam.demosaicImage(scaled, g2, p.bayerPattern, rv)
+    y2 = gcam.transformImage(gcam.blendHighlightsImage(rgb, p.highlightClip), gcam.cam_ycbcr(p.rgb_cam))
+    if despeckle:
+        y2 = gcam.despeckleImage(y2, *p.despeckleNlf)
+    assert torch.equal(y2, ycc)
+    fe.close()
+
+
+def test_raw_converter_chain(oracle_threads):
+    """Raw frame -> front end -> FP32 pyramid denoise (with the front end's
+    gradient) -> camera RGB, against the same chain on the oracle.  The
+    pyramid's own bar is 1e-4; the return to RGB multiplies by up to ~2."""
+    w, h, levels, radius = 384, 256, 4, 3
+    raw_d, raw = raw_frame(w, h, 51)
+    p = params_for()
+    nl = [(tuple(a), tuple(b)) for a, b in nlf_for(levels)]
+    prm = [gcam.DenoiseParameters(luma=1.5, chroma=1.2, chromaBoost=2.0, gradientBoost=2.0, gradientThreshold=1.5,
+                                  sharpening=1.2) for _ in range(levels)]
+    conv = gcam.RawConverter(w, h, levels=levels, search_radius=radius, precision="fp32")
+    rgb = conv.demosaic(raw_d, p, prm, list(nl)).cpu().numpy()
+    ycc_o, grad_o, _ = oracle_front(raw, p)
+    pr = [dict(luma=q.luma, chroma=q.chroma, chromaBoost=q.chromaBoost, gradientBoost=q.gradientBoost,
+               gradientThreshold=q.gradientThreshold, sharpening=q.sharpening) for q in prm]
+    den = oracle.pyramid_denoise(ycc_o, pr, np.array([[a, b] for a, b in nl]), levels=levels, radius=radius,
+                                 gradient=grad_o, lens_shading=p.lensShadingCorrection).denoised[0]
+    back = np.linalg.inv(oracle.cam_ycbcr(np.array(p.rgb_cam)).astype(np.float64)).astype(np.float32)
+    want = oracle.transform(den, back)
+    err = float(np.abs(rgb[..., :3] - want[..., :3]).max())
+    print(f"camera RGB max abs {err:.3g}")
+    assert err <= 5e-4
+    conv.close()
+
+
+def test_front_end_argument_errors():
+    with pytest.raises(gcam.GcamError):
+        gcam.RawFrontEnd(251, 130)
+    fe = gcam.RawFrontEnd(64, 32)
+    with pytest.raises(ValueError):
+        fe.run(torch.zeros((32, 66), dtype=torch.int16, device="cuda"), params_for())
+    bad = params_for()
+    bad.bayerPattern = 7
+    with pytest.raises(ValueError):
+        fe.run(torch.zeros((32, 64), dtype=torch.int16, device="cuda"), bad)
+    with pytest.raises(gcam.GcamError):  # a blur radius beyond the tile apron
+        gcam.gaussianBlurSobelImage(torch.zeros((32, 64), device="cuda"), None, (1e-5, 1e-3), radius2=8.0)
+    fe.close()
