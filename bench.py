@@ -15,6 +15,10 @@ cores on the same frame.
 
     python bench.py [--steps 20] [--warmup 3] [--precision fp16] [--no-cpu-baseline]
     python bench.py --workload burst32      # the 32-frame mixed-size burst (batched form)
+
+The raw front end that produces the denoise's inputs from the frame's Bayer
+mosaic (RawFrontEnd) is timed beside it under "front_end" (--no-front-end to
+skip).
 """
 from __future__ import annotations
 
@@ -112,6 +116,61 @@ def time_single(gcam, torch, w, h, levels, radius, precision, bm_rows, ycc, para
             "wall_ms_per_step": wall * 1e3 / steps, "stages": stages, "out": out}
 
 
+# HBM bytes per pixel of the front end's stage DAG, each stage's inputs read
+# once and outputs written once (FP32 images, 16-bit raw): scale 2 + 4,
+# gradient (Sobel fused) 4 + 8, green 4 + 8 + 4, red/blue 4 + 4 + 8 + 16,
+# red/blue-at-green + highlights + YCbCr 16 + 8 + 16, despeckle 16 + 16.
+FRONT_END_BYTES_PER_PX = 6 + 12 + 16 + 32 + 40 + 32
+
+
+def front_end_params(gcam, a, b, despeckle_nlf):
+    """The synthetic frame's raw model: 12-bit DN in 16-bit words, sRGB primaries
+    (rgb_cam = I), white balance 1; raw noise var = a x + b^2 in [0, 1] units,
+    mapped through scaleRawData's 0.9 x + 0.1."""
+    return gcam.FrontEndParameters(
+        bayerPattern=2, black_level=0.0, lensShadingCorrection=0.0, scale_mul=[65535.0 / 4095.0] * 4,
+        rawNlf=((0.81 * b * b - 0.09 * a,) * 4, (0.9 * a,) * 4), rgb_cam=[[1, 0, 0], [0, 1, 0], [0, 0, 1]],
+        highlightClip=1.0, despeckleNlf=despeckle_nlf)
+
+
+def time_front_end(gcam, torch, raw, w, h, params, steps, warmup, cpu_baseline, threads):
+    """RawFrontEnd.run (raw -> YCbCr + gradient for the denoise) timed like the denoise."""
+    stream = torch.cuda.current_stream()
+    fe = gcam.RawFrontEnd(w, h)
+    ycc = torch.empty((h, w, 4), dtype=torch.float32, device=raw.device)
+    grad = torch.empty((h, w, 2), dtype=torch.float32, device=raw.device)
+    for _ in range(warmup):
+        fe.run(raw, params, ycc, grad)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        fe.run(raw, params, ycc, grad)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    fe.close()
+    gbs = FRONT_END_BYTES_PER_PX * w * h / (ms / 1e3) / 1e9
+    out = {"ms_per_frame": round(ms, 4), "value": round(w * h / 1e6 / (ms / 1e3), 2), "unit": "Mpix/s",
+           "roofline": {"bound": "hbm", "bytes_per_px": FRONT_END_BYTES_PER_PX, "achieved": round(gbs, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}}
+    if cpu_baseline:
+        import numpy as np
+
+        import oracle
+        cw, ch = min(w, 2048), min(h, 1536)
+        crop = np.ascontiguousarray(raw[:ch, :cw].cpu().numpy().view(np.uint16))
+        p = params
+        t = time.perf_counter()
+        oracle.raw_front_end(crop, p.bayerPattern, p.scale_mul, p.black_level, p.lensShadingCorrection,
+                             p.rawNlf[0], p.rawNlf[1], np.array(p.rgb_cam, np.float32), p.highlightClip,
+                             p.despeckleNlf)
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(cw * ch / 1e6 / cpu_s, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+                               "sample": f"{cw}x{ch} crop of the frame, FP32 C++ oracle, 1 thread, {cpu_s:.2f} s"}
+    return out
+
+
 def single_frame(args) -> dict:
     import numpy as np
     import torch
@@ -123,7 +182,8 @@ def single_frame(args) -> dict:
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
 
-    ycc, clean = gcam.synth_frame(w, h, a, b, scene_seed=1234, noise_seed=5678, device=dev, want_clean=True)
+    ycc, raw, clean = gcam.synth_frame(w, h, a, b, scene_seed=1234, noise_seed=5678, device=dev, want_raw=True,
+                                       want_clean=True)
     params = [gcam.DenoiseParameters() for _ in range(levels)]
     nlf = [((1e-8,) * 3, (1e-8,) * 3) for _ in range(levels)]
 
@@ -252,6 +312,12 @@ def single_frame(args) -> dict:
                             "stage_ms_last_step": {k: [round(x, 4) for x in v] for k, v in ref32["stages"].items()}}
     if quality is not None:
         out["quality"] = quality
+    if not args.no_front_end:
+        # The inputs of this denoise from the frame's raw mosaic (front end of RawConverter::demosaic and
+        # its despeckle), reported beside the denoise; the despeckle uses the calibrated level-0 model.
+        fe_params = front_end_params(gcam, a, b, nlf[0])
+        out["front_end"] = time_front_end(gcam, torch, raw, w, h, fe_params, args.steps, args.warmup,
+                                          not args.no_cpu_baseline, 1)
     return out
 
 
@@ -357,6 +423,7 @@ def main() -> int:
     ap.add_argument("--burst-frames", type=int, default=32)
     ap.add_argument("--no-compare", action="store_true", help="skip timing the FP32 form beside the reported one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-front-end", action="store_true", help="skip timing the raw front end beside the denoise")
     ap.add_argument("--no-quality", action="store_true", help="skip the PSNR check against the FP64 oracle")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: min(cores, 16))")
     args = ap.parse_args()

@@ -15,10 +15,11 @@
 //   cam_ycbcr                     MetalPipeline/demosaic_utils.cpp:703-720
 //   orchestration                 MetalPipeline/raw_converter.cpp:148-214, 70-78
 //
-// Every kernel is a stencil over FP32 images and HBM-bound; the Sobel filter
-// is fused into the gradient blur (the Sobel tile of a workgroup lives in LDS
-// and never reaches HBM), and the last demosaic pass, the highlight blend and
-// the YCbCr transform run as one pass.  Arithmetic is FP32 evaluated in source
+// The kernels are stencils over FP32 images, HBM-bound except the gradient
+// blur (up to 29 bilinear taps of a 4-channel image per pixel: vector-ALU and
+// LDS-bound).  The Sobel filter is fused into that blur (the Sobel tile of a
+// workgroup lives in LDS and never reaches HBM), and the last demosaic pass,
+// the highlight blend and the YCbCr transform run as one pass.  Arithmetic is FP32 evaluated in source
 // order (no contraction): out-of-bounds reads clamp to the edge, the linear
 // sampler interpolates at exact sub-texel positions, so every stage except the
 // gradient direction (atan2f) reproduces a plain FP32 evaluation of the
@@ -148,29 +149,41 @@ struct BlurTaps {
 constexpr int kBTW = 64, kBTH = 16, kBApL = 5, kBApR = 6;
 constexpr int kBAW = kBTW + kBApL + kBApR, kBAH = kBTH + kBApL + kBApR;
 
-// Weighted mean of the bilinear samples of the LDS Sobel tile at the taps
-// (demosaic.metal:170-186): s = (T00 (1 - fx) + T10 fx) (1 - fy) +
-// (T01 (1 - fx) + T11 fx) fy per channel, summed in tap order.
-__device__ __forceinline__ float4 blur_taps(const float4 (*sb)[kBAW], int lx, int ly, const Tap* taps, int n) {
-    float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float norm = 0.0f;
+// Rows of output a thread blurs: consecutive, so that the horizontal
+// interpolation of a Sobel row under one tap is computed once and serves as
+// the lower row of one output and the upper row of the next.
+constexpr int kBRows = 4;
+
+// Weighted sums over the taps of the bilinear samples of the LDS Sobel tile
+// (demosaic.metal:170-186) for kBRows consecutive outputs starting at tile
+// row ly: s = (T00 (1 - fx) + T10 fx) (1 - fy) + (T01 (1 - fx) + T11 fx) fy
+// per channel, summed in tap order; the caller divides by `norm`.
+__device__ __forceinline__ void blur_taps(const float4 (*sb)[kBAW], int lx, int ly, const Tap* taps, int n,
+                                          float4 (&sum)[kBRows], float& norm) {
+#pragma unroll
+    for (int k = 0; k < kBRows; k++) sum[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    norm = 0.0f;
     for (int i = 0; i < n; i++) {
         const Tap t = taps[i];
-        const int ax = lx + t.ix, ay = ly + t.iy;
-        const float4 a = sb[ay][ax], b = sb[ay][ax + 1], c = sb[ay + 1][ax], d = sb[ay + 1][ax + 1];
         const float gx = 1.0f - t.fx, gy = 1.0f - t.fy;
-        float4 s;
-        s.x = (a.x * gx + b.x * t.fx) * gy + (c.x * gx + d.x * t.fx) * t.fy;
-        s.y = (a.y * gx + b.y * t.fx) * gy + (c.y * gx + d.y * t.fx) * t.fy;
-        s.z = (a.z * gx + b.z * t.fx) * gy + (c.z * gx + d.z * t.fx) * t.fy;
-        s.w = (a.w * gx + b.w * t.fx) * gy + (c.w * gx + d.w * t.fx) * t.fy;
-        sum.x = sum.x + t.w * s.x;
-        sum.y = sum.y + t.w * s.y;
-        sum.z = sum.z + t.w * s.z;
-        sum.w = sum.w + t.w * s.w;
+        const int ax = lx + t.ix, ay = ly + t.iy;
+        auto hrow = [&](int r) {
+            const float4 a = sb[r][ax], b = sb[r][ax + 1];
+            return make_float4(a.x * gx + b.x * t.fx, a.y * gx + b.y * t.fx, a.z * gx + b.z * t.fx,
+                               a.w * gx + b.w * t.fx);
+        };
+        float4 h0 = hrow(ay);
+#pragma unroll
+        for (int k = 0; k < kBRows; k++) {
+            const float4 h1 = hrow(ay + k + 1);
+            sum[k].x = sum[k].x + t.w * (h0.x * gy + h1.x * t.fy);
+            sum[k].y = sum[k].y + t.w * (h0.y * gy + h1.y * t.fy);
+            sum[k].z = sum[k].z + t.w * (h0.z * gy + h1.z * t.fy);
+            sum[k].w = sum[k].w + t.w * (h0.w * gy + h1.w * t.fy);
+            h0 = h1;
+        }
         norm = norm + t.w;
     }
-    return make_float4(sum.x / norm, sum.y / norm, sum.z / norm, sum.w / norm);
 }
 
 // kFromRaw: the Sobel tile is computed here from the scaled raw (the fused
@@ -212,13 +225,25 @@ __global__ __launch_bounds__(256) void k_blur_sobel(View<const float> raw, View<
     __syncthreads();
     const int x = x0 + threadIdx.x;
     if (x >= W) return;
-    for (int r = threadIdx.y; r < kBTH; r += blockDim.y) {
-        const int y = y0 + r;
-        if (y >= H) break;
-        const int lx = threadIdx.x + kBApL, ly = r + kBApL;
-        float4 res = blur_taps(sb, lx, ly, taps.t1, taps.n1);
+    const int r0 = threadIdx.y * kBRows, lx = threadIdx.x + kBApL;
+    float4 nar[kBRows], wid[kBRows];
+    float nn, nw;
+    blur_taps(sb, lx, r0 + kBApL, taps.t1, taps.n1, nar, nn);
+    bool wide[kBRows], any = false;
+#pragma unroll
+    for (int k = 0; k < kBRows; k++) {
+        const int y = min(y0 + r0 + k, H - 1);
+        nar[k] = make_float4(nar[k].x / nn, nar[k].y / nn, nar[k].z / nn, nar[k].w / nn);
         const float sigma = sqrtf(var_a + var_b * raw.at(x, y));
-        if (sqrtf(res.x * res.x + res.y * res.y) < 4.0f * sigma) res = blur_taps(sb, lx, ly, taps.t2, taps.n2);
+        wide[k] = sqrtf(nar[k].x * nar[k].x + nar[k].y * nar[k].y) < 4.0f * sigma;
+        any = any || wide[k];
+    }
+    if (any) blur_taps(sb, lx, r0 + kBApL, taps.t2, taps.n2, wid, nw);
+#pragma unroll
+    for (int k = 0; k < kBRows; k++) {
+        const int y = y0 + r0 + k;
+        if (y >= H) break;
+        const float4 res = wide[k] ? make_float4(wid[k].x / nw, wid[k].y / nw, wid[k].z / nw, wid[k].w / nw) : nar[k];
         out.at(x, y) = make_float2(copysignf(res.z, res.x), copysignf(res.w, res.y));
     }
 }
@@ -626,7 +651,7 @@ static gcam_status blur_sobel(const gcam_image* scaled, const gcam_image* sobel,
     if (!make_taps(radius1, taps.t1, kMaxTaps1, &taps.n1) || !make_taps(radius2, taps.t2, kMaxTaps2, &taps.n2))
         return bad("gaussianBlurSobelImage: radius1 must be in (0, 1.5] and radius2 in (0, 5.5]");
     const dim3 g((gradient->width + kBTW - 1) / kBTW, (gradient->height + kBTH - 1) / kBTH);
-    const dim3 b(kBTW, 4);
+    const dim3 b(kBTW, kBTH / kBRows);
     if (sobel) {
         k_blur_sobel<false><<<g, b, 0, s>>>(view<const float>(*scaled), view<const float4>(*sobel), taps,
                                             raw_variance[0], raw_variance[1], view<float2>(*gradient));
