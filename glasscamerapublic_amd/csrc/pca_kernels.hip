@@ -185,52 +185,58 @@ __device__ __forceinline__ double rsq_refined(double x) {
     return y * fma(x * y, -0.5 * y, 1.5);
 }
 
-// Cyclic Jacobi sweeps on the symmetric 25 x 25 matrix in Ac (scalar type T),
-// accumulating the rotations into V, until the off-diagonal energy is at most
-// tol of the total or max_sweeps have run.  Every thread of the 256-thread
-// workgroup calls it; on return Ac holds the rotated matrix.
+// Cyclic Jacobi sweeps on the symmetric 25 x 25 matrix C, in the parallel
+// (circle) ordering: round r of a sweep pairs the indices into super-indices
+// u = 0..11, {(r + u + 1) mod 25, (r - u - 1) mod 25}, and u = 12 = {r}, and
+// applies the 12 rotations at once, A <- J^T A J and V <- V J.
 //
-// A round applies its 12 rotations at once, A <- J^T A J and V <- V J.
-// Round r pairs the indices into super-indices u = 0..11, (p_u, q_u) = the
-// sorted ((r + u + 1) mod 25, (r - u - 1) mod 25), and u = 12 = {r}
-// (identity).  J acts on the 2 x 2 blocks A[{p_u, q_u}][{p_v, q_v}]:
-// Y = J_u^T X J_v with J = [[c, s], [-s, c]] (rows/columns p, q), and the pair
-// entry of a diagonal block is set to zero.  Every entry belongs to exactly one
-// block: threads 0..90 (wavefronts 0 and 1) own one block each (both
-// triangles), reading A from one buffer and writing the other, so a round needs
-// one workgroup barrier.  Threads 128..244 (wavefronts 2 and 3) own the rows
-// {j0, j0 + 9, j0 + 18} of V at one super-index v each and update them in
-// place.  Every thread computes the rotations it needs from the round's A
-// (bit-identical everywhere).  Rotation: the Golub & Van Loan angle
-// (|phi| <= pi/4, tan(2 phi) = 2 a_pq / (a_qq - a_pp)) from its half-angle
+// Slot frame: in round r, slot s stands for index (s + r) mod 25, so the pairs
+// are the same slots every round: super-index u < 12 is slots f = u + 1 and
+// g = 24 - u, and u = 12 is slot 0.  A is stored as the 91 2 x 2 blocks
+// A[{f_u, g_u}][{f_v, g_v}], u <= v (lo = row f, hi = row g, 16 bytes each; a
+// diagonal block keeps both copies of its off-diagonal entry, absent rows and
+// columns of u = 12 stay zero), and V as the row pairs (V[j][f_u], V[j][g_u]).
+// Moving to the next round shifts every slot down by one, so each rotated
+// entry goes to a destination that is the same in every round: the addresses
+// are computed once.  Threads 0..90 (wavefronts 0 and 1) own one block each
+// and compute the two rotations they need from the diagonal blocks; threads
+// 128..244 (wavefronts 2 and 3) own rows {j0, j0 + 9, j0 + 18} of V at one
+// super-index.  Every read is of the current buffer and every write of the
+// other one, so a round needs one barrier.  After a sweep (25 rounds) the frame
+// is the identity again.  Rotation: the Golub & Van Loan angle
+// (|phi| <= pi/4, tan(2 phi) = 2 a_fg / (a_gg - a_ff)) from its half-angle
 // form, c^2 = (1 + |d| / h) / 2, s = sign(d e) |e| / (2 h c), h = hypot(d, e),
-// d = a_qq - a_pp, e = 2 a_pq: one reciprocal square root for 1/h, one for 1/c.
+// d = a_gg - a_ff, e = 2 a_fg: one reciprocal square root for 1/h, one for
+// 1/c.  The rotation does not depend on which of a pair is called f.
+//
+// On entry C is the matrix (row pitch kLD); on return its diagonal holds the
+// eigenvalues and V (row pitch kLD) the eigenvectors as columns.  Stops when
+// the off-diagonal energy is at most tol of the total, or after max_sweeps.
 // Row pitch of the 25 x 25 matrices in LDS: odd, so the rows of a column fall
-// in different banks, and p * 33 = (p << 5) + p.
+// in different banks.
 constexpr int kLD = 33;
 
-template <class T>
-__device__ void jacobi_sweeps(T*& Ac, T*& An, T* V, double tol, int max_sweeps, double2* energy) {
+namespace slot {
+constexpr int NS = 13, NB = NS * (NS + 1) / 2;  // super-indices, blocks
+__device__ __forceinline__ int of(int u, int i) { return u == NS - 1 ? (i == 0 ? 0 : -1) : (i == 0 ? u + 1 : 24 - u); }
+__device__ __forceinline__ void loc(int s, int& u, int& i) {
+    u = s == 0 ? NS - 1 : (s <= 12 ? s - 1 : 24 - s);
+    i = s <= 12 ? 0 : 1;
+}
+__device__ __forceinline__ int block(int u, int v) { return u * NS - u * (u - 1) / 2 + (v - u); }  // u <= v
+// Element (i, j) of block b in a buffer of 4 NB doubles: lo rows first.
+__device__ __forceinline__ int elem(int b, int i, int j) { return (i == 0 ? 0 : 2 * NB) + 2 * b + j; }
+}  // namespace slot
+
+__device__ void jacobi_sweeps(double* C, double* V, double2 (*Ab)[2 * slot::NB], double2 (*Vb)[kPatch * slot::NS],
+                              double tol, int max_sweeps, double2* energy) {
+    using namespace slot;
     constexpr int N = kPatch, LD = kLD;
-    constexpr int NU = N * (N + 1) / 2;  // upper triangle, a <= b
-    constexpr int IU = (NU + 255) / 256;
-    constexpr int NS = 13, NB = NS * (NS + 1) / 2;  // super-indices, blocks
-    constexpr int V0T = 128, NVT = NS * 9;           // V threads: wavefronts 2 and 3
-    const T one = T(1), half = T(0.5), guard = sizeof(T) == 8 ? T(1e-300) : T(1e-30f);
+    constexpr int V0T = 128, NVT = NS * 9;  // V threads: wavefronts 2 and 3
+    const double one = 1.0, half = 0.5, guard = 1e-300;
     const int tid = threadIdx.x;
-    int ua[IU], ub[IU];
-#pragma unroll
-    for (int m = 0; m < IU; m++) {
-        int t = min(tid + 256 * m, NU - 1), a = 0;
-        while (t >= N - a) {
-            t -= N - a;
-            a++;
-        }
-        ua[m] = a;
-        ub[m] = a + t;
-    }
     const bool is_block = tid < NB, is_v = tid >= V0T && tid < V0T + NVT;
-    int bu = NS - 1, bv = NS - 1;  // block of a block thread; bu = the super-index of a V thread
+    int bu = NS - 1, bv = NS - 1;
     if (is_block) {
         int t = tid;
         bu = 0;
@@ -243,47 +249,96 @@ __device__ void jacobi_sweeps(T*& Ac, T*& An, T* V, double tol, int max_sweeps, 
         bu = (tid - V0T) % NS;
     }
     const int vj0 = is_v ? (tid - V0T) / NS : 0;  // rows vj0, vj0 + 9, vj0 + 18
-    // The pair of super-index u in round r is the sorted ((r + u + 1) mod 25,
-    // (r - u - 1) mod 25) (u = 12: (r, r)); both indices advance by one per
-    // round, so they are kept and stepped instead of recomputed.
-    auto first = [&](int u, int& i, int& j) {
-        i = u == NS - 1 ? 0 : u + 1;
-        j = u == NS - 1 ? 0 : N - 1 - u;
-    };
-    auto step = [&](int& i) { i = i == N - 1 ? 0 : i + 1; };
+    const int du = block(bu, bu), dv = block(bv, bv);
+
+    // Destinations in the next round of this thread's entries (-1: none) and,
+    // for an entry landing off the diagonal of a diagonal block, of its mirror.
+    int dst[4], mir[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int i = k >> 1, j = k & 1;
+        const int s = of(bu, i), t = of(bv, j);
+        dst[k] = mir[k] = -1;
+        if (!is_block || s < 0 || t < 0 || (bu == bv && i == 1 && j == 0)) continue;
+        int u1, i1, v1, j1;
+        loc((s + 24) % 25, u1, i1);
+        loc((t + 24) % 25, v1, j1);
+        if (u1 > v1) {
+            const int tu = u1, ti = i1;
+            u1 = v1;
+            i1 = j1;
+            v1 = tu;
+            j1 = ti;
+        }
+        const int b2 = block(u1, v1);
+        dst[k] = elem(b2, i1, j1);
+        if (u1 == v1 && i1 != j1) mir[k] = elem(b2, j1, i1);
+    }
+    int vd[3][2];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const int j = vj0 + 9 * m;
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int s = of(bu, i);
+            vd[m][i] = -1;
+            if (!is_v || j >= N || s < 0) continue;
+            int u1, i1;
+            loc((s + 24) % 25, u1, i1);
+            vd[m][i] = (j * NS + u1) * 2 + i1;
+        }
+    }
+
+    // Load C and V = I into slot blocks (frame of round 0: slot = index).
+    for (int e = tid; e < 2 * 2 * NB; e += 256) reinterpret_cast<double2*>(Ab)[e] = make_double2(0.0, 0.0);
+    for (int e = tid; e < 2 * N * NS; e += 256) reinterpret_cast<double2*>(Vb)[e] = make_double2(0.0, 0.0);
+    __syncthreads();
+    if (is_block) {
+        double* A = reinterpret_cast<double*>(Ab[0]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int s = of(bu, k >> 1), t = of(bv, k & 1);
+            if (s >= 0 && t >= 0) A[elem(tid, k >> 1, k & 1)] = C[s * LD + t];
+        }
+    }
+    for (int e = tid; e < N * NS; e += 256) {
+        const int j = e / NS, u = e % NS;
+        Vb[0][e] = make_double2(of(u, 0) == j ? 1.0 : 0.0, of(u, 1) == j ? 1.0 : 0.0);
+    }
+    __syncthreads();
 
     // Straight-line code (no select on the expensive operands, which the
-    // compiler would turn into branches).  The identity (p == q, or a_pq == 0)
+    // compiler would turn into branches).  The identity (u = 12, or a_fg == 0)
     // is blended in arithmetically: c + m (1 - c) = 1 and s (1 - m) = 0 exactly
     // for m = 1 (1 - c is exact for c in [1/2, 1]).  rotation2 advances two
     // independent rotations step by step, so their dependent chains overlap.
-    auto rotation2 = [&](bool id0, T app0, T aqq0, T apq0, bool id1, T app1, T aqq1, T apq1, T& c0, T& s0, T& c1,
-                         T& s1) {
-        const T e0 = T(2) * apq0, d0 = aqq0 - app0, e1 = T(2) * apq1, d1 = aqq1 - app1;
-        const T m0 = (id0 || e0 == T(0)) ? one : T(0), m1 = (id1 || e1 == T(0)) ? one : T(0);
-        const T y0 = rsq_refined(fma(d0, d0, e0 * e0) + guard), y1 = rsq_refined(fma(d1, d1, e1 * e1) + guard);
-        const T c20 = half * fma(fabs(d0), y0, one), c21 = half * fma(fabs(d1), y1, one);  // cos^2 in [1/2, 1]
-        const T z0 = rsq_refined(c20), z1 = rsq_refined(c21);                              // 1 / c
-        const T cc0 = c20 * z0, cc1 = c21 * z1;
+    auto rotation2 = [&](bool id0, double aff0, double agg0, double afg0, bool id1, double aff1, double agg1,
+                         double afg1, double& c0, double& s0, double& c1, double& s1) {
+        const double e0 = 2.0 * afg0, d0 = agg0 - aff0, e1 = 2.0 * afg1, d1 = agg1 - aff1;
+        const double m0 = (id0 || e0 == 0.0) ? one : 0.0, m1 = (id1 || e1 == 0.0) ? one : 0.0;
+        const double y0 = rsq_refined(fma(d0, d0, e0 * e0) + guard), y1 = rsq_refined(fma(d1, d1, e1 * e1) + guard);
+        const double c20 = half * fma(fabs(d0), y0, one), c21 = half * fma(fabs(d1), y1, one);  // cos^2
+        const double z0 = rsq_refined(c20), z1 = rsq_refined(c21);                              // 1 / c
+        const double cc0 = c20 * z0, cc1 = c21 * z1;
         c0 = fma(m0, one - cc0, cc0);
         c1 = fma(m1, one - cc1, cc1);
         s0 = copysign((half * fabs(e0) * y0) * z0, d0 * e0) * (one - m0);
         s1 = copysign((half * fabs(e1) * y1) * z1, d1 * e1) * (one - m1);
     };
 
+    int cur = 0;
     for (int sweep = 0; sweep < max_sweeps; sweep++) {
-        // Off-diagonal and total energy (FP64 sums): per-thread entries,
-        // wavefront sums, then the 4 wavefront sums in a fixed order (uniform).
+        // Off-diagonal and total energy: per-block entries, wavefront sums,
+        // then the 4 wavefront sums in a fixed order (uniform).
         double off = 0.0, tot = 0.0;
-#pragma unroll
-        for (int m = 0; m < IU; m++) {
-            if (tid + 256 * m < NU) {
-                const double x = (double)Ac[ua[m] * LD + ub[m]];
-                if (ua[m] == ub[m]) {
-                    tot += x * x;
-                } else {
-                    off += 2.0 * x * x;
-                }
+        if (is_block) {
+            const double2 lo = Ab[cur][tid], hi = Ab[cur][NB + tid];
+            const double q00 = lo.x * lo.x, q01 = lo.y * lo.y, q10 = hi.x * hi.x, q11 = hi.y * hi.y;
+            if (bu != bv) {
+                off = 2.0 * (q00 + q01 + q10 + q11);
+            } else {
+                tot = q00 + q11;
+                off = q01 + q10;
             }
         }
         for (int o = 32; o > 0; o >>= 1) {
@@ -297,58 +352,55 @@ __device__ void jacobi_sweeps(T*& Ac, T*& An, T* V, double tol, int max_sweeps, 
         __syncthreads();  // energy[] is rewritten by the next sweep
         if (off == 0.0 || off <= tol * tot) break;
 
-        int iu, ju, iv, jv;
-        first(bu, iu, ju);
-        first(bv, iv, jv);
-        for (int round = 0; round < N; round++, step(iu), step(ju), step(iv), step(jv)) {
+        for (int round = 0; round < N; round++) {
+            double* An = reinterpret_cast<double*>(Ab[cur ^ 1]);
+            double* Vn = reinterpret_cast<double*>(Vb[cur ^ 1]);
             if (is_block) {
-                const int pu = min(iu, ju), qu = max(iu, ju), pv = min(iv, jv), qv = max(iv, jv);
-                const T uu0 = Ac[pu * LD + pu], uu1 = Ac[qu * LD + qu], uu2 = Ac[pu * LD + qu];
-                const T vv0 = Ac[pv * LD + pv], vv1 = Ac[qv * LD + qv], vv2 = Ac[pv * LD + qv];
-                const T x00 = Ac[pu * LD + pv], x01 = Ac[pu * LD + qv];
-                const T x10 = Ac[qu * LD + pv], x11 = Ac[qu * LD + qv];
-                T cu, su, cv, sv;  // (cv, sv) = (cu, su) on a diagonal block
-                rotation2(pu == qu, uu0, uu1, uu2, pv == qv, vv0, vv1, vv2, cu, su, cv, sv);
+                const double2 xl = Ab[cur][tid], xh = Ab[cur][NB + tid];
+                const double2 ul = Ab[cur][du], uh = Ab[cur][NB + du], vl = Ab[cur][dv], vh = Ab[cur][NB + dv];
+                double cu, su, cv, sv;
+                rotation2(bu == NS - 1, ul.x, uh.y, ul.y, bv == NS - 1, vl.x, vh.y, vl.y, cu, su, cv, sv);
                 // Z = X J_v, Y = J_u^T Z
-                const T z00 = x00 * cv - x01 * sv, z01 = x00 * sv + x01 * cv;
-                const T z10 = x10 * cv - x11 * sv, z11 = x10 * sv + x11 * cv;
-                const T y00 = cu * z00 - su * z10, y11 = su * z01 + cu * z11;
-                T y01 = cu * z01 - su * z11, y10 = su * z00 + cu * z10;
-                if (bu == bv && bu != NS - 1) y01 = y10 = T(0);  // the annihilated pair
-                An[pu * LD + pv] = y00;
-                An[pu * LD + qv] = y01;
-                An[qu * LD + pv] = y10;
-                An[qu * LD + qv] = y11;
-                An[pv * LD + pu] = y00;
-                An[qv * LD + pu] = y01;
-                An[pv * LD + qu] = y10;
-                An[qv * LD + qu] = y11;
-            } else if (is_v) {
-                const int p = min(iu, ju), q = max(iu, ju);
-                const T a0 = Ac[p * LD + p], a1 = Ac[q * LD + q], a2 = Ac[p * LD + q];
-                T v0[3], v1[3];
+                const double z00 = xl.x * cv - xl.y * sv, z01 = xl.x * sv + xl.y * cv;
+                const double z10 = xh.x * cv - xh.y * sv, z11 = xh.x * sv + xh.y * cv;
+                double y[4] = {cu * z00 - su * z10, cu * z01 - su * z11, su * z00 + cu * z10, su * z01 + cu * z11};
+                if (bu == bv && bu != NS - 1) y[1] = y[2] = 0.0;  // the annihilated pair
 #pragma unroll
-                for (int m = 0; m < 3; m++) {
-                    const int j = min(vj0 + 9 * m, N - 1);
-                    v0[m] = V[j * LD + p];
-                    v1[m] = V[j * LD + q];
+                for (int k = 0; k < 4; k++) {
+                    if (dst[k] >= 0) An[dst[k]] = y[k];
+                    if (mir[k] >= 0) An[mir[k]] = y[k];
                 }
-                T c, s, c_unused, s_unused;
-                rotation2(p == q, a0, a1, a2, p == q, a0, a1, a2, c, s, c_unused, s_unused);
+            } else if (is_v) {
+                const double2 ul = Ab[cur][du], uh = Ab[cur][NB + du];
+                double2 p[3];
+#pragma unroll
+                for (int m = 0; m < 3; m++) p[m] = Vb[cur][min(vj0 + 9 * m, N - 1) * NS + bu];
+                double c, s, c_unused, s_unused;
+                rotation2(bu == NS - 1, ul.x, uh.y, ul.y, bu == NS - 1, ul.x, uh.y, ul.y, c, s, c_unused, s_unused);
 #pragma unroll
                 for (int m = 0; m < 3; m++) {
-                    const int j = vj0 + 9 * m;
-                    if (j < N) {
-                        V[j * LD + p] = c * v0[m] - s * v1[m];
-                        V[j * LD + q] = s * v0[m] + c * v1[m];
-                    }
+                    if (vd[m][0] >= 0) Vn[vd[m][0]] = c * p[m].x - s * p[m].y;
+                    if (vd[m][1] >= 0) Vn[vd[m][1]] = s * p[m].x + c * p[m].y;
                 }
             }
             __syncthreads();
-            T* sw = Ac;
-            Ac = An;
-            An = sw;
+            cur ^= 1;
         }
+    }
+
+    // Back to the plain layout (the frame is the identity at a sweep boundary).
+    const double* A = reinterpret_cast<const double*>(Ab[cur]);
+    const double* Vs = reinterpret_cast<const double*>(Vb[cur]);
+    if (tid < N) {
+        int u, i;
+        loc(tid, u, i);
+        C[tid * LD + tid] = A[elem(block(u, u), i, i)];
+    }
+    for (int e = tid; e < N * N; e += 256) {
+        const int j = e / N, a = e % N;
+        int u, i;
+        loc(a, u, i);
+        V[j * LD + a] = Vs[(j * NS + u) * 2 + i];
     }
 }
 
@@ -363,8 +415,10 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
     constexpr int N = kPatch;
     constexpr int LD = kLD;
     __shared__ double mom[kMoments];
-    __shared__ double A0[N * LD], A1[N * LD];  // current and next A
+    __shared__ double A0[N * LD];  // the covariance; its diagonal the eigenvalues after the sweeps
     __shared__ double V[N * LD];
+    __shared__ double2 Ab[2][2 * slot::NB];  // slot blocks, current and next
+    __shared__ double2 Vb[2][N * slot::NS];  // V row pairs, current and next
     __shared__ double2 energy[4];
     const int tid = threadIdx.x;
 
@@ -378,14 +432,10 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
         A0[r * LD + s] = c;
         A0[s * LD + r] = c;
     }
-    for (int e = tid; e < N * N; e += 256) {
-        const int r = e / N, s = e % N;
-        V[r * LD + s] = r == s ? 1.0 : 0.0;
-    }
     __syncthreads();
-    double *Ac = A0, *An = A1;
-    jacobi_sweeps<double>(Ac, An, V, kJacobiTol, 40, energy);
+    jacobi_sweeps(A0, V, Ab, Vb, kJacobiTol, 40, energy);
     __syncthreads();
+    const double* Ac = A0;
 
     if (tid < N) {
         const double di = Ac[tid * LD + tid];
