@@ -40,6 +40,7 @@ EXPORTED = (
     "gcam_gaussian_blur_sobel_image", "gcam_raw_variance", "gcam_demosaic_image", "gcam_blend_highlights_image",
     "gcam_transform_image", "gcam_despeckle_image", "gcam_cam_ycbcr", "gcam_matrix3_inverse",
     "gcam_front_end_create", "gcam_front_end_destroy", "gcam_front_end_run",
+    "gcam_raw_noise_statistics", "gcam_measure_raw_nlf", "gcam_front_end_raw_nlf",
 )
 GCAM_MAX_LEVELS = 8
 GCAM_MAX_BURST = 64
@@ -88,7 +89,8 @@ class FrontEndParamsC(ctypes.Structure):
     _fields_ = [("bayer_pattern", ctypes.c_int32), ("black_level", ctypes.c_float), ("lens_shading", ctypes.c_float),
                 ("scale_mul", ctypes.c_float * 4), ("raw_nlf_a", ctypes.c_float * 4),
                 ("raw_nlf_b", ctypes.c_float * 4), ("rgb_cam", ctypes.c_float * 9),
-                ("highlight_clip", ctypes.c_float), ("despeckle", ctypes.c_int32), ("despeckle_nlf", NlfC)]
+                ("highlight_clip", ctypes.c_float), ("despeckle", ctypes.c_int32), ("despeckle_nlf", NlfC),
+                ("calibrate_raw_nlf", ctypes.c_int32), ("exposure_multiplier", ctypes.c_float)]
 
 
 _lib = None
@@ -154,6 +156,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.gcam_front_end_destroy.argtypes = [_P]
     L.gcam_front_end_destroy.restype = None
     L.gcam_front_end_run.argtypes = [_P, _IP, ctypes.POINTER(FrontEndParamsC), _IP, _IP, _P]
+    L.gcam_raw_noise_statistics.argtypes = [_IP, _I, _IP, _IP, _P]
+    L.gcam_measure_raw_nlf.argtypes = [_IP, _I, _F, _FP, _FP, _P]
+    L.gcam_front_end_raw_nlf.argtypes = [_P, _FP, _FP]
     for name in EXPORTED:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int:  # default restype: gcam_status

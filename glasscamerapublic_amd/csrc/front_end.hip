@@ -11,6 +11,8 @@
 //   blendHighlightsImage          MetalPipeline/demosaic.metal:523-564
 //   transformImage                MetalPipeline/demosaic.metal:649-662
 //   despeckleLumaMedianChromaImage MetalPipeline/demosaic.metal:1160-1283
+//   basicRawNoiseStatistics       MetalPipeline/demosaic.metal:1096-1136
+//   MeasureRawNLF                 MetalPipeline/raw_converter.cpp:339-490
 //   gaussianKernelBilinearWeights MetalPipeline/demosaic_utils.cpp:939-1006
 //   cam_ycbcr                     MetalPipeline/demosaic_utils.cpp:703-720
 //   orchestration                 MetalPipeline/raw_converter.cpp:148-214, 70-78
@@ -487,6 +489,206 @@ __global__ __launch_bounds__(256) void k_despeckle(View<const float4> in, View<f
     out.at(x, y) = make_float4(clampf(centre, lo, hi), median9(cb), median9(cr), 0.0f);
 }
 
+
+// ---- raw noise model: basicRawNoiseStatistics + MeasureRawNLF ------------
+
+// Per quad: mean and population variance of the quad samples read at
+// 2 (qx, qy) + (x, y) + offset[c], x, y in [-4, 4] (the window steps by one
+// pixel, as the reference's readRAWQuad is called), summed row by row.
+__global__ __launch_bounds__(256) void k_raw_noise_stats(View<const float> raw, Quad q, View<float4> mean,
+                                                         View<float4> var) {
+    const int qx = blockIdx.x * blockDim.x + threadIdx.x, qy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (qx >= mean.w || qy >= mean.h) return;
+    float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sq[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int y = -4; y <= 4; y++) {
+        for (int x = -4; x <= 4; x++) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const float v = fetch(raw, 2 * qx + x + q.o[c].x, 2 * qy + y + q.o[c].y);
+                sum[c] = sum[c] + v;
+                sq[c] = sq[c] + v * v;
+            }
+        }
+    }
+    const float count = 81.0f;
+    mean.at(qx, qy) = make_float4(sum[0] / count, sum[1] / count, sum[2] / count, sum[3] / count);
+    var.at(qx, qy) = make_float4((sq[0] - (sum[0] * sum[0]) / count) / count, (sq[1] - (sum[1] * sum[1]) / count) / count,
+                                 (sq[2] - (sum[2] * sum[2]) / count) / count, (sq[3] - (sum[3] * sum[3]) / count) / count);
+}
+
+// Fit state on the device (FP64), advanced by the finalize kernels.
+struct RawNlfState {
+    double vmax[4], A[4], B[4], err2[4];
+    double n1, n2, accepted, vmax1;
+};
+constexpr int kRawNlfBlocks = 512, kRawNlfValues = 21;
+
+template <int PASS>
+struct RawPassWidth;
+template <>
+struct RawPassWidth<0> { static constexpr int value = 6; };   // histogram
+template <>
+struct RawPassWidth<1> { static constexpr int value = 17; };  // sx, sy, sxx, sxy (x4), N
+template <>
+struct RawPassWidth<2> { static constexpr int value = 4; };   // squared errors
+template <>
+struct RawPassWidth<3> { static constexpr int value = 21; };  // pass-1 sums over the fitting quads, errors
+
+__device__ __forceinline__ bool nan4(float4 v) { return isnan(v.x) || isnan(v.y) || isnan(v.z) || isnan(v.w); }
+
+// Fixed-order block sum of NV per-thread doubles; thread 0 writes them.
+template <int NV>
+__device__ void block_sum(double (&acc)[NV], double* dst) {
+    __shared__ double red[256];
+    for (int v = 0; v < NV; v++) {
+        red[threadIdx.x] = acc[v];
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) dst[v] = red[0];
+        __syncthreads();
+    }
+}
+
+// One pass over the statistics images (raw_converter.cpp:353-363, 379-394,
+// 402-414, 425-446); per-block partials.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_raw_nlf_pass(View<const float4> mean, View<const float4> var,
+                                                      const RawNlfState* st, double* partials) {
+    constexpr int NV = RawPassWidth<PASS>::value;
+    double acc[NV];
+    for (int v = 0; v < NV; v++) acc[v] = 0.0;
+    double vmax[4], A[4], B[4], e2[4];
+    for (int c = 0; c < 4; c++) {
+        vmax[c] = st->vmax[c];
+        A[c] = st->A[c];
+        B[c] = st->B[c];
+        e2[c] = st->err2[c];
+    }
+    const size_t n = (size_t)mean.w * mean.h;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+        const int x = (int)(k % mean.w), y = (int)(k / mean.w);
+        const float4 vv = var.at(x, y);
+        const double v[4] = {vv.x, vv.y, vv.z, vv.w};
+        if (PASS == 0) {
+            if (nan4(vv) || !(v[0] > 0 && v[1] > 0 && v[2] > 0 && v[3] > 0)) continue;
+            double mx = fmax(fmax(log10(v[0]), log10(v[1])), fmax(log10(v[2]), log10(v[3])));
+            mx = fmin(fmax(mx, -6.0), -1.0);
+            const int entry = 6 + (int)mx;
+            for (int e = 0; e < 6; e++) acc[e] += (e == entry) ? 1.0 : 0.0;
+            continue;
+        }
+        const float4 mm = mean.at(x, y);
+        if (nan4(mm) || nan4(vv)) continue;
+        const double m[4] = {mm.x, mm.y, mm.z, mm.w};
+        bool sel = true;
+        for (int c = 0; c < 4; c++) sel = sel && m[c] >= 0.001 && m[c] <= 0.9 && v[c] <= vmax[c];
+        if (!sel) continue;
+        if (PASS == 2) {
+            for (int c = 0; c < 4; c++) {
+                const double d = A[c] + B[c] * m[c] - v[c];
+                acc[c] += d * d;
+            }
+            continue;
+        }
+        double dsq[4];
+        if (PASS == 3) {
+            bool fits = true;
+            for (int c = 0; c < 4; c++) {
+                const double d = fabs(A[c] + B[c] * m[c] - v[c]);
+                dsq[c] = d * d;
+                fits = fits && dsq[c] <= 0.5 * e2[c];
+            }
+            if (!fits) continue;
+        }
+        for (int c = 0; c < 4; c++) {
+            acc[c] += m[c];
+            acc[4 + c] += v[c];
+            acc[8 + c] += m[c] * m[c];
+            acc[12 + c] += m[c] * v[c];
+        }
+        acc[16] += 1.0;
+        if (PASS == 3)
+            for (int c = 0; c < 4; c++) acc[17 + c] += dsq[c];
+    }
+    block_sum<NV>(acc, partials + (size_t)blockIdx.x * kRawNlfValues);
+}
+
+__device__ __forceinline__ double floor_max(double v, double fl) { return isnan(v) ? fl : fmax(v, fl); }
+
+// Clears the fit state with a kernel (no memset node if a caller captures the stream).
+__global__ void k_raw_nlf_init(RawNlfState* st) {
+    if (threadIdx.x < 20) reinterpret_cast<double*>(st)[threadIdx.x] = 0.0;
+}
+static_assert(sizeof(RawNlfState) == 20 * sizeof(double), "RawNlfState layout");
+
+// Sum the partials (fixed order) and advance the fit (raw_converter.cpp:366-372,
+// 397-398, 416-421, 448-484).  out: a[4], b[4] after the last pass.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_raw_nlf_finalize(const double* partials, int nblocks, RawNlfState* st,
+                                                          double exposure_multiplier, double npx, double* out) {
+    constexpr int NV = RawPassWidth<PASS>::value;
+    __shared__ double tot[kRawNlfValues];
+    __shared__ double red[256];
+    for (int v = 0; v < NV; v++) {
+        double a = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += 256) a += partials[(size_t)b * kRawNlfValues + v];
+        red[threadIdx.x] = a;
+        __syncthreads();
+        for (int s2 = 128; s2 > 0; s2 >>= 1) {
+            if ((int)threadIdx.x < s2) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s2];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) tot[v] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (PASS == 0) {
+        const double* h = tot;
+        const double vm = h[0] > 1e3 ? 1.0e-5 : h[1] > 1e3 ? 1.0e-4 : h[2] > 1e3 ? 1.0e-3
+                        : h[3] > 1e3 ? 1.0e-2 : h[4] > 1e3 ? 1.0e-1 : 1.0;
+        for (int c = 0; c < 4; c++) st->vmax[c] = vm;
+        st->vmax1 = vm;
+    } else if (PASS == 1 || PASS == 3) {
+        const double N = tot[16];
+        bool take = true;
+        if (PASS == 3) {
+            take = N > 0.001 * npx;
+            for (int c = 0; c < 4; c++) {
+                const double e = tot[17 + c] / N;
+                take = take && !isnan(e) && e < st->err2[c];
+            }
+            st->n2 = N;
+            st->accepted = take ? 1.0 : 0.0;
+        } else {
+            st->n1 = N;
+        }
+        if (take) {
+            for (int c = 0; c < 4; c++) {
+                const double sx = tot[c], sy = tot[4 + c], sxx = tot[8 + c], sxy = tot[12 + c];
+                const double B = floor_max((N * sxy - sx * sy) / (N * sxx - sx * sx), 1e-8);
+                st->B[c] = B;
+                st->A[c] = floor_max((sy - B * sx) / N, 1e-8);
+                if (PASS == 3) st->err2[c] = tot[17 + c] / N;
+            }
+        }
+        if (PASS == 3) {
+            const double adj = exposure_multiplier * exposure_multiplier;
+            for (int c = 0; c < 4; c++) {
+                out[c] = st->A[c] * adj;
+                out[4 + c] = st->B[c] * adj;
+            }
+        }
+    } else {  // PASS 2
+        for (int c = 0; c < 4; c++) {
+            st->err2[c] = tot[c] / st->n1;
+            st->vmax[c] = st->B[c];
+        }
+    }
+}
+
 // ---- host side -----------------------------------------------------------
 
 // gaussianKernelBilinearWeights (demosaic_utils.cpp:987-1006) with
@@ -569,6 +771,26 @@ gcam_status launch_status(const char* what) {
     return GCAM_OK;
 }
 
+
+// Statistics and the four fit passes on stream s; scratch: kRawNlfBlocks *
+// kRawNlfValues doubles, a RawNlfState, 8 doubles of output (a[4], b[4]).
+gcam_status raw_nlf_device(View<const float> scaled, const Quad& q, View<float4> mean, View<float4> var,
+                           double* partials, RawNlfState* state, double* out, double exposure, hipStream_t s) {
+    const dim3 b(64, 4);
+    k_raw_noise_stats<<<grid_for(mean.w, mean.h, b), b, 0, s>>>(scaled, q, mean, var);
+    k_raw_nlf_init<<<1, 64, 0, s>>>(state);
+    const View<const float4> m{mean.ptr, mean.w, mean.h, mean.stride}, v{var.ptr, var.w, var.h, var.stride};
+    const double npx = 4.0 * mean.w * mean.h;
+    k_raw_nlf_pass<0><<<kRawNlfBlocks, 256, 0, s>>>(m, v, state, partials);
+    k_raw_nlf_finalize<0><<<1, 256, 0, s>>>(partials, kRawNlfBlocks, state, exposure, npx, out);
+    k_raw_nlf_pass<1><<<kRawNlfBlocks, 256, 0, s>>>(m, v, state, partials);
+    k_raw_nlf_finalize<1><<<1, 256, 0, s>>>(partials, kRawNlfBlocks, state, exposure, npx, out);
+    k_raw_nlf_pass<2><<<kRawNlfBlocks, 256, 0, s>>>(m, v, state, partials);
+    k_raw_nlf_finalize<2><<<1, 256, 0, s>>>(partials, kRawNlfBlocks, state, exposure, npx, out);
+    k_raw_nlf_pass<3><<<kRawNlfBlocks, 256, 0, s>>>(m, v, state, partials);
+    k_raw_nlf_finalize<3><<<1, 256, 0, s>>>(partials, kRawNlfBlocks, state, exposure, npx, out);
+    return launch_status("MeasureRawNLF");
+}
 }  // namespace
 
 }  // namespace gcam
@@ -744,6 +966,53 @@ gcam_status gcam_despeckle_image(const gcam_image* in, const float var_a[3], con
     return launch_status("despeckleLumaMedianChromaImage");
 }
 
+
+gcam_status gcam_raw_noise_statistics(const gcam_image* scaled, int32_t bayer, gcam_image* mean, gcam_image* var,
+                                      void* stream) {
+    if (!valid_image(scaled, GCAM_PIXEL_R32F) || !valid_image(mean, GCAM_PIXEL_RGBA32F) ||
+        !valid_image(var, GCAM_PIXEL_RGBA32F) || !same_size(mean, var) || mean->width != scaled->width / 2 ||
+        mean->height != scaled->height / 2)
+        return bad("gcam_raw_noise_statistics: scaled R32F, mean and var RGBA32F of half its size");
+    if (bayer < 0 || bayer > 3) return bad("gcam_raw_noise_statistics: bad Bayer pattern");
+    const dim3 b(64, 4);
+    k_raw_noise_stats<<<grid_for(mean->width, mean->height, b), b, 0, (hipStream_t)stream>>>(
+        view<const float>(*scaled), quad_of(bayer), view<float4>(*mean), view<float4>(*var));
+    return launch_status("basicRawNoiseStatistics");
+}
+
+gcam_status gcam_measure_raw_nlf(const gcam_image* scaled, int32_t bayer, float exposure_multiplier,
+                                 float raw_nlf_a[4], float raw_nlf_b[4], void* stream) {
+    if (!valid_image(scaled, GCAM_PIXEL_R32F) || ((scaled->width | scaled->height) & 1) || !raw_nlf_a || !raw_nlf_b)
+        return bad("gcam_measure_raw_nlf: scaled must be R32F with even sizes, outputs non-null");
+    if (bayer < 0 || bayer > 3) return bad("gcam_measure_raw_nlf: bad Bayer pattern");
+    const hipStream_t s = (hipStream_t)stream;
+    const int qw = scaled->width / 2, qh = scaled->height / 2;
+    const size_t img = (size_t)qw * qh * sizeof(float4);
+    const size_t part = (size_t)kRawNlfBlocks * kRawNlfValues * sizeof(double);
+    char* buf = nullptr;
+    if (hipMalloc(&buf, 2 * img + part + sizeof(RawNlfState) + 8 * sizeof(double)) != hipSuccess)
+        return set_error(GCAM_ERR_OUT_OF_MEMORY, "gcam_measure_raw_nlf: device allocation failed");
+    View<float4> mean{reinterpret_cast<float4*>(buf), qw, qh, qw}, var{reinterpret_cast<float4*>(buf + img), qw, qh, qw};
+    double* partials = reinterpret_cast<double*>(buf + 2 * img);
+    RawNlfState* state = reinterpret_cast<RawNlfState*>(buf + 2 * img + part);
+    double* out = reinterpret_cast<double*>(buf + 2 * img + part + sizeof(RawNlfState));
+    gcam_status st = raw_nlf_device(view<const float>(*scaled), quad_of(bayer), mean, var, partials, state, out,
+                                    exposure_multiplier, s);
+    double host[8];
+    if (st == GCAM_OK) {
+        const hipError_t e1 = hipMemcpyAsync(host, out, sizeof(host), hipMemcpyDeviceToHost, s);
+        const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(s) : e1;
+        if (e2 != hipSuccess) st = set_error(GCAM_ERR_HIP, std::string("gcam_measure_raw_nlf: ") + hipGetErrorString(e2));
+    }
+    hipFree(buf);
+    if (st != GCAM_OK) return st;
+    for (int c = 0; c < 4; c++) {
+        raw_nlf_a[c] = (float)host[c];
+        raw_nlf_b[c] = (float)host[4 + c];
+    }
+    return GCAM_OK;
+}
+
 // ---- the front end as one call -------------------------------------------
 
 struct gcam_front_end {
@@ -752,6 +1021,8 @@ struct gcam_front_end {
     float* green;
     float4* rgb;  // interpolateRedBlue output
     float4* ycc;  // before the despeckle
+    char* nlf_scratch;      // raw statistics images, fit partials, state, output
+    float raw_a[4], raw_b[4];  // the raw noise model of the last run
 };
 
 gcam_status gcam_front_end_create(int32_t width, int32_t height, gcam_front_end** out) {
@@ -779,6 +1050,7 @@ void gcam_front_end_destroy(gcam_front_end* f) {
     hipFree(f->green);
     hipFree(f->rgb);
     hipFree(f->ycc);
+    hipFree(f->nlf_scratch);
     delete f;
 }
 
@@ -797,8 +1069,39 @@ gcam_status gcam_front_end_run(gcam_front_end* f, const gcam_image* raw, const g
     gcam_image rgb{f->rgb, w, h, w, GCAM_PIXEL_RGBA32F}, pre{f->ycc, w, h, w, GCAM_PIXEL_RGBA32F};
     gcam_status st = gcam_scale_raw_data(raw, &sc, p->bayer_pattern, p->scale_mul, p->black_level, p->lens_shading, s);
     if (st != GCAM_OK) return st;
+    for (int c = 0; c < 4; c++) {
+        f->raw_a[c] = p->raw_nlf_a[c];
+        f->raw_b[c] = p->raw_nlf_b[c];
+    }
+    if (p->calibrate_raw_nlf) {
+        // MeasureRawNLF on the scaled raw (raw_converter.cpp:187-189): statistics
+        // and fit on the device, then one synchronization for the fitted model.
+        const int qw = w / 2, qh = h / 2;
+        const size_t img = (size_t)qw * qh * sizeof(float4), part = (size_t)kRawNlfBlocks * kRawNlfValues * sizeof(double);
+        if (!f->nlf_scratch && hipMalloc(&f->nlf_scratch, 2 * img + part + sizeof(RawNlfState) + 8 * sizeof(double)) !=
+                                   hipSuccess) {
+            f->nlf_scratch = nullptr;
+            return set_error(GCAM_ERR_OUT_OF_MEMORY, "gcam_front_end_run: device allocation failed");
+        }
+        char* buf = f->nlf_scratch;
+        View<float4> mean{reinterpret_cast<float4*>(buf), qw, qh, qw}, var{reinterpret_cast<float4*>(buf + img), qw, qh, qw};
+        double* out = reinterpret_cast<double*>(buf + 2 * img + part + sizeof(RawNlfState));
+        if ((st = raw_nlf_device(View<const float>{f->scaled, w, h, w}, quad_of(p->bayer_pattern), mean, var,
+                                 reinterpret_cast<double*>(buf + 2 * img),
+                                 reinterpret_cast<RawNlfState*>(buf + 2 * img + part), out, p->exposure_multiplier, s)) !=
+            GCAM_OK)
+            return st;
+        double host[8];
+        hipError_t e = hipMemcpyAsync(host, out, sizeof(host), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return set_error(GCAM_ERR_HIP, std::string("gcam_front_end_run: ") + hipGetErrorString(e));
+        for (int c = 0; c < 4; c++) {
+            f->raw_a[c] = (float)host[c];
+            f->raw_b[c] = (float)host[4 + c];
+        }
+    }
     float rv[6];
-    gcam_raw_variance(p->raw_nlf_a, p->raw_nlf_b, rv);
+    gcam_raw_variance(f->raw_a, f->raw_b, rv);
     if ((st = blur_sobel(&sc, nullptr, 1.5f, 4.5f, rv + 2, gradient, s)) != GCAM_OK) return st;
     Mat3 t;
     gcam_cam_ycbcr(p->rgb_cam, t.m);
@@ -807,6 +1110,15 @@ gcam_status gcam_front_end_run(gcam_front_end* f, const gcam_image* raw, const g
         GCAM_OK)
         return st;
     if (p->despeckle) return gcam_despeckle_image(&pre, p->despeckle_nlf.a, p->despeckle_nlf.b, ycbcr, s);
+    return GCAM_OK;
+}
+
+gcam_status gcam_front_end_raw_nlf(const gcam_front_end* f, float raw_nlf_a[4], float raw_nlf_b[4]) {
+    if (!f || !raw_nlf_a || !raw_nlf_b) return bad("gcam_front_end_raw_nlf: null argument");
+    for (int c = 0; c < 4; c++) {
+        raw_nlf_a[c] = f->raw_a[c];
+        raw_nlf_b[c] = f->raw_b[c];
+    }
     return GCAM_OK;
 }
 

@@ -158,6 +158,26 @@ def despeckleImage(inputImage: torch.Tensor, var_a, var_b, out: Optional[torch.T
     return out
 
 
+def rawNoiseStatistics(scaledRawImage: torch.Tensor, bayerPattern=2,
+                       stream=None) -> Tuple[torch.Tensor, torch.Tensor]:
+    """basicRawNoiseStatistics (demosaic.metal:1110-1136): per-quad (mean, variance) images, (H/2, W/2, 4)."""
+    h, w = scaledRawImage.shape
+    mean, var = _like(scaledRawImage, h // 2, w // 2, 4), _like(scaledRawImage, h // 2, w // 2, 4)
+    check(lib().gcam_raw_noise_statistics(ctypes.byref(_img(scaledRawImage)), _bayer(bayerPattern),
+                                          ctypes.byref(_img(mean)), ctypes.byref(_img(var)),
+                                          ctypes.c_void_p(_stream(stream))))
+    return mean, var
+
+
+def measureRawNLF(scaledRawImage: torch.Tensor, bayerPattern=2, exposure_multiplier: float = 1.0,
+                  stream=None) -> Tuple[Tuple[float, ...], Tuple[float, ...]]:
+    """MeasureRawNLF (raw_converter.cpp:339-490): the RawNLF (a[4], b[4]); synchronizes."""
+    a, b = (ctypes.c_float * 4)(), (ctypes.c_float * 4)()
+    check(lib().gcam_measure_raw_nlf(ctypes.byref(_img(scaledRawImage)), _bayer(bayerPattern),
+                                     float(exposure_multiplier), a, b, ctypes.c_void_p(_stream(stream))))
+    return tuple(a), tuple(b)
+
+
 def cam_ycbcr(rgb_cam=np.eye(3)) -> np.ndarray:
     """cam_ycbcr(rgb_cam, xyz_sRGB) (demosaic_utils.cpp:703-720)."""
     out = (ctypes.c_float * 9)()
@@ -183,6 +203,8 @@ class FrontEndParameters:
     rgb_cam: Sequence[Sequence[float]] = field(default_factory=lambda: np.eye(3).tolist())
     highlightClip: float = 1.0
     despeckleNlf: Optional[Tuple[Sequence[float], Sequence[float]]] = None  # pyramidNlf[0]; None: no despeckle
+    calibrateRawNlf: bool = False       # fit rawNlf from the frame (MeasureRawNLF); the run then synchronizes
+    exposure_multiplier: float = 1.0
 
     def to_c(self) -> FrontEndParamsC:
         c = FrontEndParamsC()
@@ -197,6 +219,8 @@ class FrontEndParameters:
         c.despeckle = 1 if self.despeckleNlf is not None else 0
         if self.despeckleNlf is not None:
             c.despeckle_nlf = NlfC(_floats(self.despeckleNlf[0], 3), _floats(self.despeckleNlf[1], 3))
+        c.calibrate_raw_nlf = 1 if self.calibrateRawNlf else 0
+        c.exposure_multiplier = float(self.exposure_multiplier)
         return c
 
 
@@ -225,6 +249,13 @@ class RawFrontEnd:
                                        ctypes.byref(_img(ycbcr)), ctypes.byref(_img(gradient)),
                                        ctypes.c_void_p(_stream(stream))))
         return ycbcr, gradient
+
+    @property
+    def rawNlf(self) -> Tuple[Tuple[float, ...], Tuple[float, ...]]:
+        """The raw noise model of the last run (fitted when it calibrated)."""
+        a, b = (ctypes.c_float * 4)(), (ctypes.c_float * 4)()
+        check(lib().gcam_front_end_raw_nlf(self._h, a, b))
+        return tuple(a), tuple(b)
 
     def close(self) -> None:
         if self._h is not None:

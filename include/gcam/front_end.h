@@ -2,9 +2,9 @@
  * gcam front end -- C ABI of the raw front end that produces the inputs of
  * the pyramid denoise (gcam/denoise.h) on MI355X (gfx950): the steps of
  * RawConverter::demosaic up to the YCbCr image and the gradient image
- * (MetalPipeline/raw_converter.cpp:148-214), and the despeckle that
- * RawConverter::denoise runs before PyramidProcessor::denoise
- * (raw_converter.cpp:70-78).
+ * (MetalPipeline/raw_converter.cpp:148-214), the raw noise-model fit
+ * (MeasureRawNLF, :339-490), and the despeckle that RawConverter::denoise runs
+ * before PyramidProcessor::denoise (raw_converter.cpp:70-78).
  *
  * Same conventions as gcam/denoise.h: device images, stream-ordered
  * asynchronous calls, status codes, gcam_last_error() for the message.
@@ -84,6 +84,22 @@ gcam_status gcam_transform_image(const gcam_image* in, const float m[9], gcam_im
 gcam_status gcam_despeckle_image(const gcam_image* in, const float var_a[3], const float var_b[3], gcam_image* out,
                                  void* stream);
 
+/* basicRawNoiseStatistics (demosaic.metal:1096-1136, driven at
+ * raw_converter.cpp:340): per 2x2 quad, the mean and variance of the red,
+ * green, blue and green2 samples over a 9x9 window of quad reads (the window
+ * steps one pixel at a time, as in the reference).  mean, var: RGBA32F of half
+ * the scaled raw's size. */
+gcam_status gcam_raw_noise_statistics(const gcam_image* scaled, int32_t bayer_pattern, gcam_image* mean,
+                                      gcam_image* var, void* stream);
+
+/* MeasureRawNLF (MetalPipeline/raw_converter.cpp:339-490): the statistics and
+ * the reference's FP64 least-squares fit var = a + b * mean per quad colour
+ * (histogram-chosen variance ceiling, refit on the quads within half the mean
+ * squared error), on the device; synchronizes the stream and writes the
+ * RawNLF (a[4], b[4], scaled by exposure_multiplier^2) to the host arrays. */
+gcam_status gcam_measure_raw_nlf(const gcam_image* scaled, int32_t bayer_pattern, float exposure_multiplier,
+                                 float raw_nlf_a[4], float raw_nlf_b[4], void* stream);
+
 /* cam_ycbcr(rgb_cam, xyz_sRGB) (MetalPipeline/demosaic_utils.cpp:703-720,
  * raw_converter.cpp:166): the camera-RGB -> YCbCr matrix, row-major.  Host only. */
 void gcam_cam_ycbcr(const float rgb_cam[9], float out[9]);
@@ -108,6 +124,11 @@ typedef struct gcam_front_end_params {
     float highlight_clip;        /* blendHighlightsImage clip; 1.0 in the reference  */
     int32_t despeckle;           /* 1: run the despeckle with despeckle_nlf          */
     gcam_ycbcr_nlf despeckle_nlf; /* NoiseModel.pyramidNlf[0]                        */
+    int32_t calibrate_raw_nlf;   /* 1: fit the raw noise model from the frame
+                                    (MeasureRawNLF, raw_converter.cpp:187-189) and
+                                    use it instead of raw_nlf_a / raw_nlf_b; the
+                                    call then synchronizes once, after the fit      */
+    float exposure_multiplier;   /* MeasureRawNLF's exposure_multiplier              */
 } gcam_front_end_params;
 
 typedef struct gcam_front_end gcam_front_end;
@@ -121,6 +142,11 @@ void gcam_front_end_destroy(gcam_front_end* front_end);
  * gradient (RG32F, its gradient image).  One handle per stream. */
 gcam_status gcam_front_end_run(gcam_front_end* front_end, const gcam_image* raw, const gcam_front_end_params* params,
                                gcam_image* ycbcr, gcam_image* gradient, void* stream);
+
+/* The raw noise model the last gcam_front_end_run used (fitted when it
+ * calibrated; NoiseModel.rawNlf, written back by the reference at
+ * raw_converter.cpp:187-189).  Host only. */
+gcam_status gcam_front_end_raw_nlf(const gcam_front_end* front_end, float raw_nlf_a[4], float raw_nlf_b[4]);
 
 #ifdef __cplusplus
 }

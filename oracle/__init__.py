@@ -80,6 +80,8 @@ def _declare(L):
     L.oracle_transform.argtypes = [_fp, _i, _i, _fp, _fp]
     L.oracle_despeckle.argtypes = [_fp, _i, _i, _fp, _fp, _fp]
     L.oracle_cam_ycbcr.argtypes = [_fp, _fp]
+    L.oracle_raw_noise_stats.argtypes = [_fp, _i, _i, _i, _fp, _fp]
+    L.oracle_fit_raw_nlf.argtypes = [_fp, _fp, _i, _i, _d, _d, _fp, _fp, _fp]
     L.oracle_set_threads.argtypes = [_i]
     L.oracle_get_threads.restype = _i
 
@@ -426,11 +428,54 @@ def cam_ycbcr(rgb_cam=np.eye(3)):
     return out.reshape(3, 3)
 
 
+def raw_noise_stats(scaled, bayer=2):
+    """basicRawNoiseStatistics, MetalPipeline/demosaic.metal:1096-1136: (mean, var), (H/2, W/2, 4)."""
+    scaled = _f32(scaled)
+    h, w = scaled.shape
+    mean = np.empty((h // 2, w // 2, 4), np.float32)
+    var = np.empty((h // 2, w // 2, 4), np.float32)
+    lib().oracle_raw_noise_stats(_p(scaled), w, h, int(bayer), _p(mean), _p(var))
+    return mean, var
+
+
+@dataclass
+class RawNlfFit:
+    a: np.ndarray
+    b: np.ndarray
+    n1: float
+    n2: float
+    accepted: bool
+    variance_max: float
+    err2: np.ndarray
+
+
+def fit_raw_nlf(mean, var, raw_w, raw_h, exposure_multiplier=1.0) -> RawNlfFit:
+    """MeasureRawNLF's FP64 fit (MetalPipeline/raw_converter.cpp:352-489) on statistics images."""
+    mean, var = _f32(mean), _f32(var)
+    qh, qw = mean.shape[:2]
+    a, b, diag = np.empty(4), np.empty(4), np.empty(8)
+    lib().oracle_fit_raw_nlf(_p(mean), _p(var), qw, qh, float(exposure_multiplier), float(raw_w) * float(raw_h),
+                             _p(a), _p(b), _p(diag))
+    return RawNlfFit(a, b, diag[0], diag[1], bool(diag[2]), diag[3], diag[4:])
+
+
+def measure_raw_nlf(scaled, bayer=2, exposure_multiplier=1.0) -> RawNlfFit:
+    """MeasureRawNLF, MetalPipeline/raw_converter.cpp:339-490."""
+    h, w = np.asarray(scaled).shape
+    mean, var = raw_noise_stats(scaled, bayer)
+    return fit_raw_nlf(mean, var, w, h, exposure_multiplier)
+
+
 def raw_front_end(raw, bayer=2, scale_mul=(1.0, 1.0, 1.0, 1.0), black_level=0.0, lens_correction=0.0,
-                  raw_nlf_a=(1e-5,) * 4, raw_nlf_b=(1e-3,) * 4, rgb_cam=np.eye(3), clip=1.0, despeckle_nlf=None):
+                  raw_nlf_a=(1e-5,) * 4, raw_nlf_b=(1e-3,) * 4, rgb_cam=np.eye(3), clip=1.0, despeckle_nlf=None,
+                  calibrate=False, exposure_multiplier=1.0):
     """The denoise inputs from a raw frame (MetalPipeline/raw_converter.cpp:179-214 and 70-78):
-    returns (ycbcr image after the despeckle, gradient, ycbcr before it)."""
+    returns (ycbcr image after the despeckle, gradient, ycbcr before it); with calibrate the raw
+    noise model is fitted from the frame (MeasureRawNLF) and replaces raw_nlf_a / raw_nlf_b."""
     scaled = scale_raw(raw, bayer, scale_mul, black_level, lens_correction)
+    if calibrate:
+        fit = measure_raw_nlf(scaled, bayer, exposure_multiplier)
+        raw_nlf_a, raw_nlf_b = fit.a.astype(np.float32), fit.b.astype(np.float32)
     rv = raw_variance(raw_nlf_a, raw_nlf_b)
     grad = blur_sobel(scaled, raw_sobel(scaled), rv[1])
     _, _, rgb = demosaic(scaled, grad, bayer, rv)

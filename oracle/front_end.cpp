@@ -3,8 +3,10 @@
 // TEST INFRASTRUCTURE ONLY.  A plain FP32 restatement of the steps of
 // RawConverter::demosaic (MetalPipeline/raw_converter.cpp:148-213) that turn a
 // Bayer raw frame into the YCbCr image and the gradient image given to
-// PyramidProcessor::denoise, and of the despeckle that RawConverter::denoise
-// runs first (raw_converter.cpp:70-78).  Only the test suite loads it.
+// PyramidProcessor::denoise, of the despeckle that RawConverter::denoise runs
+// first (raw_converter.cpp:70-78), and of the raw noise-model fit
+// (MeasureRawNLF, raw_converter.cpp:339-490, FP64 as in the reference).  Only
+// the test suite loads it.
 //
 // Conventions fixed here (the reference leaves them to Metal):
 //   * every out-of-bounds texture.read() clamps to the edge;
@@ -481,6 +483,151 @@ void oracle_cam_ycbcr(const float* rgb_cam, float* out) {
     const float m[9] = {kr, kg, kb, -0.5f * kr / (1.0f - kb), -0.5f * kg / (1 - kb), 0.5f,
                         0.5f, -0.5f * kg / (1 - kr), -0.5f * kb / (1 - kr)};
     std::copy(m, m + 9, out);
+}
+
+// MetalPipeline/demosaic.metal:1096-1136 (readRAWQuad, basicRawNoiseStatistics):
+// per quad (w/2 x h/2), the mean and population variance of the four quad
+// samples read at 2 (qx, qy) + (x, y) + offset[c] for x, y in [-4, 4] (the
+// window steps by one pixel, so odd steps read the quad shifted by a pixel,
+// as the reference does), in FP32, summed row by row.
+void oracle_raw_noise_stats(const float* raw, int w, int h, int bayer, float* mean4, float* var4) {
+    const Img R{raw, w, h, 1};
+    const int qw = w / 2, qh = h / 2;
+    const float count = 81.0f;
+    for (int qy = 0; qy < qh; qy++) {
+        for (int qx = 0; qx < qw; qx++) {
+            float sum[4] = {0, 0, 0, 0}, sq[4] = {0, 0, 0, 0};
+            for (int y = -4; y <= 4; y++) {
+                for (int x = -4; x <= 4; x++) {
+                    for (int c = 0; c < 4; c++) {
+                        const float v = R.at(2 * qx + x + kBayer[bayer][c][0], 2 * qy + y + kBayer[bayer][c][1]);
+                        sum[c] = sum[c] + v;
+                        sq[c] = sq[c] + v * v;
+                    }
+                }
+            }
+            for (int c = 0; c < 4; c++) {
+                mean4[((size_t)qy * qw + qx) * 4 + c] = sum[c] / count;
+                var4[((size_t)qy * qw + qx) * 4 + c] = (sq[c] - (sum[c] * sum[c]) / count) / count;
+            }
+        }
+    }
+}
+
+// MetalPipeline/raw_converter.cpp:339-490 (MeasureRawNLF), FP64 on the
+// statistics images (qw x qh quads; npx = the raw frame's pixel count):
+// 1. log10 histogram of the largest channel variance (quads with four positive
+//    variances) picks varianceMax;
+// 2. per-channel least squares var = A + B mean over quads whose four means
+//    are in [0.001, 0.9] and four variances <= varianceMax; B, A >= 1e-8;
+// 3. the fit's mean squared error per channel; varianceMax = B;
+// 4. refit on the quads whose squared residuals are all <= err2 / 2, kept if
+//    they are more than 0.1 % of npx, their error is finite and smaller in
+//    every channel (the reference writes newErr2 < err2 on a vector type);
+// 5. A and B times exposure_multiplier^2.
+// diag (nullable, 8): N1, N2, accepted, varianceMax of pass 1, then the four
+// err2 values of the kept fit.
+void oracle_fit_raw_nlf(const float* mean4, const float* var4, int qw, int qh, double exposure_multiplier,
+                        double npx, double* a4, double* b4, double* diag) {
+    const size_t n = (size_t)qw * qh;
+    auto nan4 = [](const float* p) { return std::isnan(p[0]) || std::isnan(p[1]) || std::isnan(p[2]) || std::isnan(p[3]); };
+    double hist[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t k = 0; k < n; k++) {
+        const float* v = var4 + 4 * k;
+        // log10 needs positive variances: quads with a NaN, zero or negative
+        // (rounding) variance are not counted, as in the YCbCr fit.
+        if (nan4(v) || !(v[0] > 0 && v[1] > 0 && v[2] > 0 && v[3] > 0)) continue;
+        double mx = -INFINITY;
+        for (int c = 0; c < 4; c++) mx = std::fmax(mx, std::log10((double)v[c]));
+        const int entry = 6 + (int)std::min(std::max(mx, -6.0), -1.0);
+        hist[entry] += 1;
+    }
+    double vmax[4];
+    const double vm = hist[0] > 1e3 ? 1.0e-5 : hist[1] > 1e3 ? 1.0e-4 : hist[2] > 1e3 ? 1.0e-3
+                    : hist[3] > 1e3 ? 1.0e-2 : hist[4] > 1e3 ? 1.0e-1 : 1.0;
+    for (int c = 0; c < 4; c++) vmax[c] = vm;
+    auto selected = [&](size_t k) {
+        const float* m = mean4 + 4 * k;
+        const float* v = var4 + 4 * k;
+        if (nan4(m) || nan4(v)) return false;
+        for (int c = 0; c < 4; c++)
+            if (!((double)m[c] >= 0.001 && (double)m[c] <= 0.9 && (double)v[c] <= vmax[c])) return false;
+        return true;
+    };
+    auto floor_max = [](double x, double f) { return std::isnan(x) ? f : std::max(x, f); };
+    double sx[4] = {0}, sy[4] = {0}, sxx[4] = {0}, sxy[4] = {0}, N1 = 0;
+    for (size_t k = 0; k < n; k++) {
+        if (!selected(k)) continue;
+        for (int c = 0; c < 4; c++) {
+            const double m = mean4[4 * k + c], v = var4[4 * k + c];
+            sx[c] += m;
+            sy[c] += v;
+            sxx[c] += m * m;
+            sxy[c] += m * v;
+        }
+        N1 += 1;
+    }
+    double A[4], B[4];
+    for (int c = 0; c < 4; c++) {
+        B[c] = floor_max((N1 * sxy[c] - sx[c] * sy[c]) / (N1 * sxx[c] - sx[c] * sx[c]), 1e-8);
+        A[c] = floor_max((sy[c] - B[c] * sx[c]) / N1, 1e-8);
+    }
+    double err2[4] = {0, 0, 0, 0};
+    for (size_t k = 0; k < n; k++) {
+        if (!selected(k)) continue;
+        for (int c = 0; c < 4; c++) {
+            const double d = A[c] + B[c] * mean4[4 * k + c] - var4[4 * k + c];
+            err2[c] += d * d;
+        }
+    }
+    for (int c = 0; c < 4; c++) err2[c] /= N1;
+    const double vmax1 = vm;
+    for (int c = 0; c < 4; c++) vmax[c] = B[c];
+    double tx[4] = {0}, ty[4] = {0}, txx[4] = {0}, txy[4] = {0}, e2n[4] = {0}, N2 = 0;
+    for (size_t k = 0; k < n; k++) {
+        if (!selected(k)) continue;
+        double dsq[4];
+        bool fits = true;
+        for (int c = 0; c < 4; c++) {
+            const double d = std::fabs(A[c] + B[c] * mean4[4 * k + c] - var4[4 * k + c]);
+            dsq[c] = d * d;
+            fits = fits && dsq[c] <= 0.5 * err2[c];
+        }
+        if (!fits) continue;
+        for (int c = 0; c < 4; c++) {
+            const double m = mean4[4 * k + c], v = var4[4 * k + c];
+            tx[c] += m;
+            ty[c] += v;
+            txx[c] += m * m;
+            txy[c] += m * v;
+            e2n[c] += dsq[c];
+        }
+        N2 += 1;
+    }
+    bool better = N2 > 0.001 * npx;
+    for (int c = 0; c < 4; c++) {
+        e2n[c] /= N2;
+        better = better && !std::isnan(e2n[c]) && e2n[c] < err2[c];
+    }
+    if (better) {
+        for (int c = 0; c < 4; c++) {
+            err2[c] = e2n[c];
+            B[c] = floor_max((N2 * txy[c] - tx[c] * ty[c]) / (N2 * txx[c] - tx[c] * tx[c]), 1e-8);
+            A[c] = floor_max((ty[c] - B[c] * tx[c]) / N2, 1e-8);
+        }
+    }
+    const double adj = exposure_multiplier * exposure_multiplier;
+    for (int c = 0; c < 4; c++) {
+        a4[c] = A[c] * adj;
+        b4[c] = B[c] * adj;
+    }
+    if (diag) {
+        diag[0] = N1;
+        diag[1] = N2;
+        diag[2] = better ? 1.0 : 0.0;
+        diag[3] = vmax1;
+        for (int c = 0; c < 4; c++) diag[4 + c] = err2[c];
+    }
 }
 
 }  // extern "C"

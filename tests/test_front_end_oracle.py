@@ -273,3 +273,71 @@ def test_despeckle_with_a_negative_variance_clamps_to_the_second_extremes():
     lum = np.sort(np.concatenate([img[..., 0].ravel(), [0.0, 0.0]]))
     lmin = np.sort(img[..., 0].ravel())
     assert out[1, 1, 0] == np.float32(min(max(-0.05, lmin[1]), lum[-2]))
+
+
+@pytest.mark.parametrize("bayer", [0, 2])
+def test_raw_noise_stats_closed_form_on_a_solid_mosaic(bayer):
+    """The statistics window steps one pixel at a time, so a channel's 81 reads
+    are its own colour where both offsets are even (25), the horizontally and
+    vertically adjacent colours where one is odd (20 + 20) and the diagonal
+    colour where both are odd (16)."""
+    col = np.array([0.6, 0.3, 0.15, 0.33], np.float32)  # red, green, blue, green2
+    solid = np.empty((24, 32, 3), np.float32)
+    solid[...] = (col[0], col[1], col[2])
+    raw = mosaic(solid, bayer)
+    offs = [((1, 0), (0, 0), (0, 1), (1, 1)), ((0, 1), (0, 0), (1, 0), (1, 1)),
+            ((0, 0), (0, 1), (1, 1), (1, 0)), ((1, 1), (0, 1), (0, 0), (1, 0))][bayer]
+    g2 = offs[3]
+    raw[g2[1]::2, g2[0]::2] = col[3]  # second green differs
+    mean, var = oracle.raw_noise_stats(raw, bayer)
+    inner = mean[3:-3, 3:-3], var[3:-3, 3:-3]
+    colour_at = {}
+    for c, (ox, oy) in enumerate(offs):
+        colour_at[(ox, oy)] = col[c]
+    for c, (ox, oy) in enumerate(offs):
+        vals = np.array([colour_at[((ox + x) % 2, (oy + y) % 2)] for y in range(-4, 5) for x in range(-4, 5)], np.float64)
+        np.testing.assert_allclose(inner[0][..., c], vals.mean(), rtol=1e-6)
+        np.testing.assert_allclose(inner[1][..., c], vals.var(), rtol=1e-3, atol=1e-7)
+
+
+def linear_stats(n, A, B, rng, lo=0.01, hi=0.8):
+    m = rng.uniform(lo, hi, (n, 1, 4)).astype(np.float32)
+    v = (np.asarray(A, np.float32) + np.asarray(B, np.float32) * m).astype(np.float32)
+    return m, v
+
+
+def test_raw_nlf_fit_recovers_an_exact_linear_model():
+    rng = np.random.default_rng(21)
+    A, B = np.array([2e-6, 1e-6, 3e-6, 1.5e-6]), np.array([4e-4, 3e-4, 5e-4, 3.2e-4])
+    m, v = linear_stats(5000, A, B, rng)
+    fit = oracle.fit_raw_nlf(m, v, 100, 200)
+    np.testing.assert_allclose(fit.b, v[..., :].reshape(-1, 4).astype(np.float64).std(0) / m.reshape(-1, 4).std(0),
+                               rtol=1e-4)  # slope of the data itself (float32-rounded)
+    np.testing.assert_allclose(fit.b, B, rtol=1e-5)
+    np.testing.assert_allclose(fit.a, A, rtol=2e-3, atol=1e-9)
+    # largest variances ~4e-4: log10 -3.4 truncates to -3 (the reference's (int)), bin 3 -> ceiling 1e-2
+    assert fit.n1 == 5000 and fit.variance_max == 1e-2
+    fit2 = oracle.fit_raw_nlf(m, v, 100, 200, exposure_multiplier=2.0)
+    np.testing.assert_allclose(fit2.a, 4 * fit.a, rtol=1e-12)
+    np.testing.assert_allclose(fit2.b, 4 * fit.b, rtol=1e-12)
+
+
+def test_raw_nlf_fit_ceiling_and_refit():
+    """Quads above the histogram's variance ceiling are ignored; quads far from
+    the first fit are dropped by the refit, which is then accepted."""
+    rng = np.random.default_rng(22)
+    A, B = np.array([2e-6] * 4), np.array([4e-4] * 4)
+    m, v = linear_stats(6000, A, B, rng)
+    v = v * (1 + 0.02 * rng.standard_normal(v.shape)).astype(np.float32)  # scatter around the line
+    out_m, out_v = linear_stats(300, A, B, rng)
+    out_v = out_v * 2.5                                                     # outliers below the ceiling
+    hi_m, hi_v = linear_stats(200, A, B, rng)
+    hi_v = hi_v + np.float32(0.05)                                          # above the 1e-2 ceiling
+    M = np.concatenate([m, out_m, hi_m]).astype(np.float32)
+    V = np.concatenate([v, out_v, hi_v]).astype(np.float32)
+    fit = oracle.fit_raw_nlf(M, V, 200, 200)
+    assert fit.n1 == 6300           # the 200 above the ceiling are never used
+    assert fit.accepted and fit.n2 < 6000
+    np.testing.assert_allclose(fit.b, B, rtol=0.05)
+    plain = oracle.fit_raw_nlf(np.concatenate([m, out_m]), np.concatenate([v, out_v]), 200, 200)
+    np.testing.assert_allclose(plain.b, fit.b, rtol=1e-12)   # identical: the ceiling excluded them exactly

@@ -171,3 +171,48 @@ def test_front_end_argument_errors():
     with pytest.raises(gcam.GcamError):  # a blur radius beyond the tile apron
         gcam.gaussianBlurSobelImage(torch.zeros((32, 64), device="cuda"), None, (1e-5, 1e-3), radius2=8.0)
     fe.close()
+
+
+@pytest.mark.parametrize("bayer", [0, 1, 2, 3])
+def test_raw_noise_statistics_are_bit_exact(bayer):
+    _, raw = raw_frame(258, 130, 61 + bayer)
+    scaled = oracle.scale_raw(raw, bayer, [SCALE_12BIT] * 4)
+    mean, var = gcam.rawNoiseStatistics(dev(scaled), bayer)
+    m_o, v_o = oracle.raw_noise_stats(scaled, bayer)
+    np.testing.assert_array_equal(mean.cpu().numpy(), m_o)
+    np.testing.assert_array_equal(var.cpu().numpy(), v_o)
+
+
+def test_measure_raw_nlf_matches_the_oracle():
+    """The FP64 fit on the device (block sums in a fixed order) against the
+    oracle's sequential FP64 sums: the same quads are selected, so the models
+    agree to the rounding of the sums."""
+    w, h = 1024, 768
+    _, raw = raw_frame(w, h, 71, a=6e-3, b=3e-3)
+    scaled = oracle.scale_raw(raw, 2, [SCALE_12BIT * g for g in (1.9, 1.0, 1.5, 1.0)])
+    a, b = gcam.measureRawNLF(dev(scaled), 2, exposure_multiplier=1.5)
+    fit = oracle.measure_raw_nlf(scaled, 2, exposure_multiplier=1.5)
+    print("device", a, b, "oracle", fit.a, fit.b, "n1", fit.n1, "n2", fit.n2, "accepted", fit.accepted)
+    np.testing.assert_allclose(np.array(a), fit.a, rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(np.array(b), fit.b, rtol=1e-5)
+
+
+def test_front_end_calibrating_the_raw_model():
+    w, h = 512, 384
+    raw_d, raw = raw_frame(w, h, 81)
+    p = params_for()
+    p.calibrateRawNlf = True
+    fe = gcam.RawFrontEnd(w, h)
+    ycc, grad = fe.run(raw_d, p)
+    fitted = fe.rawNlf
+    scaled = gcam.scaleRawData(raw_d, p.bayerPattern, p.scale_mul, p.black_level, p.lensShadingCorrection)
+    assert fitted == gcam.measureRawNLF(scaled, p.bayerPattern)
+    q = params_for()
+    q.rawNlf = fitted
+    ycc2, grad2 = fe.run(raw_d, q)
+    assert fe.rawNlf == fitted
+    assert torch.equal(grad, grad2) and torch.equal(ycc, ycc2)
+    fit = oracle.measure_raw_nlf(oracle.scale_raw(raw, p.bayerPattern, p.scale_mul, p.black_level,
+                                                  p.lensShadingCorrection), p.bayerPattern)
+    np.testing.assert_allclose(np.array(fitted[1]), fit.b, rtol=1e-5)
+    fe.close()
