@@ -41,6 +41,7 @@ EXPORTED = (
     "gcam_transform_image", "gcam_despeckle_image", "gcam_cam_ycbcr", "gcam_matrix3_inverse",
     "gcam_front_end_create", "gcam_front_end_destroy", "gcam_front_end_run",
     "gcam_raw_noise_statistics", "gcam_measure_raw_nlf", "gcam_front_end_raw_nlf",
+    "gcam_histogram_reset", "gcam_histogram_image", "gcam_histogram_statistics",
 )
 GCAM_MAX_LEVELS = 8
 GCAM_MAX_BURST = 64
@@ -159,6 +160,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.gcam_raw_noise_statistics.argtypes = [_IP, _I, _IP, _IP, _P]
     L.gcam_measure_raw_nlf.argtypes = [_IP, _I, _F, _FP, _FP, _P]
     L.gcam_front_end_raw_nlf.argtypes = [_P, _FP, _FP]
+    L.gcam_histogram_reset.argtypes = [_P, _P]
+    L.gcam_histogram_image.argtypes = [_IP, _P, _P]
+    L.gcam_histogram_statistics.argtypes = [_P, _I, _I, _P]
     for name in EXPORTED:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int:  # default restype: gcam_status

@@ -13,6 +13,7 @@
 //   despeckleLumaMedianChromaImage MetalPipeline/demosaic.metal:1160-1283
 //   basicRawNoiseStatistics       MetalPipeline/demosaic.metal:1096-1136
 //   MeasureRawNLF                 MetalPipeline/raw_converter.cpp:339-490
+//   histogramImage, histogramStatistics MetalPipeline/demosaic.metal:1357-1453
 //   gaussianKernelBilinearWeights MetalPipeline/demosaic_utils.cpp:939-1006
 //   cam_ycbcr                     MetalPipeline/demosaic_utils.cpp:703-720
 //   orchestration                 MetalPipeline/raw_converter.cpp:148-214, 70-78
@@ -689,6 +690,72 @@ __global__ __launch_bounds__(256) void k_raw_nlf_finalize(const double* partials
     }
 }
 
+
+// ---- histogramImage + histogramStatistics ---------------------------------
+
+// Each workgroup counts its pixels in an LDS histogram, then adds its nonzero
+// bins to the global one (integer counts: exact in any order).
+__global__ __launch_bounds__(256) void k_histogram(View<const float4> in, uint32_t* hist) {
+    __shared__ uint32_t lh[256];
+    const int tid = threadIdx.x;
+    lh[tid] = 0;
+    __syncthreads();
+    const size_t n = (size_t)in.w * in.h;
+    for (size_t k = (size_t)blockIdx.x * 256 + tid; k < n; k += (size_t)gridDim.x * 256) {
+        const float y = in.at((int)(k % in.w), (int)(k / in.w)).x;
+        const float v = sqrtf(y / 2.0f) * 255.0f;
+        const int idx = v >= 0.0f ? min((int)v, 255) : 0;  // NaN (negative luma): bin 0
+        atomicAdd(&lh[idx], 1u);
+    }
+    __syncthreads();
+    if (lh[tid]) atomicAdd(&hist[tid], lh[tid]);
+}
+
+// The reference's single-thread pass over the bins (demosaic.metal:1386-1453).
+__global__ void k_histogram_statistics(gcam_histogram_data* d, uint32_t image_size) {
+    if (threadIdx.x != 0) return;
+    float mean = 0.0f;
+    bool found_median = false, found_black = false, found_white = false;
+    int median_index = 0, black_index = 0, white_index = 0;
+    uint32_t sum = 0;
+    for (int band = 0; band < 8; band++) {
+        for (int i = 0; i < 32; i++) {
+            const int index = band * 32 + i;
+            const uint32_t entry = d->histogram[index];
+            d->bands[band] += entry;
+            mean += (float)(entry * (uint32_t)(index + 1)) / 256.0f;
+            sum += entry;
+            if (!found_median && sum >= (image_size + 1) / 2) {
+                median_index = index;
+                found_median = true;
+            }
+            if (!found_black && sum >= 0.001 * image_size) {
+                black_index = index;
+                found_black = true;
+            }
+            if (!found_white && sum >= 0.999 * image_size) {
+                white_index = index;
+                found_white = true;
+            }
+        }
+    }
+    mean /= (float)image_size;
+    const float vb = (float)(black_index - 1) / 255.0f, vw = (float)white_index / 255.0f;
+    d->black_level = 2.0f * vb * vb;
+    d->white_level = 2.0f * vw * vw;
+    d->median = (float)median_index / 255.0f - d->black_level;
+    d->mean = mean - d->black_level;
+    d->highlights = 1.0f + 2.0f * smoothstepf(0.01f, 0.1f, (float)(d->bands[5] + d->bands[6] + d->bands[7]) /
+                                                             (float)image_size);
+    d->shadows = 0.8f - 0.5f * smoothstepf(0.25f, 0.5f, (float)(d->bands[0] + d->bands[1]) / (float)image_size);
+}
+
+// Zeroes a histogram_data with a kernel (histogramImageKernel::reset).
+__global__ void k_histogram_reset(gcam_histogram_data* d) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(d);
+    for (int i = threadIdx.x; i < (int)(sizeof(gcam_histogram_data) / 4); i += blockDim.x) w[i] = 0u;
+}
+
 // ---- host side -----------------------------------------------------------
 
 // gaussianKernelBilinearWeights (demosaic_utils.cpp:987-1006) with
@@ -1011,6 +1078,27 @@ gcam_status gcam_measure_raw_nlf(const gcam_image* scaled, int32_t bayer, float 
         raw_nlf_b[c] = (float)host[4 + c];
     }
     return GCAM_OK;
+}
+
+
+gcam_status gcam_histogram_reset(gcam_histogram_data* data, void* stream) {
+    if (!data) return bad("gcam_histogram_reset: null histogram");
+    k_histogram_reset<<<1, 256, 0, (hipStream_t)stream>>>(data);
+    return launch_status("histogramImageKernel::reset");
+}
+
+gcam_status gcam_histogram_image(const gcam_image* in, gcam_histogram_data* data, void* stream) {
+    if (!valid_image(in, GCAM_PIXEL_RGBA32F) || !data) return bad("gcam_histogram_image: RGBA32F image and a histogram");
+    const size_t n = (size_t)in->width * in->height;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 1024);
+    k_histogram<<<blocks, 256, 0, (hipStream_t)stream>>>(view<const float4>(*in), data->histogram);
+    return launch_status("histogramImage");
+}
+
+gcam_status gcam_histogram_statistics(gcam_histogram_data* data, int32_t width, int32_t height, void* stream) {
+    if (!data || width <= 0 || height <= 0) return bad("gcam_histogram_statistics: histogram and image size");
+    k_histogram_statistics<<<1, 64, 0, (hipStream_t)stream>>>(data, (uint32_t)width * (uint32_t)height);
+    return launch_status("histogramStatistics");
 }
 
 // ---- the front end as one call -------------------------------------------

@@ -178,6 +178,29 @@ def measureRawNLF(scaledRawImage: torch.Tensor, bayerPattern=2, exposure_multipl
     return tuple(a), tuple(b)
 
 
+HISTOGRAM_WORDS = 256 + 8 + 6  # histogramImageKernel::histogram_data: 256 + 8 uint32, 6 float
+
+
+def _histogram_dict(words: torch.Tensor) -> dict:
+    host = words.cpu().numpy()
+    f = host[264:].view(np.float32)
+    keys = ("black_level", "white_level", "shadows", "highlights", "mean", "median")
+    return dict(histogram=host[:256].view(np.uint32).copy(), bands=host[256:264].view(np.uint32).copy(),
+                **{k: float(v) for k, v in zip(keys, f)})
+
+
+def histogramImage(image: torch.Tensor, stream=None) -> dict:
+    """histogramImageKernel (demosaic_kernels.hpp:485-535): reset, histogramImage and
+    histogramStatistics on the device; returns histogram_data as a dict (synchronizes)."""
+    words = torch.empty(HISTOGRAM_WORDS, dtype=torch.int32, device=image.device)
+    ptr = ctypes.c_void_p(words.data_ptr())
+    s = ctypes.c_void_p(_stream(stream))
+    check(lib().gcam_histogram_reset(ptr, s))
+    check(lib().gcam_histogram_image(ctypes.byref(_img(image)), ptr, s))
+    check(lib().gcam_histogram_statistics(ptr, image.shape[1], image.shape[0], s))
+    return _histogram_dict(words)
+
+
 def cam_ycbcr(rgb_cam=np.eye(3)) -> np.ndarray:
     """cam_ycbcr(rgb_cam, xyz_sRGB) (demosaic_utils.cpp:703-720)."""
     out = (ctypes.c_float * 9)()
@@ -290,6 +313,9 @@ class RawConverter:
         ycc, grad = self.front.run(rawImage, params, stream=stream)
         den = self.pyramid.denoise(denoiseParameters, ycc, grad, pyramidNlf, exposure_multiplier,
                                    params.lensShadingCorrection, calibrateFromImage, stream=stream)
+        # RawConverter::denoise's histogram of the denoised level 3 (raw_converter.cpp:87-89)
+        hist_level = min(3, self.levels - 1)
+        self.histogramData = histogramImage(self.pyramid.denoisedImagePyramid[hist_level], stream=stream)
         back = inverse3(cam_ycbcr(np.asarray(params.rgb_cam, np.float32).reshape(3, 3)))
         return transformImage(den, back, stream=stream)
 

@@ -100,6 +100,32 @@ gcam_status gcam_raw_noise_statistics(const gcam_image* scaled, int32_t bayer_pa
 gcam_status gcam_measure_raw_nlf(const gcam_image* scaled, int32_t bayer_pattern, float exposure_multiplier,
                                  float raw_nlf_a[4], float raw_nlf_b[4], void* stream);
 
+/* histogramImageKernel::histogram_data (MetalPipeline/demosaic_kernels.hpp:494-503):
+ * the 256-bin luma histogram RawConverter::denoise takes of denoisedImagePyramid[3]
+ * (raw_converter.cpp:87-89) and its statistics.  Lives in device memory. */
+typedef struct gcam_histogram_data {
+    uint32_t histogram[256];
+    uint32_t bands[8];
+    float black_level;
+    float white_level;
+    float shadows;
+    float highlights;
+    float mean;
+    float median;
+} gcam_histogram_data;
+
+/* histogramImageKernel::reset (demosaic_kernels.hpp:507-509), on the stream. */
+gcam_status gcam_histogram_reset(gcam_histogram_data* data, void* stream);
+
+/* histogramImage (demosaic.metal:1373-1383): adds the image's luma, bin
+ * clamp((int)(255 sqrt(Y / 2)), 0, 255), to data->histogram (RGBA32F image). */
+gcam_status gcam_histogram_image(const gcam_image* image, gcam_histogram_data* data, void* stream);
+
+/* histogramStatistics (demosaic.metal:1385-1453): bands, black / white level,
+ * median, mean, highlights and shadows of data->histogram for an image of
+ * width x height pixels (bands accumulate, as in the reference). */
+gcam_status gcam_histogram_statistics(gcam_histogram_data* data, int32_t width, int32_t height, void* stream);
+
 /* cam_ycbcr(rgb_cam, xyz_sRGB) (MetalPipeline/demosaic_utils.cpp:703-720,
  * raw_converter.cpp:166): the camera-RGB -> YCbCr matrix, row-major.  Host only. */
 void gcam_cam_ycbcr(const float rgb_cam[9], float out[9]);

@@ -81,6 +81,8 @@ def _declare(L):
     L.oracle_despeckle.argtypes = [_fp, _i, _i, _fp, _fp, _fp]
     L.oracle_cam_ycbcr.argtypes = [_fp, _fp]
     L.oracle_raw_noise_stats.argtypes = [_fp, _i, _i, _i, _fp, _fp]
+    L.oracle_histogram.argtypes = [_fp, _i, _i, _fp]
+    L.oracle_histogram_statistics.argtypes = [_fp, _fp, _i, _i, _fp]
     L.oracle_fit_raw_nlf.argtypes = [_fp, _fp, _i, _i, _d, _d, _fp, _fp, _fp]
     L.oracle_set_threads.argtypes = [_i]
     L.oracle_get_threads.restype = _i
@@ -483,3 +485,18 @@ def raw_front_end(raw, bayer=2, scale_mul=(1.0, 1.0, 1.0, 1.0), black_level=0.0,
     if despeckle_nlf is None:
         return ycc, grad, ycc
     return despeckle(ycc, despeckle_nlf[0], despeckle_nlf[1]), grad, ycc
+
+
+def histogram(img):
+    """histogramImage + histogramStatistics, MetalPipeline/demosaic.metal:1357-1453, on a zeroed
+    histogram_data: returns dict(histogram[256], bands[8], black_level, white_level, shadows,
+    highlights, mean, median)."""
+    img = _f32(img)
+    h, w = img.shape[:2]
+    hist = np.zeros(256, np.uint32)
+    bands = np.zeros(8, np.uint32)
+    stats = np.zeros(6, np.float32)
+    lib().oracle_histogram(_p(img), w, h, _p(hist))
+    lib().oracle_histogram_statistics(_p(hist), _p(bands), w, h, _p(stats))
+    keys = ("black_level", "white_level", "shadows", "highlights", "mean", "median")
+    return dict(histogram=hist, bands=bands, **{k: float(v) for k, v in zip(keys, stats)})

@@ -630,4 +630,60 @@ void oracle_fit_raw_nlf(const float* mean4, const float* var4, int qw, int qh, d
     }
 }
 
+// MetalPipeline/demosaic.metal:1357-1384 (histogram_data, histogramImage):
+// the 256-bin histogram of index = clamp((int)(255 sqrt(Y / 2)), 0, 255) over
+// the image's luma (a NaN index, from negative luma, counts in bin 0).
+// Accumulates into hist (the caller zeroes it, like histogramImageKernel::reset).
+void oracle_histogram(const float* in4, int w, int h, uint32_t* hist) {
+    for (size_t e = 0; e < (size_t)w * h; e++) {
+        const float v = std::sqrt(in4[4 * e] / 2.0f) * 255.0f;
+        int idx = v >= 0.0f ? (int)v : 0;
+        hist[idx < 0 ? 0 : (idx > 255 ? 255 : idx)] += 1;
+    }
+}
+
+// MetalPipeline/demosaic.metal:1386-1453 (histogramStatistics), one thread:
+// bands (8 x 32 bins, added to bands), mean, the bins where the cumulative
+// count first reaches half, 0.1 % and 99.9 % of the pixels, and from them
+// black / white level, median, mean, highlights, shadows (FP32, in the
+// reference's order).  stats: black_level, white_level, shadows, highlights,
+// mean, median.
+void oracle_histogram_statistics(const uint32_t* hist, uint32_t* bands, int w, int h, float* stats) {
+    const uint32_t image_size = (uint32_t)w * (uint32_t)h;
+    float mean = 0.0f;
+    bool found_median = false, found_black = false, found_white = false;
+    int median_index = 0, black_index = 0, white_index = 0;
+    uint32_t sum = 0;
+    for (int band = 0; band < 8; band++) {
+        for (int i = 0; i < 32; i++) {
+            const int index = band * 32 + i;
+            const uint32_t entry = hist[index];
+            bands[band] += entry;
+            mean += (float)(entry * (uint32_t)(index + 1)) / 256.0f;
+            sum += entry;
+            if (!found_median && sum >= (image_size + 1) / 2) {
+                median_index = index;
+                found_median = true;
+            }
+            if (!found_black && sum >= 0.001 * image_size) {
+                black_index = index;
+                found_black = true;
+            }
+            if (!found_white && sum >= 0.999 * image_size) {
+                white_index = index;
+                found_white = true;
+            }
+        }
+    }
+    mean /= (float)image_size;
+    const float vb = (float)(black_index - 1) / 255.0f, vw = (float)white_index / 255.0f;
+    const float black = 2.0f * vb * vb, white = 2.0f * vw * vw;
+    stats[0] = black;
+    stats[1] = white;
+    stats[5] = (float)median_index / 255.0f - black;
+    stats[4] = mean - black;
+    stats[3] = 1.0f + 2.0f * smoothstepf(0.01f, 0.1f, (float)(bands[5] + bands[6] + bands[7]) / (float)image_size);
+    stats[2] = 0.8f - 0.5f * smoothstepf(0.25f, 0.5f, (float)(bands[0] + bands[1]) / (float)image_size);
+}
+
 }  // extern "C"

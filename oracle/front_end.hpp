@@ -28,6 +28,8 @@ void oracle_blend_highlights(const float* in4, int w, int h, float clip, float* 
 void oracle_transform(const float* in4, int w, int h, const float* m, float* out4);
 void oracle_despeckle(const float* in4, int w, int h, const float* var_a, const float* var_b, float* out4);
 void oracle_cam_ycbcr(const float* rgb_cam, float* out);
+void oracle_histogram(const float* in4, int w, int h, uint32_t* hist);
+void oracle_histogram_statistics(const uint32_t* hist, uint32_t* bands, int w, int h, float* stats);
 void oracle_raw_noise_stats(const float* raw, int w, int h, int bayer, float* mean4, float* var4);
 void oracle_fit_raw_nlf(const float* mean4, const float* var4, int qw, int qh, double exposure_multiplier,
                         double npx, double* a4, double* b4, double* diag);

@@ -341,3 +341,27 @@ def test_raw_nlf_fit_ceiling_and_refit():
     np.testing.assert_allclose(fit.b, B, rtol=0.05)
     plain = oracle.fit_raw_nlf(np.concatenate([m, out_m]), np.concatenate([v, out_v]), 200, 200)
     np.testing.assert_allclose(plain.b, fit.b, rtol=1e-12)   # identical: the ceiling excluded them exactly
+
+
+def test_histogram_bins_and_statistics():
+    rng = np.random.default_rng(30)
+    img = np.zeros((40, 50, 4), np.float32)
+    img[..., 0] = rng.uniform(-0.05, 2.5, (40, 50)).astype(np.float32)
+    img[0, 0, 0] = np.nan
+    d = oracle.histogram(img)
+    y = img[..., 0].ravel()
+    with np.errstate(invalid="ignore"):
+        v = np.sqrt(y / np.float32(2)) * np.float32(255)
+    idx = np.where(v >= 0, np.minimum(np.nan_to_num(v, nan=0).astype(np.int64), 255), 0)
+    np.testing.assert_array_equal(d["histogram"], np.bincount(idx, minlength=256))
+    np.testing.assert_array_equal(d["bands"], d["histogram"].reshape(8, 32).sum(1))
+    for k in (3, 100, 200):   # every pixel in bin k: median, black and white all at k
+        one = np.zeros((10, 20, 4), np.float32)
+        one[..., 0] = 2 * ((k + 0.5) / 255) ** 2
+        s = oracle.histogram(one)
+        assert s["histogram"][k] == 200
+        black = 2 * ((k - 1) / 255) ** 2
+        assert abs(s["black_level"] - black) < 1e-6 and abs(s["white_level"] - 2 * (k / 255) ** 2) < 1e-6
+        assert abs(s["median"] - (k / 255 - black)) < 1e-6 and abs(s["mean"] - ((k + 1) / 256 - black)) < 1e-6
+        assert s["highlights"] == (3.0 if k >= 160 else 1.0)
+        assert abs(s["shadows"] - (0.3 if k < 64 else 0.8)) < 1e-6

@@ -216,3 +216,29 @@ def test_front_end_calibrating_the_raw_model():
                                                   p.lensShadingCorrection), p.bayerPattern)
     np.testing.assert_allclose(np.array(fitted[1]), fit.b, rtol=1e-5)
     fe.close()
+
+
+def test_histogram_is_exact():
+    rng = np.random.default_rng(91)
+    img = np.zeros((378, 504, 4), np.float32)
+    img[..., 0] = rng.uniform(-0.02, 1.4, (378, 504)).astype(np.float32)
+    got = gcam.histogramImage(dev(img))
+    want = oracle.histogram(img)
+    np.testing.assert_array_equal(got["histogram"], want["histogram"])
+    np.testing.assert_array_equal(got["bands"], want["bands"])
+    for k in ("black_level", "white_level", "shadows", "highlights", "mean", "median"):
+        assert got[k] == want[k], k
+
+
+def test_raw_converter_histogram_of_level_3():
+    w, h = 512, 384
+    raw_d, _ = raw_frame(w, h, 95)
+    p = params_for()
+    nl = [(tuple(a), tuple(b)) for a, b in nlf_for(5)]
+    conv = gcam.RawConverter(w, h, levels=5, search_radius=4, precision="fp16")
+    conv.demosaic(raw_d, p, [gcam.DenoiseParameters() for _ in range(5)], list(nl))
+    lvl3 = conv.pyramid.denoisedImagePyramid[3].cpu().numpy()
+    want = oracle.histogram(lvl3)
+    np.testing.assert_array_equal(conv.histogramData["histogram"], want["histogram"])
+    assert conv.histogramData["median"] == want["median"]
+    conv.close()
