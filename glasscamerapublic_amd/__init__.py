@@ -14,7 +14,8 @@ from .pyramid import (DenoiseParameters, PyramidProcessor, basic_noise_statistic
 from .burst import BurstProcessor  # noqa: F401
 from .front_end import (FrontEndParameters, RawConverter, RawFrontEnd, blendHighlightsImage, cam_ycbcr,  # noqa: F401
                         demosaicImage, despeckleImage, gaussianBlurSobelImage, gaussianKernelBilinearWeights,
-                        getRawVariance, histogramImage, inverse3, measureRawNLF, rawImageSobel, rawNoiseStatistics, scaleRawData,
+                        getRawVariance, histogramImage, inverse3, LocalToneMapping, LTMParameters,
+                        guidedFilterABImage, boxFilterGFImage, measureRawNLF, rawImageSobel, rawNoiseStatistics, scaleRawData,
                         transformImage)
 
 __all__ = [
@@ -23,5 +24,5 @@ __all__ = [
     "denoise_image", "downsample_image", "measure_ycbcr_nlf", "pca_projection", "subtract_noise_image",
     "synth_frame", "FrontEndParameters", "RawConverter", "RawFrontEnd", "blendHighlightsImage", "cam_ycbcr",
     "demosaicImage", "despeckleImage", "gaussianBlurSobelImage", "gaussianKernelBilinearWeights", "getRawVariance",
-    "histogramImage", "inverse3", "measureRawNLF", "rawImageSobel", "rawNoiseStatistics", "scaleRawData", "transformImage",
+    "histogramImage", "inverse3", "LocalToneMapping", "LTMParameters", "guidedFilterABImage", "boxFilterGFImage", "measureRawNLF", "rawImageSobel", "rawNoiseStatistics", "scaleRawData", "transformImage",
 ]

@@ -42,6 +42,7 @@ EXPORTED = (
     "gcam_front_end_create", "gcam_front_end_destroy", "gcam_front_end_run",
     "gcam_raw_noise_statistics", "gcam_measure_raw_nlf", "gcam_front_end_raw_nlf",
     "gcam_histogram_reset", "gcam_histogram_image", "gcam_histogram_statistics",
+    "gcam_guided_filter_ab_image", "gcam_box_filter_gf_image", "gcam_local_tone_mapping_mask_image",
 )
 GCAM_MAX_LEVELS = 8
 GCAM_MAX_BURST = 64
@@ -84,6 +85,11 @@ class SynthParamsC(ctypes.Structure):
 class BurstTuningC(ctypes.Structure):
     _fields_ = [("rows_per_thread", ctypes.c_int32 * 8), ("grid", ctypes.c_int32 * 8),
                 ("schedule", ctypes.c_int32 * 8), ("level_ms", ctypes.c_float * 8), ("tuned", ctypes.c_int32)]
+
+
+class LtmParamsC(ctypes.Structure):
+    _fields_ = [("eps", ctypes.c_float), ("shadows", ctypes.c_float), ("highlights", ctypes.c_float),
+                ("detail", ctypes.c_float * 3)]
 
 
 class FrontEndParamsC(ctypes.Structure):
@@ -163,6 +169,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.gcam_histogram_reset.argtypes = [_P, _P]
     L.gcam_histogram_image.argtypes = [_IP, _P, _P]
     L.gcam_histogram_statistics.argtypes = [_P, _I, _I, _P]
+    L.gcam_guided_filter_ab_image.argtypes = [_IP, _F, _IP, _P]
+    L.gcam_box_filter_gf_image.argtypes = [_IP, _IP, _P]
+    L.gcam_local_tone_mapping_mask_image.argtypes = [_IP, _IP, _IP, _IP, _IP, ctypes.POINTER(LtmParamsC), _F, _F,
+                                                     _P, _IP, _P]
     for name in EXPORTED:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int:  # default restype: gcam_status

@@ -14,6 +14,8 @@
 //   basicRawNoiseStatistics       MetalPipeline/demosaic.metal:1096-1136
 //   MeasureRawNLF                 MetalPipeline/raw_converter.cpp:339-490
 //   histogramImage, histogramStatistics MetalPipeline/demosaic.metal:1357-1453
+//   GuidedFilterABImage, BoxFilterGFImage, localToneMappingMaskImage
+//                                 MetalPipeline/demosaic.metal:1293-1355, 1456-1535
 //   gaussianKernelBilinearWeights MetalPipeline/demosaic_utils.cpp:939-1006
 //   cam_ycbcr                     MetalPipeline/demosaic_utils.cpp:703-720
 //   orchestration                 MetalPipeline/raw_converter.cpp:148-214, 70-78
@@ -756,6 +758,111 @@ __global__ void k_histogram_reset(gcam_histogram_data* d) {
     for (int i = threadIdx.x; i < (int)(sizeof(gcam_histogram_data) / 4); i += blockDim.x) w[i] = 0u;
 }
 
+
+// ---- local tone-mapping mask ----------------------------------------------
+
+// GuidedFilterABImage: the sampler reads the guide at texel centres, so the
+// 5x5 window is a plain clamped box of channel 0.
+__global__ __launch_bounds__(256) void k_guided_filter_ab(View<const float4> g, float eps, View<float2> ab) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= ab.w || y >= ab.h) return;
+    float sum = 0.0f, sq = 0.0f;
+    for (int j = -2; j <= 2; j++)
+        for (int i = -2; i <= 2; i++) {
+            const float v = fetch(g, x + i, y + j).x;
+            sum = sum + v;
+            sq = sq + v * v;
+        }
+    const float mean = sum / 25, var = (sq - sum * sum / 25) / 25;
+    const float a = var / (var + eps);
+    ab.at(x, y) = make_float2(a, mean * (1 - a));
+}
+
+// Bilinear sample of a 2-channel image at texel-space x0 + fx, y0 + fy.
+__device__ __forceinline__ float2 sample2(const View<const float2>& im, int x0, float fx, int y0, float fy) {
+    const float2 a = fetch(im, x0, y0), b = fetch(im, x0 + 1, y0), c = fetch(im, x0, y0 + 1), d = fetch(im, x0 + 1, y0 + 1);
+    const float gx = 1.0f - fx, gy = 1.0f - fy;
+    return make_float2((a.x * gx + b.x * fx) * gy + (c.x * gx + d.x * fx) * fy,
+                       (a.y * gx + b.y * fx) * gy + (c.y * gx + d.y * fx) * fy);
+}
+
+// BoxFilterGFImage: the 5x5 box as 9 bilinear taps (demosaic.metal:1329-1339).
+__global__ __launch_bounds__(256) void k_box_filter_gf(View<const float2> in, View<float2> out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    const float w[9] = {0.16f, 0.16f, 0.08f, 0.16f, 0.16f, 0.08f, 0.08f, 0.08f, 0.04f};
+    const int ix[9] = {-2, 0, 2, -2, 0, 2, -2, 0, 2}, iy[9] = {-2, -2, -2, 0, 0, 0, 2, 2, 2};
+    const float fx[9] = {0.5f, 0.5f, 0.0f, 0.5f, 0.5f, 0.0f, 0.5f, 0.5f, 0.0f};
+    const float fy[9] = {0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.0f, 0.0f, 0.0f};
+    float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        const float2 s2 = sample2(in, x + ix[t], fx[t], y + iy[t], fy[t]);
+        acc.x = acc.x + w[t] * s2.x;
+        acc.y = acc.y + w[t] * s2.y;
+    }
+    out.at(x, y) = acc;
+}
+
+// Centre of output pixel o (of `out` pixels) in the texel space of an image of
+// `in` pixels: ((2 o + 1) in - out) / (2 out) = i0 + frac, in integers.
+__device__ __forceinline__ void centre_map(int o, int in, int out, int& i0, float& frac) {
+    const long long num = (long long)(2 * o + 1) * in - out, den = 2LL * out;
+    long long q = num / den;
+    if (num % den != 0 && num < 0) q -= 1;
+    i0 = (int)q;
+    frac = (float)(num - q * den) / (float)den;
+}
+
+// computeLtmMultiplier (demosaic.metal:1456-1472).
+__device__ __forceinline__ float ltm_multiplier(float yv, float ill, float shadows, float highlights, float detail) {
+    const float mid = 0.22f;
+    const float reflectance = yv / ill;
+    const float adj = ill <= mid ? mid * powf(ill / mid, shadows)
+                                 : (1 - mid) * powf((ill - mid) / (1 - mid), highlights) + mid;
+    return adj * powf(reflectance, detail) / yv;
+}
+
+struct LtmBands {
+    View<const float2> b[3];  // box-filtered (a, b): low, mid, high frequency
+    float detail[3];
+};
+
+// localToneMappingMaskImage (demosaic.metal:1481-1535).
+__global__ __launch_bounds__(256) void k_ltm_mask(View<const float4> in, View<const float2> grad, LtmBands bands,
+                                                  float nlf_a, float nlf_b, const gcam_histogram_data* hist,
+                                                  View<float> mask) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= mask.w || y >= mask.h) return;
+    const float yv = in.at(x, y).x;
+    const float shadows = hist->shadows, highlights = hist->highlights;
+    float m = 1.0f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        int x0, y0;
+        float fx, fy;
+        centre_map(x, bands.b[k].w, mask.w, x0, fx);
+        centre_map(y, bands.b[k].h, mask.h, y0, fy);
+        const float2 ab = sample2(bands.b[k], x0, fx, y0, fy);
+        const float ill = ab.x * yv + ab.y;
+        float detail = bands.detail[k];
+        if (detail > 1.0f) {
+            float d = (1.0f - smoothstepf(0.75f, 0.95f, ill)) * smoothstepf(0.05f, 0.1f, ill);
+            if (k == 2) {
+                const float2 gg = grad.at(x, y);
+                const float g = sqrtf(gg.x * gg.x + gg.y * gg.y);
+                const float sigma = sqrtf(nlf_a + nlf_b * ill);
+                d = smoothstepf(2 * sigma, 8 * sigma, g) * (1 - smoothstepf(32 * sigma, 128 * sigma, g)) *
+                    (1.0f - smoothstepf(0.75f, 0.95f, ill)) * smoothstepf(0.05f, 0.1f, ill);
+            }
+            detail = 1 + (detail - 1) * d;
+        }
+        const float mk = ltm_multiplier(yv, ill, shadows, k == 0 ? highlights : 1.0f, detail);
+        m = k == 0 ? mk : m * mk;
+    }
+    mask.at(x, y) = m;
+}
+
 // ---- host side -----------------------------------------------------------
 
 // gaussianKernelBilinearWeights (demosaic_utils.cpp:987-1006) with
@@ -1099,6 +1206,45 @@ gcam_status gcam_histogram_statistics(gcam_histogram_data* data, int32_t width, 
     if (!data || width <= 0 || height <= 0) return bad("gcam_histogram_statistics: histogram and image size");
     k_histogram_statistics<<<1, 64, 0, (hipStream_t)stream>>>(data, (uint32_t)width * (uint32_t)height);
     return launch_status("histogramStatistics");
+}
+
+
+gcam_status gcam_guided_filter_ab_image(const gcam_image* guide, float eps, gcam_image* ab, void* stream) {
+    if (!valid_image(guide, GCAM_PIXEL_RGBA32F) || !valid_image(ab, GCAM_PIXEL_RG32F) || !same_size(guide, ab))
+        return bad("gcam_guided_filter_ab_image: RGBA32F guide and RG32F output of one size");
+    k_guided_filter_ab<<<grid_for(ab->width, ab->height, kBlock), kBlock, 0, (hipStream_t)stream>>>(
+        view<const float4>(*guide), eps, view<float2>(*ab));
+    return launch_status("GuidedFilterABImage");
+}
+
+gcam_status gcam_box_filter_gf_image(const gcam_image* in, gcam_image* out, void* stream) {
+    if (!valid_image(in, GCAM_PIXEL_RG32F) || !valid_image(out, GCAM_PIXEL_RG32F) || !same_size(in, out) ||
+        in->data == out->data)
+        return bad("gcam_box_filter_gf_image: two distinct RG32F images of one size");
+    k_box_filter_gf<<<grid_for(out->width, out->height, kBlock), kBlock, 0, (hipStream_t)stream>>>(
+        view<const float2>(*in), view<float2>(*out));
+    return launch_status("BoxFilterGFImage");
+}
+
+gcam_status gcam_local_tone_mapping_mask_image(const gcam_image* image, const gcam_image* gradient,
+                                               const gcam_image* ab_mean_lf, const gcam_image* ab_mean_mf,
+                                               const gcam_image* ab_mean_hf, const gcam_ltm_params* params,
+                                               float nlf_a, float nlf_b, const gcam_histogram_data* histogram,
+                                               gcam_image* mask, void* stream) {
+    if (!valid_image(image, GCAM_PIXEL_RGBA32F) || !valid_image(gradient, GCAM_PIXEL_RG32F) ||
+        !valid_image(mask, GCAM_PIXEL_R32F) || !same_size(image, gradient) || !same_size(image, mask) || !params ||
+        !histogram)
+        return bad("gcam_local_tone_mapping_mask_image: RGBA32F image, RG32F gradient, R32F mask of one size");
+    const gcam_image* b[3] = {ab_mean_lf, ab_mean_mf, ab_mean_hf};
+    LtmBands bands;
+    for (int k = 0; k < 3; k++) {
+        if (!valid_image(b[k], GCAM_PIXEL_RG32F)) return bad("gcam_local_tone_mapping_mask_image: RG32F band images");
+        bands.b[k] = view<const float2>(*b[k]);
+        bands.detail[k] = params->detail[k];
+    }
+    k_ltm_mask<<<grid_for(mask->width, mask->height, kBlock), kBlock, 0, (hipStream_t)stream>>>(
+        view<const float4>(*image), view<const float2>(*gradient), bands, nlf_a, nlf_b, histogram, view<float>(*mask));
+    return launch_status("localToneMappingMaskImage");
 }
 
 // ---- the front end as one call -------------------------------------------

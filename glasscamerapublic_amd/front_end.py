@@ -189,16 +189,83 @@ def _histogram_dict(words: torch.Tensor) -> dict:
                 **{k: float(v) for k, v in zip(keys, f)})
 
 
-def histogramImage(image: torch.Tensor, stream=None) -> dict:
-    """histogramImageKernel (demosaic_kernels.hpp:485-535): reset, histogramImage and
-    histogramStatistics on the device; returns histogram_data as a dict (synchronizes)."""
+def _histogram_device(image: torch.Tensor, stream=None) -> torch.Tensor:
+    """histogram_data of `image` in device memory (reset, histogramImage, histogramStatistics)."""
     words = torch.empty(HISTOGRAM_WORDS, dtype=torch.int32, device=image.device)
     ptr = ctypes.c_void_p(words.data_ptr())
     s = ctypes.c_void_p(_stream(stream))
     check(lib().gcam_histogram_reset(ptr, s))
     check(lib().gcam_histogram_image(ctypes.byref(_img(image)), ptr, s))
     check(lib().gcam_histogram_statistics(ptr, image.shape[1], image.shape[0], s))
-    return _histogram_dict(words)
+    return words
+
+
+def histogramImage(image: torch.Tensor, stream=None) -> dict:
+    """histogramImageKernel (demosaic_kernels.hpp:485-535): reset, histogramImage and
+    histogramStatistics on the device; returns histogram_data as a dict (synchronizes)."""
+    return _histogram_dict(_histogram_device(image, stream))
+
+
+@dataclass
+class LTMParameters:
+    """LTMParameters (demosaic.hpp:95-100)."""
+    eps: float = 0.01
+    shadows: float = 0.8
+    highlights: float = 1.05
+    detail: Sequence[float] = (1.1, 1.2, 1.3)
+
+    def to_c(self) -> "_lib.LtmParamsC":
+        return _lib.LtmParamsC(float(self.eps), float(self.shadows), float(self.highlights), _floats(self.detail, 3))
+
+
+def guidedFilterABImage(guideImage: torch.Tensor, eps: float, out: Optional[torch.Tensor] = None,
+                        stream=None) -> torch.Tensor:
+    """GuidedFilterABImage (demosaic.metal:1295-1323): (H, W, 2) of (a, b)."""
+    h, w = guideImage.shape[:2]
+    out = _like(guideImage, h, w, 2) if out is None else out
+    check(lib().gcam_guided_filter_ab_image(ctypes.byref(_img(guideImage)), float(eps), ctypes.byref(_img(out)),
+                                            ctypes.c_void_p(_stream(stream))))
+    return out
+
+
+def boxFilterGFImage(inputImage: torch.Tensor, out: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+    """BoxFilterGFImage (demosaic.metal:1325-1355): 5x5 box mean of an (H, W, 2) image."""
+    out = torch.empty_like(inputImage) if out is None else out
+    check(lib().gcam_box_filter_gf_image(ctypes.byref(_img(inputImage)), ctypes.byref(_img(out)),
+                                         ctypes.c_void_p(_stream(stream))))
+    return out
+
+
+class LocalToneMapping:
+    """LocalToneMapping (raw_converter.hpp:25-78): createMask from the denoised image, its
+    gradient, the guides denoisedImagePyramid[4], [2], [0], the level-0 luma noise model and the
+    histogram of denoisedImagePyramid[3]; getMask() returns the (H, W) multiplier mask."""
+
+    def __init__(self, width: int, height: int, device=None):
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        sizes = [(height // 16, width // 16), (height // 4, width // 4), (height, width)]
+        self.ab = [torch.empty((h, w, 2), dtype=torch.float32, device=dev) for h, w in sizes]
+        self.ab_mean = [torch.empty_like(t) for t in self.ab]
+        self.mask = torch.empty((height, width), dtype=torch.float32, device=dev)
+
+    def createMask(self, image: torch.Tensor, gradientImage: torch.Tensor, guideImage: Sequence[torch.Tensor],
+                   nlf: Tuple[float, float], ltmParameters: LTMParameters, histogram: torch.Tensor,
+                   stream=None) -> torch.Tensor:
+        for g, ab, m in zip(guideImage, self.ab, self.ab_mean):
+            if tuple(g.shape[:2]) != tuple(ab.shape[:2]):
+                raise ValueError(f"guide {tuple(g.shape[:2])} does not match its band {tuple(ab.shape[:2])}")
+            guidedFilterABImage(g, ltmParameters.eps, out=ab, stream=stream)
+            boxFilterGFImage(ab, out=m, stream=stream)
+        pc = ltmParameters.to_c()
+        check(lib().gcam_local_tone_mapping_mask_image(
+            ctypes.byref(_img(image)), ctypes.byref(_img(gradientImage)), ctypes.byref(_img(self.ab_mean[0])),
+            ctypes.byref(_img(self.ab_mean[1])), ctypes.byref(_img(self.ab_mean[2])), ctypes.byref(pc),
+            float(nlf[0]), float(nlf[1]), ctypes.c_void_p(histogram.data_ptr()), ctypes.byref(_img(self.mask)),
+            ctypes.c_void_p(_stream(stream))))
+        return self.mask
+
+    def getMask(self) -> torch.Tensor:
+        return self.mask
 
 
 def cam_ycbcr(rgb_cam=np.eye(3)) -> np.ndarray:
@@ -303,11 +370,16 @@ class RawConverter:
         self.pyramid = PyramidProcessor(width, height, levels=levels, search_radius=search_radius,
                                         precision=precision, device=device)
         self.levels = levels
+        self.ltm: Optional[LocalToneMapping] = None
+        self.histogramData: Optional[dict] = None
 
     def demosaic(self, rawImage: torch.Tensor, params: FrontEndParameters,
                  denoiseParameters: Sequence[DenoiseParameters], pyramidNlf, exposure_multiplier: float = 1.0,
-                 calibrateFromImage: bool = False, stream=None) -> torch.Tensor:
-        """Returns linear camera RGB (H, W, 4); pyramidNlf is updated in place when calibrating."""
+                 calibrateFromImage: bool = False, ltmParameters: Optional[LTMParameters] = None,
+                 stream=None) -> torch.Tensor:
+        """Returns linear camera RGB (H, W, 4); pyramidNlf is updated in place when calibrating.
+        histogramData holds the statistics of denoised level 3 and, with ltmParameters, ltm.getMask()
+        the local tone-mapping mask."""
         if params.despeckleNlf is None:
             params = FrontEndParameters(**{**params.__dict__, "despeckleNlf": pyramidNlf[0]})
         ycc, grad = self.front.run(rawImage, params, stream=stream)
@@ -315,10 +387,20 @@ class RawConverter:
                                    params.lensShadingCorrection, calibrateFromImage, stream=stream)
         # RawConverter::denoise's histogram of the denoised level 3 (raw_converter.cpp:87-89)
         hist_level = min(3, self.levels - 1)
-        self.histogramData = histogramImage(self.pyramid.denoisedImagePyramid[hist_level], stream=stream)
+        hist = _histogram_device(self.pyramid.denoisedImagePyramid[hist_level], stream)
+        if ltmParameters is not None:  # the local tone-mapping mask (raw_converter.cpp:104-112)
+            if self.levels != 5:
+                raise ValueError("the local tone-mapping mask uses pyramid levels 4, 2 and 0 (5 levels)")
+            if self.ltm is None:
+                self.ltm = LocalToneMapping(self.front.width, self.front.height, self.front.device)
+            d = self.pyramid.denoisedImagePyramid
+            self.ltm.createMask(den, grad, [d[4], d[2], d[0]], (pyramidNlf[0][0][0], pyramidNlf[0][1][0]),
+                                ltmParameters, hist, stream=stream)
+        self.histogramData = _histogram_dict(hist)
         back = inverse3(cam_ycbcr(np.asarray(params.rgb_cam, np.float32).reshape(3, 3)))
         return transformImage(den, back, stream=stream)
 
     def close(self) -> None:
         self.front.close()
         self.pyramid.close()
+        self.ltm = None

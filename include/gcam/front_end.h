@@ -126,6 +126,37 @@ gcam_status gcam_histogram_image(const gcam_image* image, gcam_histogram_data* d
  * width x height pixels (bands accumulate, as in the reference). */
 gcam_status gcam_histogram_statistics(gcam_histogram_data* data, int32_t width, int32_t height, void* stream);
 
+/* LTMParameters (MetalPipeline/demosaic.hpp:95-100); defaults {0.01, 0.8, 1.05,
+ * {1.1, 1.2, 1.3}}.  The mask reads shadows and highlights from the histogram. */
+typedef struct gcam_ltm_params {
+    float eps;
+    float shadows;
+    float highlights;
+    float detail[3];
+} gcam_ltm_params;
+
+/* GuidedFilterABImage (demosaic.metal:1295-1323): per guide pixel the guided
+ * filter's (a, b) = (var / (var + eps), mean (1 - a)) of the 5x5 luma box.
+ * guide RGBA32F, ab RG32F, one size. */
+gcam_status gcam_guided_filter_ab_image(const gcam_image* guide, float eps, gcam_image* ab, void* stream);
+
+/* BoxFilterGFImage (demosaic.metal:1325-1355): 5x5 box mean of an RG32F image. */
+gcam_status gcam_box_filter_gf_image(const gcam_image* in, gcam_image* out, void* stream);
+
+/* localToneMappingMaskImage (demosaic.metal:1481-1535), as driven by
+ * LocalToneMapping::createMask (raw_converter.hpp:61-75, demosaic_kernels.hpp:
+ * 565-600): the R32F mask for `image` (denoisedImagePyramid[0]) from the box
+ * means of the guided-filter (a, b) of denoised levels 4, 2 and 0 (ab_mean_lf
+ * at W/16, ab_mean_mf at W/4, ab_mean_hf at W), the gradient, the level-0
+ * luma noise model (nlf_a, nlf_b = pyramidNlf[0].first[0], .second[0]) and the
+ * shadows / highlights of a histogram after gcam_histogram_statistics
+ * (device memory). */
+gcam_status gcam_local_tone_mapping_mask_image(const gcam_image* image, const gcam_image* gradient,
+                                               const gcam_image* ab_mean_lf, const gcam_image* ab_mean_mf,
+                                               const gcam_image* ab_mean_hf, const gcam_ltm_params* params,
+                                               float nlf_a, float nlf_b, const gcam_histogram_data* histogram,
+                                               gcam_image* mask, void* stream);
+
 /* cam_ycbcr(rgb_cam, xyz_sRGB) (MetalPipeline/demosaic_utils.cpp:703-720,
  * raw_converter.cpp:166): the camera-RGB -> YCbCr matrix, row-major.  Host only. */
 void gcam_cam_ycbcr(const float rgb_cam[9], float out[9]);

@@ -82,6 +82,9 @@ def _declare(L):
     L.oracle_cam_ycbcr.argtypes = [_fp, _fp]
     L.oracle_raw_noise_stats.argtypes = [_fp, _i, _i, _i, _fp, _fp]
     L.oracle_histogram.argtypes = [_fp, _i, _i, _fp]
+    L.oracle_guided_filter_ab.argtypes = [_fp, _i, _i, _f, _fp]
+    L.oracle_box_filter_gf.argtypes = [_fp, _i, _i, _fp]
+    L.oracle_ltm_mask.argtypes = [_fp, _fp, _i, _i, _fp, _i, _i, _fp, _i, _i, _fp, _i, _i, _fp, _f, _f, _f, _f, _fp]
     L.oracle_histogram_statistics.argtypes = [_fp, _fp, _i, _i, _fp]
     L.oracle_fit_raw_nlf.argtypes = [_fp, _fp, _i, _i, _d, _d, _fp, _fp, _fp]
     L.oracle_set_threads.argtypes = [_i]
@@ -500,3 +503,41 @@ def histogram(img):
     lib().oracle_histogram_statistics(_p(hist), _p(bands), w, h, _p(stats))
     keys = ("black_level", "white_level", "shadows", "highlights", "mean", "median")
     return dict(histogram=hist, bands=bands, **{k: float(v) for k, v in zip(keys, stats)})
+
+
+LTM_DEFAULTS = dict(eps=0.01, shadows=0.8, highlights=1.05, detail=(1.1, 1.2, 1.3))  # demosaic.hpp:95-100
+
+
+def guided_filter_ab(guide, eps):
+    """GuidedFilterABImage, MetalPipeline/demosaic.metal:1295-1323: (H, W, 2) of (a, b)."""
+    guide = _f32(guide)
+    h, w = guide.shape[:2]
+    out = np.empty((h, w, 2), np.float32)
+    lib().oracle_guided_filter_ab(_p(guide), w, h, float(eps), _p(out))
+    return out
+
+
+def box_filter_gf(img2):
+    """BoxFilterGFImage, MetalPipeline/demosaic.metal:1325-1355."""
+    img2 = _f32(img2)
+    h, w = img2.shape[:2]
+    out = np.empty((h, w, 2), np.float32)
+    lib().oracle_box_filter_gf(_p(img2), w, h, _p(out))
+    return out
+
+
+def ltm_mask(image, gradient, guides, params=None, nlf=(1e-5, 1e-3), hist_shadows=0.8, hist_highlights=1.0):
+    """LocalToneMapping::createMask (MetalPipeline/raw_converter.hpp:61-75, demosaic_kernels.hpp:537-600):
+    guided-filter (a, b) of the three guides (denoised levels 4, 2, 0), their box means, and the
+    mask.  Returns (mask (H, W), [abMean lf, mf, hf])."""
+    p = dict(LTM_DEFAULTS, **(params or {}))
+    image, gradient = _f32(image), _f32(gradient)
+    h, w = image.shape[:2]
+    means = [box_filter_gf(guided_filter_ab(g, p["eps"])) for g in guides]
+    par = np.array([p["eps"], p["shadows"], p["highlights"], *p["detail"]], np.float32)
+    mask = np.empty((h, w), np.float32)
+    (lh, lw), (mh, mw), (hh, hw) = (m.shape[:2] for m in means)
+    lib().oracle_ltm_mask(_p(image), _p(gradient), w, h, _p(means[0]), lw, lh, _p(means[1]), mw, mh,
+                          _p(means[2]), hw, hh, _p(par), float(nlf[0]), float(nlf[1]), float(hist_shadows),
+                          float(hist_highlights), _p(mask))
+    return mask, means

@@ -686,4 +686,128 @@ void oracle_histogram_statistics(const uint32_t* hist, uint32_t* bands, int w, i
     stats[2] = 0.8f - 0.5f * smoothstepf(0.25f, 0.5f, (float)(bands[0] + bands[1]) / (float)image_size);
 }
 
+// MetalPipeline/demosaic.metal:1295-1323 (GuidedFilterABImage): per guide
+// pixel, the 5x5 box mean and population variance of channel 0 (the linear
+// sampler at texel centres reads the texels themselves, clamped), then
+// a = var / (var + eps), b = mean (1 - a).
+void oracle_guided_filter_ab(const float* guide4, int w, int h, float eps, float* ab2) {
+    const Img G{guide4, w, h, 4};
+    for (int y = 0; y < h; y++) {
+        for (int x = 0; x < w; x++) {
+            float sum = 0.0f, sq = 0.0f;
+            for (int j = -2; j <= 2; j++)
+                for (int i = -2; i <= 2; i++) {
+                    const float v = G.at(x + i, y + j, 0);
+                    sum = sum + v;
+                    sq = sq + v * v;
+                }
+            const float mean = sum / 25, var = (sq - sum * sum / 25) / 25;
+            const float a = var / (var + eps);
+            ab2[((size_t)y * w + x) * 2] = a;
+            ab2[((size_t)y * w + x) * 2 + 1] = mean * (1 - a);
+        }
+    }
+}
+
+namespace {
+// The 5x5 box as 9 bilinear taps {weight, offset} (demosaic.metal:1329-1339).
+const float kBox[9][3] = {{0.16f, -1.5f, -1.5f}, {0.16f, 0.5f, -1.5f}, {0.08f, 2.0f, -1.5f},
+                          {0.16f, -1.5f, 0.5f},  {0.16f, 0.5f, 0.5f},  {0.08f, 2.0f, 0.5f},
+                          {0.08f, -1.5f, 2.0f},  {0.08f, 0.5f, 2.0f},  {0.04f, 2.0f, 2.0f}};
+
+// Bilinear sample of a 2-channel image (iw x ih) at the texel-space position
+// x0 + fx, y0 + fy (texel i centred at i), clamped.
+void sample2(const float* img, int iw, int ih, int x0, float fx, int y0, float fy, float out[2]) {
+    const Img I{img, iw, ih, 2};
+    for (int c = 0; c < 2; c++) {
+        const float s0 = I.at(x0, y0, c) * (1.0f - fx) + I.at(x0 + 1, y0, c) * fx;
+        const float s1 = I.at(x0, y0 + 1, c) * (1.0f - fx) + I.at(x0 + 1, y0 + 1, c) * fx;
+        out[c] = s0 * (1.0f - fy) + s1 * fy;
+    }
+}
+
+// Texel-space position in an image of size `in` of the centre of pixel `o`
+// of an image of size `out`: ((2 o + 1) in - out) / (2 out) = i0 + frac, in
+// integers (exact).
+void centre_map(int o, int in, int out, int& i0, float& frac) {
+    const long long num = (long long)(2 * o + 1) * in - out, den = 2LL * out;
+    long long q = num / den;
+    if (num % den != 0 && num < 0) q -= 1;
+    i0 = (int)q;
+    frac = (float)(num - q * den) / (float)den;
+}
+
+// computeLtmMultiplier (demosaic.metal:1456-1472).
+float ltm_multiplier(float y, float ill, float shadows, float highlights, float detail) {
+    const float mid = 0.22f;
+    const float reflectance = y / ill;
+    const float adj = ill <= mid ? mid * std::pow(ill / mid, shadows)
+                                 : (1 - mid) * std::pow((ill - mid) / (1 - mid), highlights) + mid;
+    return adj * std::pow(reflectance, detail) / y;
+}
+}  // namespace
+
+// MetalPipeline/demosaic.metal:1325-1355 (BoxFilterGFImage): the 9 taps in
+// table order, sum of weight x sample.
+void oracle_box_filter_gf(const float* in2, int w, int h, float* out2) {
+    for (int y = 0; y < h; y++) {
+        for (int x = 0; x < w; x++) {
+            float acc[2] = {0.0f, 0.0f};
+            for (int t = 0; t < 9; t++) {
+                const float ox = kBox[t][1], oy = kBox[t][2];
+                const float fx0 = std::floor(ox), fy0 = std::floor(oy);
+                float s[2];
+                sample2(in2, w, h, x + (int)fx0, ox - fx0, y + (int)fy0, oy - fy0, s);
+                acc[0] = acc[0] + kBox[t][0] * s[0];
+                acc[1] = acc[1] + kBox[t][0] * s[1];
+            }
+            out2[((size_t)y * w + x) * 2] = acc[0];
+            out2[((size_t)y * w + x) * 2 + 1] = acc[1];
+        }
+    }
+}
+
+// MetalPipeline/demosaic.metal:1474-1535 (LTMParameters,
+// localToneMappingMaskImage): per pixel, the illuminance a Y + b from each
+// band's filtered (a, b), sampled bilinearly at the pixel centre; the
+// multipliers of the low band (shadows and highlights from the histogram),
+// the mid band and the high band (highlights 1), with the detail gains
+// faded by the ringing and noise guards; their product.  params: eps,
+// shadows, highlights, detail[3] (only detail enters here).
+void oracle_ltm_mask(const float* in4, const float* grad2, int w, int h, const float* lf2, int lw, int lh,
+                     const float* mf2, int mw, int mh, const float* hf2, int hw, int hh, const float* params,
+                     float nlf_a, float nlf_b, float hist_shadows, float hist_highlights, float* mask) {
+    const float* bands[3] = {lf2, mf2, hf2};
+    const int bw[3] = {lw, mw, hw}, bh[3] = {lh, mh, hh};
+    for (int y = 0; y < h; y++) {
+        for (int x = 0; x < w; x++) {
+            const float Y = in4[((size_t)y * w + x) * 4];
+            float m = 1.0f;
+            for (int k = 0; k < 3; k++) {
+                int x0, y0;
+                float fx, fy, ab[2];
+                centre_map(x, bw[k], w, x0, fx);
+                centre_map(y, bh[k], h, y0, fy);
+                sample2(bands[k], bw[k], bh[k], x0, fx, y0, fy, ab);
+                const float ill = ab[0] * Y + ab[1];
+                float detail = params[3 + k];
+                if (detail > 1.0f) {
+                    float d = (1.0f - smoothstepf(0.75f, 0.95f, ill)) * smoothstepf(0.05f, 0.1f, ill);
+                    if (k == 2) {
+                        const float gx = grad2[((size_t)y * w + x) * 2], gy = grad2[((size_t)y * w + x) * 2 + 1];
+                        const float g = std::sqrt(gx * gx + gy * gy);
+                        const float sigma = std::sqrt(nlf_a + nlf_b * ill);
+                        d = smoothstepf(2 * sigma, 8 * sigma, g) * (1 - smoothstepf(32 * sigma, 128 * sigma, g)) *
+                            (1.0f - smoothstepf(0.75f, 0.95f, ill)) * smoothstepf(0.05f, 0.1f, ill);
+                    }
+                    detail = 1 + (detail - 1) * d;
+                }
+                const float mk = ltm_multiplier(Y, ill, hist_shadows, k == 0 ? hist_highlights : 1.0f, detail);
+                m = k == 0 ? mk : m * mk;
+            }
+            mask[(size_t)y * w + x] = m;
+        }
+    }
+}
+
 }  // extern "C"

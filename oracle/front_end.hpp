@@ -30,6 +30,11 @@ void oracle_despeckle(const float* in4, int w, int h, const float* var_a, const 
 void oracle_cam_ycbcr(const float* rgb_cam, float* out);
 void oracle_histogram(const float* in4, int w, int h, uint32_t* hist);
 void oracle_histogram_statistics(const uint32_t* hist, uint32_t* bands, int w, int h, float* stats);
+void oracle_guided_filter_ab(const float* guide4, int w, int h, float eps, float* ab2);
+void oracle_box_filter_gf(const float* in2, int w, int h, float* out2);
+void oracle_ltm_mask(const float* in4, const float* grad2, int w, int h, const float* lf2, int lw, int lh,
+                     const float* mf2, int mw, int mh, const float* hf2, int hw, int hh, const float* params,
+                     float nlf_a, float nlf_b, float hist_shadows, float hist_highlights, float* mask);
 void oracle_raw_noise_stats(const float* raw, int w, int h, int bayer, float* mean4, float* var4);
 void oracle_fit_raw_nlf(const float* mean4, const float* var4, int qw, int qh, double exposure_multiplier,
                         double npx, double* a4, double* b4, double* diag);

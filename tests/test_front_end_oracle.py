@@ -365,3 +365,43 @@ def test_histogram_bins_and_statistics():
         assert abs(s["median"] - (k / 255 - black)) < 1e-6 and abs(s["mean"] - ((k + 1) / 256 - black)) < 1e-6
         assert s["highlights"] == (3.0 if k >= 160 else 1.0)
         assert abs(s["shadows"] - (0.3 if k < 64 else 0.8)) < 1e-6
+
+
+def box5(a):
+    """5x5 box mean with edge padding, channel-wise."""
+    p = np.pad(a.astype(np.float64), ((2, 2), (2, 2), (0, 0)), mode="edge")
+    h, w = a.shape[:2]
+    return sum(p[j:j + h, i:i + w] for j in range(5) for i in range(5)) / 25
+
+
+def test_guided_filter_and_box_filter():
+    rng = np.random.default_rng(40)
+    g = np.zeros((20, 28, 4), np.float32)
+    g[..., 0] = rng.uniform(0.05, 0.9, (20, 28))
+    ab = oracle.guided_filter_ab(g, 0.01)
+    y = g[..., :1]
+    mean, sq = box5(y)[..., 0], box5(y * y)[..., 0]
+    var = sq - mean * mean
+    a = var / (var + 0.01)
+    np.testing.assert_allclose(ab[..., 0], a, atol=2e-6)
+    np.testing.assert_allclose(ab[..., 1], mean * (1 - a), atol=2e-6)
+    two = rng.standard_normal((20, 28, 2)).astype(np.float32)
+    np.testing.assert_allclose(oracle.box_filter_gf(two), box5(two), atol=2e-6)
+
+
+@pytest.mark.parametrize("c", [0.12, 0.22, 0.5])
+def test_ltm_mask_of_a_flat_image(c):
+    """A flat image: zero variance, so a = 0 and b = the level, the illuminance
+    is c in every band and the reflectance 1; each band's multiplier is its
+    tone curve at c over c (mid and high bands: highlights 1)."""
+    h, w = 32, 48
+    img = np.zeros((h, w, 4), np.float32)
+    img[..., 0] = c
+    guides = [img[::16, ::16].copy(), img[::4, ::4].copy(), img]
+    mask, _ = oracle.ltm_mask(img, np.zeros((h, w, 2), np.float32), guides, hist_shadows=0.7, hist_highlights=1.3)
+
+    def curve(ill, shadows, highlights):
+        mid = 0.22
+        return mid * (ill / mid) ** shadows if ill <= mid else (1 - mid) * ((ill - mid) / (1 - mid)) ** highlights + mid
+    want = curve(c, 0.7, 1.3) / c * (curve(c, 0.7, 1.0) / c) ** 2
+    np.testing.assert_allclose(mask, want, rtol=2e-5)

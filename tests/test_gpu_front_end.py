@@ -242,3 +242,52 @@ def test_raw_converter_histogram_of_level_3():
     np.testing.assert_array_equal(conv.histogramData["histogram"], want["histogram"])
     assert conv.histogramData["median"] == want["median"]
     conv.close()
+
+
+def test_ltm_filters_bit_exact_and_mask():
+    rng = np.random.default_rng(101)
+    h, w = 128, 192
+    img = np.zeros((h, w, 4), np.float32)
+    img[..., 0] = (0.1 + 0.7 * rng.random((h, w))).astype(np.float32)
+    guides = [np.ascontiguousarray(img[::16, ::16]), np.ascontiguousarray(img[::4, ::4]), img]
+    ab = [gcam.guidedFilterABImage(dev(g), 0.01) for g in guides]
+    means = [gcam.boxFilterGFImage(a) for a in ab]
+    grad = (0.02 * rng.standard_normal((h, w, 2))).astype(np.float32)
+    mask_o, means_o = oracle.ltm_mask(img, grad, guides, nlf=(1e-5, 1e-3), hist_shadows=0.75, hist_highlights=1.4)
+    for a_d, g in zip(ab, guides):
+        np.testing.assert_array_equal(a_d.cpu().numpy(), oracle.guided_filter_ab(g, 0.01))
+    for m_d, m_o in zip(means, means_o):
+        np.testing.assert_array_equal(m_d.cpu().numpy(), m_o)
+    hist = torch.zeros(270, dtype=torch.int32, device="cuda")
+    hist.view(torch.float32)[264 + 2] = 0.75  # shadows
+    hist.view(torch.float32)[264 + 3] = 1.4   # highlights
+    ltm = gcam.LocalToneMapping(w, h)
+    ltm.ab_mean = means
+    import ctypes
+    from glasscamerapublic_amd._lib import check
+    from glasscamerapublic_amd.front_end import _img
+    pc = gcam.LTMParameters().to_c()
+    img_d, grad_d = dev(img), dev(grad)  # held until the launch (the image structs keep only pointers)
+    check(gcam.lib().gcam_local_tone_mapping_mask_image(
+        ctypes.byref(_img(img_d)), ctypes.byref(_img(grad_d)), ctypes.byref(_img(means[0])),
+        ctypes.byref(_img(means[1])), ctypes.byref(_img(means[2])), ctypes.byref(pc), 1e-5, 1e-3,
+        ctypes.c_void_p(hist.data_ptr()), ctypes.byref(_img(ltm.mask)), None))
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(ltm.mask.cpu().numpy(), mask_o, rtol=1e-5)
+
+
+def test_raw_converter_ltm_mask():
+    w, h = 512, 384
+    raw_d, _ = raw_frame(w, h, 111)
+    p = params_for()
+    nl = [(tuple(a), tuple(b)) for a, b in nlf_for(5)]
+    conv = gcam.RawConverter(w, h, levels=5, search_radius=4, precision="fp16")
+    conv.demosaic(raw_d, p, [gcam.DenoiseParameters() for _ in range(5)], list(nl),
+                  ltmParameters=gcam.LTMParameters())
+    d = [t.cpu().numpy() for t in conv.pyramid.denoisedImagePyramid]
+    grad = conv.front.run(raw_d, p)[1].cpu().numpy()
+    hd = conv.histogramData
+    mask_o, _ = oracle.ltm_mask(d[0], grad, [d[4], d[2], d[0]], nlf=(nl[0][0][0], nl[0][1][0]),
+                                hist_shadows=hd["shadows"], hist_highlights=hd["highlights"])
+    np.testing.assert_allclose(conv.ltm.getMask().cpu().numpy(), mask_o, rtol=1e-5)
+    conv.close()
