@@ -17,6 +17,8 @@ from .front_end import (FrontEndParameters, RawConverter, RawFrontEnd, blendHigh
                         getRawVariance, histogramImage, inverse3, LocalToneMapping, LTMParameters,
                         guidedFilterABImage, boxFilterGFImage, measureRawNLF, rawImageSobel, rawNoiseStatistics, scaleRawData,
                         transformImage)
+from .registration import (FindHomography, FindLeastSquaresHomography, ScaleHomography, fuseBurst,  # noqa: F401
+                           ransacSamples, registerAndFuse, registerBayerImage, registerImage)
 
 __all__ = [
     "GcamError", "lib", "DenoiseParameters", "PyramidProcessor", "BurstProcessor", "basic_noise_statistics",
@@ -25,4 +27,6 @@ __all__ = [
     "synth_frame", "FrontEndParameters", "RawConverter", "RawFrontEnd", "blendHighlightsImage", "cam_ycbcr",
     "demosaicImage", "despeckleImage", "gaussianBlurSobelImage", "gaussianKernelBilinearWeights", "getRawVariance",
     "histogramImage", "inverse3", "LocalToneMapping", "LTMParameters", "guidedFilterABImage", "boxFilterGFImage", "measureRawNLF", "rawImageSobel", "rawNoiseStatistics", "scaleRawData", "transformImage",
+    "FindHomography", "FindLeastSquaresHomography", "ScaleHomography", "fuseBurst", "ransacSamples",
+    "registerAndFuse", "registerBayerImage", "registerImage",
 ]

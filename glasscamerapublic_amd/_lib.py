@@ -43,6 +43,8 @@ EXPORTED = (
     "gcam_raw_noise_statistics", "gcam_measure_raw_nlf", "gcam_front_end_raw_nlf",
     "gcam_histogram_reset", "gcam_histogram_image", "gcam_histogram_statistics",
     "gcam_guided_filter_ab_image", "gcam_box_filter_gf_image", "gcam_local_tone_mapping_mask_image",
+    "gcam_register_and_fuse_image", "gcam_register_image", "gcam_register_bayer_image",
+    "gcam_find_least_squares_homography", "gcam_find_homography", "gcam_ransac_samples", "gcam_scale_homography",
 )
 GCAM_MAX_LEVELS = 8
 GCAM_MAX_BURST = 64
@@ -173,6 +175,15 @@ def _declare(L: ctypes.CDLL) -> None:
     L.gcam_box_filter_gf_image.argtypes = [_IP, _IP, _P]
     L.gcam_local_tone_mapping_mask_image.argtypes = [_IP, _IP, _IP, _IP, _IP, ctypes.POINTER(LtmParamsC), _F, _F,
                                                      _P, _IP, _P]
+    L.gcam_register_and_fuse_image.argtypes = [_IP, _IP, _IP, _FP, _I, _P]
+    L.gcam_register_image.argtypes = [_IP, _IP, _FP, _P]
+    L.gcam_register_bayer_image.argtypes = [_IP, _IP, _FP, _P]
+    L.gcam_find_least_squares_homography.argtypes = [_P, _I, _FP, _P]
+    L.gcam_find_homography.argtypes = [_P, _I, _F, _I, ctypes.c_uint64, _FP, ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32), _P]
+    L.gcam_ransac_samples.argtypes = [ctypes.c_uint64, _I, _I, _I, ctypes.POINTER(ctypes.c_int32)]
+    L.gcam_scale_homography.argtypes = [_FP, _F, _FP]
+    L.gcam_scale_homography.restype = None
     for name in EXPORTED:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int:  # default restype: gcam_status

@@ -87,6 +87,14 @@ def _declare(L):
     L.oracle_ltm_mask.argtypes = [_fp, _fp, _i, _i, _fp, _i, _i, _fp, _i, _i, _fp, _i, _i, _fp, _f, _f, _f, _f, _fp]
     L.oracle_histogram_statistics.argtypes = [_fp, _fp, _i, _i, _fp]
     L.oracle_fit_raw_nlf.argtypes = [_fp, _fp, _i, _i, _d, _d, _fp, _fp, _fp]
+    L.oracle_register_and_fuse.argtypes = [_fp, _fp, _i, _i, _i, _i, _fp, _i, _fp]
+    L.oracle_register_image.argtypes = [_fp, _i, _i, _i, _i, _i, _fp, _fp]
+    L.oracle_register_bayer.argtypes = [_fp, _i, _i, _fp, _fp]
+    L.oracle_jacobi9.argtypes = [_fp, _fp, _fp]
+    L.oracle_least_squares_homography.argtypes = [_fp, _i, _fp]
+    L.oracle_least_squares_homography.restype = _i
+    L.oracle_ransac_samples.argtypes = [ctypes.c_uint64, _i, _i, _i, _fp]
+    L.oracle_find_homography.argtypes = [_fp, _i, _f, _i, ctypes.c_uint64, _fp, _fp, _fp, _fp]
     L.oracle_set_threads.argtypes = [_i]
     L.oracle_get_threads.restype = _i
 
@@ -541,3 +549,87 @@ def ltm_mask(image, gradient, guides, params=None, nlf=(1e-5, 1e-3), hist_shadow
                           _p(means[2]), hw, hh, _p(par), float(nlf[0]), float(nlf[1]), float(hist_shadows),
                           float(hist_highlights), _p(mask))
     return mask, means
+
+
+# ---- burst registration and fusion (ImageRegistration/) ----
+
+def _h9(H):
+    return np.ascontiguousarray(np.asarray(H, np.float32).reshape(9))
+
+
+def register_and_fuse(fused, image, H, count):
+    """registerAndFuse, ImageRegistration/SURF.metal:493-516: ((count - 1) fused + image(H p)) / count
+    over fused's size ((H, W, 4) float32)."""
+    fused, image = _f32(fused), _f32(image)
+    oh, ow = fused.shape[:2]
+    ih, iw = image.shape[:2]
+    out = np.empty_like(fused)
+    lib().oracle_register_and_fuse(_p(fused), _p(image), ow, oh, iw, ih, _p(_h9(H)), int(count), _p(out))
+    return out
+
+
+def register_image(image, H):
+    """registerImage, ImageRegistration/SURF.metal:518-538 ((H, W) or (H, W, 4) float32)."""
+    image = _f32(image)
+    h, w = image.shape[:2]
+    ch = 1 if image.ndim == 2 else image.shape[2]
+    out = np.empty_like(image)
+    lib().oracle_register_image(_p(image), ch, w, h, w, h, _p(_h9(H)), _p(out))
+    return out
+
+
+def register_bayer(raw, H):
+    """registerBayerImage, ImageRegistration/SURF.metal:540-559 ((H, W) uint16)."""
+    raw = np.ascontiguousarray(raw, np.uint16)
+    h, w = raw.shape
+    out = np.empty_like(raw)
+    lib().oracle_register_bayer(_p(raw), w, h, _p(_h9(H)), _p(out))
+    return out
+
+
+def jacobi9(a):
+    """The 9 x 9 symmetric eigensolve of LeastSquaresHomography.cpp:216-349: (eigenvalues
+    descending, eigenvectors as rows)."""
+    a = np.ascontiguousarray(a, np.float64)
+    w = np.empty(9)
+    v = np.empty((9, 9))
+    lib().oracle_jacobi9(_p(a), _p(w), _p(v))
+    return w, v
+
+
+def least_squares_homography(matches):
+    """FindLeastSquaresHomography, LeastSquaresHomography.cpp:356-421, on (n, 4) matches
+    (x1, y1, x2, y2); None where the reference throws std::range_error."""
+    m = _f32(matches)
+    H = np.empty(9, np.float32)
+    if lib().oracle_least_squares_homography(_p(m), m.shape[0], _p(H)) != 0:
+        return None
+    return H.reshape(3, 3)
+
+
+def ransac_samples(seed, count, first, n):
+    out = np.empty((n, 4), np.int32)
+    lib().oracle_ransac_samples(int(seed), int(count), int(first), int(n), _p(out))
+    return out
+
+
+def find_homography(matches, threshold=1.0, max_iterations=2000, seed=0):
+    """FindHomography, ImageRegistration/Homography.cpp:72-116: (H, inlier indices, best
+    hypothesis, its outlier count)."""
+    m = _f32(matches)
+    n = m.shape[0]
+    H = np.empty(9, np.float32)
+    inl = np.empty(n, np.int32)
+    cnt = np.zeros(1, np.int32)
+    best = np.zeros(2, np.int32)
+    lib().oracle_find_homography(_p(m), n, float(threshold), int(max_iterations), int(seed), _p(H), _p(inl),
+                                 _p(cnt), _p(best))
+    return H.reshape(3, 3), inl[:cnt[0]].copy(), int(best[0]), int(best[1])
+
+
+def scale_homography(H, scale):
+    """ScaleHomography, ImageRegistration/Homography.hpp:29-37: S H S^-1, S = diag(scale, scale, 1)."""
+    H = np.asarray(H, np.float32).reshape(3, 3)
+    s = np.array([scale, scale, 1.0], np.float32)
+    inv = np.array([1.0 / np.float32(scale), 1.0 / np.float32(scale), 1.0], np.float32)
+    return ((s[:, None] * H) * inv[None, :]).astype(np.float32)
