@@ -18,7 +18,8 @@ cores on the same frame.
 
 The raw front end that produces the denoise's inputs from the frame's Bayer
 mosaic (RawFrontEnd) is timed beside it under "front_end" (--no-front-end to
-skip).
+skip), and burst fusion (registerAndFuse of the frame, FindHomography on 1500
+matches) under "registration" (--no-registration to skip).
 """
 from __future__ import annotations
 
@@ -171,6 +172,71 @@ def time_front_end(gcam, torch, raw, w, h, params, steps, warmup, cpu_baseline, 
     return out
 
 
+REGISTER_AND_FUSE_BYTES_PER_PX = 48  # fused RGBA32F read + warped input read (one texel per pixel) + write
+
+
+def time_registration(gcam, torch, ycc, w, h, steps, warmup, cpu_baseline):
+    """Burst fusion beside the denoise: registerAndFuse of the frame into a fused frame (in place,
+    as pyramidFusion.cpp:345 calls it) under a small rotation + translation, and FindHomography on
+    1500 matches (30% outliers) with the reference's threshold 1 and 2000 iterations."""
+    import numpy as np
+
+    stream = torch.cuda.current_stream()
+    c, s_ = np.cos(0.002), np.sin(0.002)
+    H = np.array([[c, -s_, 1.7], [s_, c, -2.3], [1e-7, -2e-7, 1.0]], np.float32)
+    fused = ycc.clone()
+    for _ in range(warmup):
+        gcam.registerAndFuse(fused, ycc, fused, H, 2)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        gcam.registerAndFuse(fused, ycc, fused, H, 2)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    gbs = REGISTER_AND_FUSE_BYTES_PER_PX * w * h / (ms / 1e3) / 1e9
+    rng = np.random.default_rng(77)
+    n = 1500
+    p = rng.uniform(0, w / 2, (n, 2))
+    ph = np.c_[p, np.ones(n)] @ H.astype(np.float64).T
+    q = ph[:, :2] / ph[:, 2:] + 0.05 * rng.standard_normal((n, 2))
+    bad = rng.random(n) < 0.3
+    q[bad] += rng.uniform(5, 50, (bad.sum(), 2))
+    m = np.c_[p, q].astype(np.float32)
+    md = torch.from_numpy(m).to(ycc.device)
+    for _ in range(2):
+        gcam.FindHomography(md, 1.0, 2000)
+    t = time.perf_counter()
+    fits = 10
+    for _ in range(fits):
+        gcam.FindHomography(md, 1.0, 2000)
+    fit_ms = (time.perf_counter() - t) / fits * 1e3
+    out = {"register_and_fuse": {
+               "ms_per_frame": round(ms, 4), "value": round(w * h / 1e6 / (ms / 1e3), 2), "unit": "Mpix/s",
+               "roofline": {"bound": "hbm", "bytes_per_px": REGISTER_AND_FUSE_BYTES_PER_PX, "achieved": round(gbs, 1),
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}},
+           "find_homography": {"ms_per_fit": round(fit_ms, 3), "matches": n, "iterations": 2000,
+                               "note": "host-inclusive: scratch allocation, 2 launches, result copy"}}
+    if cpu_baseline:
+        import oracle
+        cw, ch = min(w, 2048), min(h, 1536)
+        img = np.ascontiguousarray(ycc[:ch, :cw].cpu().numpy())
+        t = time.perf_counter()
+        oracle.register_and_fuse(img, img, H, 2)
+        cpu_s = time.perf_counter() - t
+        out["register_and_fuse"]["cpu_baseline"] = {
+            "value": round(cw * ch / 1e6 / cpu_s, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": f"{cw}x{ch} crop of the frame, FP32 C++ oracle, 1 thread, {cpu_s:.2f} s"}
+        t = time.perf_counter()
+        oracle.find_homography(m, 1.0, 2000, 0)
+        cpu_s = time.perf_counter() - t
+        out["find_homography"]["cpu_baseline"] = {
+            "ms_per_fit": round(cpu_s * 1e3, 1), "cores": 1, "kind": "port",
+            "sample": "the same 1500 matches, 2000 hypotheses, C++ oracle (reference algorithm), 1 thread"}
+    return out
+
+
 def single_frame(args) -> dict:
     import numpy as np
     import torch
@@ -318,6 +384,9 @@ def single_frame(args) -> dict:
         fe_params = front_end_params(gcam, a, b, nlf[0])
         out["front_end"] = time_front_end(gcam, torch, raw, w, h, fe_params, args.steps, args.warmup,
                                           not args.no_cpu_baseline, 1)
+    if not args.no_registration:
+        out["registration"] = time_registration(gcam, torch, ycc, w, h, args.steps, args.warmup,
+                                                not args.no_cpu_baseline)
     return out
 
 
@@ -424,6 +493,8 @@ def main() -> int:
     ap.add_argument("--no-compare", action="store_true", help="skip timing the FP32 form beside the reported one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-front-end", action="store_true", help="skip timing the raw front end beside the denoise")
+    ap.add_argument("--no-registration", action="store_true",
+                    help="skip timing burst registration (registerAndFuse, FindHomography) beside the denoise")
     ap.add_argument("--no-quality", action="store_true", help="skip the PSNR check against the FP64 oracle")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: min(cores, 16))")
     args = ap.parse_args()
