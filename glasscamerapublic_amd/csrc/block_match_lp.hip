@@ -27,6 +27,10 @@
 //    exponent is k2 (f_p.f_q - |f_q|^2/2) + kl |f_p|^2 + kl t0^2 -- exactly the
 //    squared distance of the rounded features up to FP32 rounding.  Weights
 //    and sums are FP32; reciprocals are the hardware v_rcp_f32 (1 ulp).
+// 4. The inner loop is bound by vector-ALU issue (a transcendental costs
+//    about two slots, so exp2 stays at the two weights): component 0 enters
+//    the exponent as -(r f0q - r f0p)^2 with r = sqrt(-kl), one FMA for the
+//    difference instead of a subtraction and a product.
 #include "gcam_internal.hpp"
 #include "burst_tables.hpp"
 
@@ -196,7 +200,7 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
     const int x = x0 + threadIdx.x, yb = y0 + threadIdx.y * PV;
     if (x < out.w && yb < out.h) {
         h8 fp[PV];
-        float f0p[PV], kl[PV], k2[PV], cp[PV], sb[PV], sr[PV], sbc[PV], src[PV];
+        float r1[PV], pr[PV], kl[PV], k2[PV], cp[PV], sb[PV], sr[PV], sbc[PV], src[PV];
         float fY[PV], fCb[PV], fCr[PV], nY[PV], nC[PV];
         const float kw = -2.0f * kLog2e, kroot = sqrtf(2.0f * kLog2e);
         const float rcb = __builtin_amdgcn_rcpf(p.chromaBoost), rscale = sref > 0.0f ? sref : 1.0f;
@@ -206,7 +210,7 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
             const int c = (threadIdx.y * PV + k + R) * T::FW + threadIdx.x + R;
             const float4 v = ycn[c];
             fp[k] = fh[c];
-            f0p[k] = f0s[c];
+            const float f0p = f0s[c];
             float lens = 1.0f;
             if (p.lensShading > 0.0f) {
                 const float cx = (float)(in.w / 2), cy = (float)(in.h / 2);
@@ -229,6 +233,8 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
             kl[k] = -2.0f * kLog2e * lm * lm;  // per unit of scaled squared distance
             k2[k] = -2.0f * kl[k];
             cp[k] = k2[k] * v.w;               // kl |f_p|^2 (v.w = -|f_p|^2 / 2)
+            r1[k] = sqrtf(-kl[k]);             // kl t0^2 = -(r1 t0)^2
+            pr[k] = -f0p * r1[k];
             sb[k] = kroot * (dm1 * rcb);
             sr[k] = kroot * (dm2 * rcb);
             sbc[k] = sb[k] * v.y;
@@ -257,8 +263,8 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
                     acc = P::dot2(h2{fq[2], fq[3]}, h2{fp[k][2], fp[k][3]}, acc);
                     acc = P::dot2(h2{fq[4], fq[5]}, h2{fp[k][4], fp[k][5]}, acc);
                     acc = P::dot2(h2{fq[6], fq[7]}, h2{fp[k][6], fp[k][7]}, acc);
-                    const float t0 = f0q - f0p[k];
-                    const float wl = __builtin_amdgcn_exp2f(fmaf(k2[k], acc, fmaf(kl[k] * t0, t0, cp[k])));
+                    const float t0 = fmaf(f0q, r1[k], pr[k]);
+                    const float wl = __builtin_amdgcn_exp2f(fmaf(k2[k], acc, fmaf(-t0, t0, cp[k])));
                     const float wY = wd * wl;
                     const float tb = fmaf(sb[k], qy.y, -sbc[k]), tr = fmaf(sr[k], qy.z, -src[k]);
                     float e = fmaf(-tb, tb, lwd);
