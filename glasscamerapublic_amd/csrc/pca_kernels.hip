@@ -406,10 +406,287 @@ __device__ void jacobi_sweeps(double* C, double* V, double2 (*Ab)[2 * slot::NB],
     }
 }
 
-// Covariance + Jacobi eigensolve + basis selection.
+// ---- leading eigenpairs without Jacobi sweeps ----
+//
+// The forward pass needs the 8 leading eigenvectors and the 25 eigenvalues.
+// Wave 0 reduces the covariance (scaled by its largest entry) to tridiagonal
+// form T = Q^T C Q with 23 Householder reflections H_k = I - u_k u_k^T; every
+// eigenvalue of T is bracketed by 20 rounds of 9-way multisection (8 lanes per
+// eigenvalue; Sturm counts from the division-free recurrence of the leading
+// minors, p_i = (d_i - x) p_{i-1} - e_{i-1}^2 p_{i-2}, rescaled by powers of
+// two); 8 lanes run two steps of inverse iteration on T - lambda I (Gaussian
+// elimination with partial pivoting, from a fixed start vector) and the
+// vectors go back through the reflections.  The result is accepted when every
+// residual |C v - lambda v| and every |v_i . v_j - delta_ij| is at most 1e-10
+// of the matrix scale; otherwise (eigenvalues too close for inverse iteration
+// to separate, a degenerate matrix) the caller runs the Jacobi sweeps.
+struct Top8 {
+    double W[kPatch][kPatch + 1];  // working matrix, reduced in place
+    double U[kPatch - 2][kPatch];  // u_k at entries k + 1 .. 24
+    double Q[32];
+    double d[kPatch], e[kPatch];
+    double2 de2[kPatch];           // (d_i, e_{i-1}^2), e_{-1} = 0
+    double lam[32];                // eigenvalues of T, descending
+    double X[kComponents][kPatch];
+    double f0r[kComponents][kPatch], f1[kComponents][kPatch], f2[kComponents][kPatch], fl[kComponents][kPatch];
+    double red[4];
+    double lo, hi;
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Eigenvalues of T below x: sign changes along p_0 = 1, p_1, .., p_25
+// (p_{-1} = 0), five terms per chunk, the pair rescaled after each chunk.
+__device__ __forceinline__ int sturm_below(const double2* de2, double x) {
+    double pm2 = 0.0, pm1 = 1.0;
+    int cnt = 0;
+#pragma unroll 1
+    for (int i0 = 0; i0 < kPatch; i0 += 5) {
+#pragma unroll
+        for (int i = i0; i < i0 + 5; i++) {
+            const double2 t = de2[i];
+            const double p = fma(t.x - x, pm1, -t.y * pm2);
+            cnt += (p < 0.0) != (pm1 < 0.0) ? 1 : 0;
+            pm2 = pm1;
+            pm1 = p;
+        }
+        const double a = fmax(fabs(pm1), fabs(pm2));
+        const double f = a > 0x1p+400 ? 0x1p-400 : (a < 0x1p-400 ? 0x1p+400 : 1.0);
+        pm1 *= f;
+        pm2 *= f;
+    }
+    return cnt;
+}
+
+// Writes S.lam (scaled back: eigenvalues of C, descending) and S.X (the 8
+// leading eigenvectors); returns whether they pass the checks.  A0: the
+// covariance (leading dimension kLD).  Every thread of the workgroup calls it.
+__device__ bool top8_eigen(const double* A0, Top8& S) {
+    constexpr int N = kPatch, LD = kLD;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    double mx = 0.0;
+    for (int e = tid; e < N * N; e += 256) mx = fmax(mx, fabs(A0[(e / N) * LD + e % N]));
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    if (lane == 0) S.red[wave] = mx;
+    __syncthreads();
+    const double scale = fmax(fmax(S.red[0], S.red[1]), fmax(S.red[2], S.red[3]));
+    if (!(scale > 0.0) || !(scale < 1e300)) return false;
+    const double rscale = 1.0 / scale;
+    for (int e = tid; e < N * N; e += 256) S.W[e / N][e % N] = A0[(e / N) * LD + e % N] * rscale;
+    __syncthreads();
+
+    // Householder tridiagonalisation (wave 0).  Step k: x = W[k+1 .., k],
+    // v = x - alpha e_1 with alpha = -sign(x_0) |x|, u = v sqrt(2 / |v|^2),
+    // p = W' u, q = p - (u.p / 2) u, W' -= u q^T + q u^T on the trailing block.
+    if (wave == 0) {
+        const int r = lane & 31, half = lane >> 5;
+        for (int k = 0; k < N - 2; k++) {
+            const int m = N - 1 - k;
+            const double x = lane < m ? S.W[k + 1 + lane][k] : 0.0;
+            double n2 = x * x;
+            for (int o = 32; o > 0; o >>= 1) n2 += __shfl_xor(n2, o);
+            const double x0 = __shfl(x, 0);
+            const double nrm = sqrt(n2);
+            const double alpha = x0 < 0.0 ? nrm : -nrm;
+            const double v0 = x0 - alpha;
+            const double nv = fma(v0, v0, fma(-x0, x0, n2));
+            const double f = nv > 0.0 ? sqrt(2.0 / nv) : 0.0;
+            const double u = (lane == 0 ? v0 : x) * f;
+            if (lane < m) S.U[k][k + 1 + lane] = u;
+            wave_sync();
+            double p = 0.0;
+            if (r < m)
+                for (int c = half; c < m; c += 2) p = fma(S.W[k + 1 + r][k + 1 + c], S.U[k][k + 1 + c], p);
+            p += __shfl_xor(p, 32);
+            const double ur = __shfl(u, r);
+            double kp = (half == 0 && r < m) ? ur * p : 0.0;
+            for (int o = 32; o > 0; o >>= 1) kp += __shfl_xor(kp, o);
+            const double q = fma(-0.5 * kp, ur, p);
+            if (half == 0 && r < m) S.Q[r] = q;
+            if (lane == 0) {
+                S.d[k] = S.W[k][k];
+                S.e[k] = alpha;
+            }
+            wave_sync();
+            if (r < m)
+                for (int c = half; c < m; c += 2) {
+                    double& w = S.W[k + 1 + r][k + 1 + c];
+                    w = fma(-ur, S.Q[c], fma(-q, S.U[k][k + 1 + c], w));
+                }
+            wave_sync();
+        }
+        if (lane == 0) {
+            S.d[N - 2] = S.W[N - 2][N - 2];
+            S.d[N - 1] = S.W[N - 1][N - 1];
+            S.e[N - 2] = S.W[N - 1][N - 2];
+            double lo = 1e300, hi = -1e300;
+            for (int i = 0; i < N; i++) {
+                const double rad = (i > 0 ? fabs(S.e[i - 1]) : 0.0) + (i < N - 1 ? fabs(S.e[i]) : 0.0);
+                lo = fmin(lo, S.d[i] - rad);
+                hi = fmax(hi, S.d[i] + rad);
+                S.de2[i] = make_double2(S.d[i], i > 0 ? S.e[i - 1] * S.e[i - 1] : 0.0);
+            }
+            const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
+            S.lo = lo - pad;
+            S.hi = hi + pad;
+        }
+    }
+    __syncthreads();
+
+    // Multisection: eigenvalue j (j-th largest) on lanes 8j' .. 8j' + 7 of
+    // wave j / 8; lane m tests lo + (m + 1) w / 9.
+    {
+        const int j = wave * 8 + (lane >> 3), m = lane & 7;
+        const int target = N - 1 - j;  // x is above the j-th largest iff more than target lie below x
+        double lo = S.lo, hi = S.hi;
+        for (int it = 0; it < 20; it++) {
+            const double w = hi - lo;
+            const double x = fma(w, (double)(m + 1) / 9.0, lo);
+            const bool above = sturm_below(S.de2, x) > target;
+            const uint64_t bal = __ballot(above);
+            const uint32_t grp = (uint32_t)(bal >> (lane & ~7)) & 0xffu;
+            const int ms = grp ? __builtin_ctz(grp) : 8;
+            const double nlo = ms == 0 ? lo : fma(w, (double)ms / 9.0, lo);
+            const double nhi = ms == 8 ? hi : fma(w, (double)(ms + 1) / 9.0, lo);
+            lo = nlo;
+            hi = nhi;
+        }
+        if (m == 0 && j < N) S.lam[j] = 0.5 * (lo + hi);
+    }
+    __syncthreads();
+
+    // Inverse iteration, eigenvector j on lane j of wave 0.
+    if (wave == 0 && lane < kComponents) {
+        const int j = lane;
+        const double lam = S.lam[j];
+        const double tiny = 1e-15 * fmax(fabs(S.lo), fabs(S.hi)) + 1e-300;
+        uint32_t piv = 0;
+        double c0 = S.d[0] - lam, c1 = S.e[0];
+#pragma unroll 1
+        for (int i = 0; i < N - 1; i++) {
+            const double nl = S.e[i], n0 = S.d[i + 1] - lam, n1 = i + 1 < N - 1 ? S.e[i + 1] : 0.0;
+            if (fabs(c0) >= fabs(nl)) {
+                const double pv = fabs(c0) < tiny ? copysign(tiny, c0) : c0;
+                const double rp = 1.0 / pv;
+                const double mu = nl * rp;
+                S.f0r[j][i] = rp;
+                S.f1[j][i] = c1;
+                S.f2[j][i] = 0.0;
+                S.fl[j][i] = mu;
+                c0 = fma(-mu, c1, n0);
+                c1 = n1;
+            } else {
+                const double mu = c0 / nl;
+                piv |= 1u << i;
+                S.f0r[j][i] = 1.0 / nl;
+                S.f1[j][i] = n0;
+                S.f2[j][i] = n1;
+                S.fl[j][i] = mu;
+                c0 = fma(-mu, n0, c1);
+                c1 = -mu * n1;
+            }
+        }
+        S.f0r[j][N - 1] = 1.0 / (fabs(c0) < tiny ? copysign(tiny, c0) : c0);
+        double* b = S.X[j];  // the iterate lives in LDS; a rolling window in registers
+#pragma unroll 1
+        for (int i = 0; i < N; i++) b[i] = 1.0 + 0.25 * (double)((i * 7 + j * 3) % 11) / 11.0;
+#pragma unroll 1
+        for (int itn = 0; itn < 2; itn++) {
+            double bi = b[0];
+#pragma unroll 1
+            for (int i = 0; i < N - 1; i++) {  // L^-1 P b
+                const double mu = S.fl[j][i], bn = b[i + 1];
+                const bool sw = (piv >> i) & 1u;
+                b[i] = sw ? bn : bi;
+                bi = sw ? fma(-mu, bn, bi) : fma(-mu, bi, bn);
+            }
+            double x1 = bi * S.f0r[j][N - 1];  // U^-1
+            b[N - 1] = x1;
+            double x0 = fma(-S.f1[j][N - 2], x1, b[N - 2]) * S.f0r[j][N - 2];
+            b[N - 2] = x0;
+            double n2 = fma(x1, x1, x0 * x0);
+#pragma unroll 1
+            for (int i = N - 3; i >= 0; i--) {
+                const double xi = fma(-S.f2[j][i], x1, fma(-S.f1[j][i], x0, b[i])) * S.f0r[j][i];
+                b[i] = xi;
+                n2 = fma(xi, xi, n2);
+                x1 = x0;
+                x0 = xi;
+            }
+            const double rn = rsq_refined(n2);
+#pragma unroll 1
+            for (int i = 0; i < N; i++) b[i] *= rn;
+        }
+    }
+    __syncthreads();
+
+    // Back through the reflections: v = H_0 H_1 .. H_22 x (wave 0: vector
+    // lane / 8, entries lane % 8 + 8 t).
+    if (wave == 0) {
+        const int j = lane >> 3, g = lane & 7;
+        double v[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) v[t] = g + 8 * t < N ? S.X[j][g + 8 * t] : 0.0;
+        for (int k = N - 3; k >= 0; k--) {
+            double dp = 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int i = g + 8 * t;
+                if (i > k && i < N) dp = fma(S.U[k][i], v[t], dp);
+            }
+            dp += __shfl_xor(dp, 1);
+            dp += __shfl_xor(dp, 2);
+            dp += __shfl_xor(dp, 4);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int i = g + 8 * t;
+                if (i > k && i < N) v[t] = fma(-S.U[k][i], dp, v[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            if (g + 8 * t < N) S.X[j][g + 8 * t] = v[t];
+    }
+    if (tid < N) S.lam[tid] *= scale;
+    __syncthreads();
+
+    // Checks: residuals (threads 0 .. 199) and the Gram matrix (200 .. 235).
+    double err = 0.0;
+    if (tid < kComponents * N) {
+        const int j = tid / N, r = tid % N;
+        double acc = -S.lam[j] * S.X[j][r];
+        for (int c = 0; c < N; c++) acc = fma(A0[r * LD + c], S.X[j][c], acc);
+        err = fabs(acc) * rscale;
+    } else if (tid < kComponents * N + 36) {
+        int pidx = tid - kComponents * N, a = 0;
+        while (pidx >= kComponents - a) {
+            pidx -= kComponents - a;
+            a++;
+        }
+        const int b2 = a + pidx;
+        double g = a == b2 ? -1.0 : 0.0;
+        for (int i = 0; i < N; i++) g = fma(S.X[a][i], S.X[b2][i], g);
+        err = fabs(g);
+    }
+    if (!(err <= 1e-10)) err = 1.0;  // NaN too
+    for (int o = 32; o > 0; o >>= 1) err = fmax(err, __shfl_xor(err, o));
+    if (lane == 0) S.red[wave] = err;
+    __syncthreads();
+    const bool ok = fmax(fmax(S.red[0], S.red[1]), fmax(S.red[2], S.red[3])) <= 1e-10;
+    __syncthreads();
+    return ok;
+}
+
+// Covariance + eigensolve + basis selection.
 //
 // C = (sum x x^T - (sum x)(sum x)^T / n) / (n - 1)   -- PCA.cpp:31-32
-// FP64 Jacobi sweeps until the off-diagonal energy is kJacobiTol of the total.
+// Leading eigenpairs by top8_eigen; where its checks fail, FP64 Jacobi sweeps
+// until the off-diagonal energy is kJacobiTol of the total.
 // The 8 largest eigenpairs, largest first (PCA.cpp:49-55), are written as
 // basis[r * 8 + c]; ties in eigenvalue keep index order; each vector is
 // oriented to a non-negative dot product with sign_ref.
@@ -435,6 +712,21 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
         A0[s * LD + r] = c;
     }
     __syncthreads();
+    // The basis and eigenvalues come from the leading-eigenpair solver when
+    // its checks pass; the Jacobi sweeps run when they fail, and for the full
+    // decomposition of the reverse pass (which then only fills eig_full).
+    __shared__ Top8 t8;
+    const bool fast = top8_eigen(A0, t8);
+    if (fast) {
+        if (tid < N && eigvals) eigvals[tid] = t8.lam[tid];
+        if (tid < kComponents) {
+            double dot = 0.0;
+            for (int r = 0; r < N; r++) dot += t8.X[tid][r] * sign_ref(r);
+            const double sg = dot < 0.0 ? -1.0 : 1.0;
+            for (int r = 0; r < N; r++) basis[r * kComponents + tid] = (float)(sg * t8.X[tid][r]);
+        }
+        if (!eig_full) return;
+    }
     jacobi_sweeps(A0, V, Ab, Vb, kJacobiTol, 40, energy);
     __syncthreads();
     const double* Ac = A0;
@@ -446,13 +738,14 @@ __device__ void pca_solve_body(const double* moments, double n, float* basis, do
             const double dj = Ac[j * LD + j];
             rank += (dj > di || (dj == di && j < tid)) ? 1 : 0;
         }
-        if (eigvals) eigvals[rank] = di;
+        if (eigvals && !fast) eigvals[rank] = di;
         double sg = 1.0;
         if (rank < kComponents) {
             double dot = 0.0;
             for (int r = 0; r < N; r++) dot += V[r * LD + tid] * sign_ref(r);
             sg = dot < 0.0 ? -1.0 : 1.0;
-            for (int r = 0; r < N; r++) basis[r * kComponents + rank] = (float)(sg * V[r * LD + tid]);
+            if (!fast)
+                for (int r = 0; r < N; r++) basis[r * kComponents + rank] = (float)(sg * V[r * LD + tid]);
         }
         // Full decomposition for the reverse pass: V (column = eigenvector,
         // descending), eigenvalues, the 8 orientations, the mean patch.

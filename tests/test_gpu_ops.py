@@ -91,6 +91,43 @@ def test_build_pca_space_parity(n):
     np.testing.assert_allclose(basis.T @ basis, np.eye(8), atol=1e-6)
 
 
+@pytest.mark.parametrize("n", [744, 20000])
+def test_build_pca_space_all_eigenvalues(n):
+    """Every eigenvalue (the leading eight and the other seventeen) against the oracle's Jacobi."""
+    rng = np.random.default_rng(n + 1)
+    pts = (rng.normal(size=(n, 25)) @ rng.normal(size=(25, 25)) * 0.05 + 0.4).astype(np.float32)
+    _, ev = gcam.build_pca_space(to_dev(pts))
+    _, rev = oracle.build_pca_space(pts)
+    np.testing.assert_allclose(to_host(ev), rev, rtol=0, atol=1e-12 * abs(rev[0]))
+
+
+@pytest.mark.parametrize("kind", ["equal", "zero", "near"])
+def test_build_pca_space_degenerate_spectra(kind):
+    """An exactly diagonal covariance with an 8-fold leading eigenvalue (inverse iteration
+    cannot separate the eigenvectors, so the solve falls back to the Jacobi sweeps, which
+    keep index order for ties like the oracle), the zero matrix, and a near-degenerate
+    spectrum (eigenvalues equal up to rounding: only the spanned subspace is defined)."""
+    if kind == "equal":
+        pts = np.zeros((17, 25), np.float32)
+        for i in range(8):
+            pts[2 * i, i], pts[2 * i + 1, i] = 1.0, -1.0
+    elif kind == "zero":
+        pts = np.full((64, 25), 0.25, np.float32)
+    else:
+        pts = np.full((17, 25), 0.4, np.float32)
+        for i in range(8):
+            pts[2 * i, i] += 2.0
+            pts[2 * i + 1, i] -= 2.0
+    basis, ev = gcam.build_pca_space(to_dev(pts))
+    basis, ev = to_host(basis).astype(np.float64), to_host(ev)
+    rb, rev = oracle.build_pca_space(pts)
+    np.testing.assert_allclose(ev, rev, rtol=0, atol=1e-12 * max(abs(rev[0]), 1e-30))
+    np.testing.assert_allclose(basis.T @ basis, np.eye(8), atol=1e-6)
+    np.testing.assert_allclose(basis @ basis.T, rb @ rb.T, atol=1e-6)
+    if kind != "near":
+        np.testing.assert_array_equal(basis, rb)
+
+
 def test_image_moments_match_collected_patches():
     """The fused path (moments straight from the image) equals collect + build_pca_space."""
     img = rand_img(189, 252, 8)
