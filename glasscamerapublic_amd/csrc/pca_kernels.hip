@@ -422,13 +422,13 @@ __device__ void jacobi_sweeps(double* C, double* V, double2 (*Ab)[2 * slot::NB],
 // to separate, a degenerate matrix) the caller runs the Jacobi sweeps.
 struct Top8 {
     double W[kPatch][kPatch + 1];  // working matrix, reduced in place
-    double U[kPatch - 2][kPatch];  // u_k at entries k + 1 .. 24
-    double Q[32];
+    double U[kPatch - 2][kPatch];  // u_k at entries k + 1 .. 24 (zero below)
+    double bx[64], bu[64], bq[64];  // per-lane vectors of a reduction step
     double d[kPatch], e[kPatch];
     double2 de2[kPatch];           // (d_i, e_{i-1}^2), e_{-1} = 0
     double lam[32];                // eigenvalues of T, descending
     double X[kComponents][kPatch];
-    double f0r[kComponents][kPatch], f1[kComponents][kPatch], f2[kComponents][kPatch], fl[kComponents][kPatch];
+    double f0r[kComponents][kPatch], f1[kComponents][kPatch], f2[kComponents][kPatch];  // U of T - lambda I
     double red[4];
     double lo, hi;
 };
@@ -440,26 +440,31 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Eigenvalues of T below x: sign changes along p_0 = 1, p_1, .., p_25
-// (p_{-1} = 0), five terms per chunk, the pair rescaled after each chunk.
-__device__ __forceinline__ int sturm_below(const double2* de2, double x) {
+// (p_{-1} = 0); d[i] = T_ii, e2[i] = T_{i,i-1}^2 (e2[0] = 0), held in
+// registers; the pair is rescaled by a power of two every five terms.
+__device__ __forceinline__ int sturm_below(const double (&d)[kPatch], const double (&e2)[kPatch], double x) {
     double pm2 = 0.0, pm1 = 1.0;
     int cnt = 0;
-#pragma unroll 1
-    for (int i0 = 0; i0 < kPatch; i0 += 5) {
 #pragma unroll
-        for (int i = i0; i < i0 + 5; i++) {
-            const double2 t = de2[i];
-            const double p = fma(t.x - x, pm1, -t.y * pm2);
-            cnt += (p < 0.0) != (pm1 < 0.0) ? 1 : 0;
-            pm2 = pm1;
-            pm1 = p;
+    for (int i = 0; i < kPatch; i++) {
+        const double p = fma(d[i] - x, pm1, -e2[i] * pm2);
+        cnt += (p < 0.0) != (pm1 < 0.0) ? 1 : 0;
+        pm2 = pm1;
+        pm1 = p;
+        if (i % 5 == 4) {
+            const double a = fmax(fabs(pm1), fabs(pm2));
+            const double f = a > 0x1p+400 ? 0x1p-400 : (a < 0x1p-400 ? 0x1p+400 : 1.0);
+            pm1 *= f;
+            pm2 *= f;
         }
-        const double a = fmax(fabs(pm1), fabs(pm2));
-        const double f = a > 0x1p+400 ? 0x1p-400 : (a < 0x1p-400 ? 0x1p+400 : 1.0);
-        pm1 *= f;
-        pm2 *= f;
     }
     return cnt;
+}
+
+__device__ __forceinline__ double rcp_refined(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(y, fma(-x, y, 1.0), y);
+    return fma(y, fma(-x, y, 1.0), y);
 }
 
 // Writes S.lam (scaled back: eigenvalues of C, descending) and S.X (the 8
@@ -480,50 +485,72 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
     for (int e = tid; e < N * N; e += 256) S.W[e / N][e % N] = A0[(e / N) * LD + e % N] * rscale;
     __syncthreads();
 
-    // Householder tridiagonalisation (wave 0).  Step k: x = W[k+1 .., k],
+    // Householder tridiagonalisation (wave 0, lane r holding row r of W in
+    // registers, every step unrolled).  Step k: x = W[k+1 .., k],
     // v = x - alpha e_1 with alpha = -sign(x_0) |x|, u = v sqrt(2 / |v|^2),
-    // p = W' u, q = p - (u.p / 2) u, W' -= u q^T + q u^T on the trailing block.
+    // p = W' u, q = p - (u.p / 2) u, W' -= u q^T + q u^T on the trailing block;
+    // the vectors x, u, u.p and q are exchanged through LDS.
     if (wave == 0) {
-        const int r = lane & 31, half = lane >> 5;
+        double w[N];
+#pragma unroll
+        for (int c = 0; c < N; c++) w[c] = lane < N ? S.W[lane][c] : 0.0;
+#pragma unroll
         for (int k = 0; k < N - 2; k++) {
-            const int m = N - 1 - k;
-            const double x = lane < m ? S.W[k + 1 + lane][k] : 0.0;
-            double n2 = x * x;
-            for (int o = 32; o > 0; o >>= 1) n2 += __shfl_xor(n2, o);
-            const double x0 = __shfl(x, 0);
+            const bool below = lane > k && lane < N;
+            const double x = below ? w[k] : 0.0;
+            S.bx[lane] = x;
+            wave_sync();
+            double n2a = 0.0, n2b = 0.0;
+#pragma unroll
+            for (int c = k + 1; c < N; c += 2) {
+                const double a = S.bx[c];
+                n2a = fma(a, a, n2a);
+                if (c + 1 < N) {
+                    const double b2 = S.bx[c + 1];
+                    n2b = fma(b2, b2, n2b);
+                }
+            }
+            const double n2 = n2a + n2b, x0 = S.bx[k + 1];
             const double nrm = sqrt(n2);
             const double alpha = x0 < 0.0 ? nrm : -nrm;
             const double v0 = x0 - alpha;
             const double nv = fma(v0, v0, fma(-x0, x0, n2));
             const double f = nv > 0.0 ? sqrt(2.0 / nv) : 0.0;
-            const double u = (lane == 0 ? v0 : x) * f;
-            if (lane < m) S.U[k][k + 1 + lane] = u;
+            const double u = (lane == k + 1 ? v0 : x) * f;
+            S.bu[lane] = u;
+            if (lane < N) S.U[k][lane] = u;
             wave_sync();
-            double p = 0.0;
-            if (r < m)
-                for (int c = half; c < m; c += 2) p = fma(S.W[k + 1 + r][k + 1 + c], S.U[k][k + 1 + c], p);
-            p += __shfl_xor(p, 32);
-            const double ur = __shfl(u, r);
-            double kp = (half == 0 && r < m) ? ur * p : 0.0;
-            for (int o = 32; o > 0; o >>= 1) kp += __shfl_xor(kp, o);
-            const double q = fma(-0.5 * kp, ur, p);
-            if (half == 0 && r < m) S.Q[r] = q;
-            if (lane == 0) {
-                S.d[k] = S.W[k][k];
-                S.e[k] = alpha;
+            double pa = 0.0, pb = 0.0;
+#pragma unroll
+            for (int c = k + 1; c < N; c += 2) {
+                pa = fma(w[c], S.bu[c], pa);
+                if (c + 1 < N) pb = fma(w[c + 1], S.bu[c + 1], pb);
             }
+            const double p = below ? pa + pb : 0.0;
+            S.bx[lane] = u * p;
             wave_sync();
-            if (r < m)
-                for (int c = half; c < m; c += 2) {
-                    double& w = S.W[k + 1 + r][k + 1 + c];
-                    w = fma(-ur, S.Q[c], fma(-q, S.U[k][k + 1 + c], w));
-                }
+            double ka = 0.0, kb = 0.0;
+#pragma unroll
+            for (int c = k + 1; c < N; c += 2) {
+                ka += S.bx[c];
+                if (c + 1 < N) kb += S.bx[c + 1];
+            }
+            const double q = below ? fma(-0.5 * (ka + kb), u, p) : 0.0;
+            S.bq[lane] = q;
+            if (lane == k) S.d[k] = w[k];
+            if (lane == 0) S.e[k] = alpha;
+            wave_sync();
+#pragma unroll
+            for (int c = k + 1; c < N; c++) w[c] = fma(-u, S.bq[c], fma(-q, S.bu[c], w[c]));
             wave_sync();
         }
+        if (lane == N - 2) S.d[N - 2] = w[N - 2];
+        if (lane == N - 1) {
+            S.d[N - 1] = w[N - 1];
+            S.e[N - 2] = w[N - 2];
+        }
+        wave_sync();
         if (lane == 0) {
-            S.d[N - 2] = S.W[N - 2][N - 2];
-            S.d[N - 1] = S.W[N - 1][N - 1];
-            S.e[N - 2] = S.W[N - 1][N - 2];
             double lo = 1e300, hi = -1e300;
             for (int i = 0; i < N; i++) {
                 const double rad = (i > 0 ? fabs(S.e[i - 1]) : 0.0) + (i < N - 1 ? fabs(S.e[i]) : 0.0);
@@ -539,15 +566,23 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
     __syncthreads();
 
     // Multisection: eigenvalue j (j-th largest) on lanes 8j' .. 8j' + 7 of
-    // wave j / 8; lane m tests lo + (m + 1) w / 9.
+    // wave j / 8; lane m tests lo + (m + 1) w / 9.  17 rounds shrink the
+    // Gershgorin interval by 9^17 > 2^53.
     {
         const int j = wave * 8 + (lane >> 3), m = lane & 7;
         const int target = N - 1 - j;  // x is above the j-th largest iff more than target lie below x
+        double dd[N], ee[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const double2 t = S.de2[i];
+            dd[i] = t.x;
+            ee[i] = t.y;
+        }
         double lo = S.lo, hi = S.hi;
-        for (int it = 0; it < 20; it++) {
+        for (int it = 0; it < 17; it++) {
             const double w = hi - lo;
             const double x = fma(w, (double)(m + 1) / 9.0, lo);
-            const bool above = sturm_below(S.de2, x) > target;
+            const bool above = sturm_below(dd, ee, x) > target;
             const uint64_t bal = __ballot(above);
             const uint32_t grp = (uint32_t)(bal >> (lane & ~7)) & 0xffu;
             const int ms = grp ? __builtin_ctz(grp) : 8;
@@ -560,68 +595,56 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
     }
     __syncthreads();
 
-    // Inverse iteration, eigenvector j on lane j of wave 0.
+    // Inverse iteration, eigenvector j on lane j of wave 0: one solve of
+    // (T - lambda I) y = b from a fixed start vector b (lambda is accurate to
+    // rounding, so one solve suffices), by Gaussian elimination with partial
+    // pivoting applied to b as it goes (U has two superdiagonals; pivots
+    // below 1e-15 |T| are raised to that size), then back substitution and
+    // normalisation.  U and the eliminated b live in LDS.
     if (wave == 0 && lane < kComponents) {
         const int j = lane;
         const double lam = S.lam[j];
         const double tiny = 1e-15 * fmax(fabs(S.lo), fabs(S.hi)) + 1e-300;
-        uint32_t piv = 0;
-        double c0 = S.d[0] - lam, c1 = S.e[0];
+        double* ur = S.f0r[j];
+        double* u1 = S.f1[j];
+        double* u2 = S.f2[j];
+        double* bb = S.X[j];
+        auto start = [&](int i) { return 1.0 + 0.25 * (double)((i * 7 + j * 3) % 11) / 11.0; };
+        double c0 = S.d[0] - lam, c1 = S.e[0], cb = start(0);
 #pragma unroll 1
         for (int i = 0; i < N - 1; i++) {
             const double nl = S.e[i], n0 = S.d[i + 1] - lam, n1 = i + 1 < N - 1 ? S.e[i + 1] : 0.0;
-            if (fabs(c0) >= fabs(nl)) {
-                const double pv = fabs(c0) < tiny ? copysign(tiny, c0) : c0;
-                const double rp = 1.0 / pv;
-                const double mu = nl * rp;
-                S.f0r[j][i] = rp;
-                S.f1[j][i] = c1;
-                S.f2[j][i] = 0.0;
-                S.fl[j][i] = mu;
-                c0 = fma(-mu, c1, n0);
-                c1 = n1;
-            } else {
-                const double mu = c0 / nl;
-                piv |= 1u << i;
-                S.f0r[j][i] = 1.0 / nl;
-                S.f1[j][i] = n0;
-                S.f2[j][i] = n1;
-                S.fl[j][i] = mu;
-                c0 = fma(-mu, n0, c1);
-                c1 = -mu * n1;
-            }
+            const double nb = start(i + 1);
+            const bool sw = fabs(c0) < fabs(nl);
+            const double pv = sw ? nl : (fabs(c0) < tiny ? copysign(tiny, c0) : c0);
+            const double rp = rcp_refined(pv);
+            const double mu = (sw ? c0 : nl) * rp;
+            ur[i] = rp;
+            u1[i] = sw ? n0 : c1;
+            u2[i] = sw ? n1 : 0.0;
+            bb[i] = sw ? nb : cb;
+            const double nc0 = sw ? fma(-mu, n0, c1) : fma(-mu, c1, n0);
+            const double ncb = sw ? fma(-mu, nb, cb) : fma(-mu, cb, nb);
+            c1 = sw ? -mu * n1 : n1;
+            c0 = nc0;
+            cb = ncb;
         }
-        S.f0r[j][N - 1] = 1.0 / (fabs(c0) < tiny ? copysign(tiny, c0) : c0);
-        double* b = S.X[j];  // the iterate lives in LDS; a rolling window in registers
+        double x1 = cb * rcp_refined(fabs(c0) < tiny ? copysign(tiny, c0) : c0);
+        double x0 = fma(-u1[N - 2], x1, bb[N - 2]) * ur[N - 2];
+        bb[N - 1] = x1;
+        bb[N - 2] = x0;
+        double n2 = fma(x1, x1, x0 * x0);
 #pragma unroll 1
-        for (int i = 0; i < N; i++) b[i] = 1.0 + 0.25 * (double)((i * 7 + j * 3) % 11) / 11.0;
-#pragma unroll 1
-        for (int itn = 0; itn < 2; itn++) {
-            double bi = b[0];
-#pragma unroll 1
-            for (int i = 0; i < N - 1; i++) {  // L^-1 P b
-                const double mu = S.fl[j][i], bn = b[i + 1];
-                const bool sw = (piv >> i) & 1u;
-                b[i] = sw ? bn : bi;
-                bi = sw ? fma(-mu, bn, bi) : fma(-mu, bi, bn);
-            }
-            double x1 = bi * S.f0r[j][N - 1];  // U^-1
-            b[N - 1] = x1;
-            double x0 = fma(-S.f1[j][N - 2], x1, b[N - 2]) * S.f0r[j][N - 2];
-            b[N - 2] = x0;
-            double n2 = fma(x1, x1, x0 * x0);
-#pragma unroll 1
-            for (int i = N - 3; i >= 0; i--) {
-                const double xi = fma(-S.f2[j][i], x1, fma(-S.f1[j][i], x0, b[i])) * S.f0r[j][i];
-                b[i] = xi;
-                n2 = fma(xi, xi, n2);
-                x1 = x0;
-                x0 = xi;
-            }
-            const double rn = rsq_refined(n2);
-#pragma unroll 1
-            for (int i = 0; i < N; i++) b[i] *= rn;
+        for (int i = N - 3; i >= 0; i--) {
+            const double xi = fma(-u2[i], x1, fma(-u1[i], x0, bb[i])) * ur[i];
+            bb[i] = xi;
+            n2 = fma(xi, xi, n2);
+            x1 = x0;
+            x0 = xi;
         }
+        const double rn = rsq_refined(n2);
+#pragma unroll 1
+        for (int i = 0; i < N; i++) bb[i] *= rn;
     }
     __syncthreads();
 
