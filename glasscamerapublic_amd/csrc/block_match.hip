@@ -202,7 +202,7 @@ gcam_status launch_block_match(View<const float4> in, View<const float2> grad, V
                                const NlfDev* nlf_level, const BmParams& p, int radius, int rows_per_thread,
                                View<float4> out, hipStream_t s) {
     static const DirTable dir = make_dir_table();
-    const int rows = rows_per_thread == 0 ? (radius <= 6 ? 2 : 1) : rows_per_thread;
+    const int rows = rows_per_thread == 0 ? default_bm_rows(radius, out.w, out.h) : rows_per_thread;
     switch (radius) {
         case 1: return launch_r<1>(in, grad, feat8, nlf_level, p, dir, rows, out, s);
         case 2: return launch_r<2>(in, grad, feat8, nlf_level, p, dir, rows, out, s);

@@ -28,9 +28,12 @@
 //    squared distance of the rounded features up to FP32 rounding.  Weights
 //    and sums are FP32; reciprocals are the hardware v_rcp_f32 (1 ulp).
 // 4. The inner loop is bound by vector-ALU issue (a transcendental costs
-//    about two slots, so exp2 stays at the two weights): component 0 enters
-//    the exponent as -(r f0q - r f0p)^2 with r = sqrt(-kl), one FMA for the
-//    difference instead of a subtraction and a product.
+//    about four FMA slots, so exp2 stays at the two weights): component 0
+//    enters the exponent as -(r f0q - r f0p)^2 with r = sqrt(-kl), one FMA for
+//    the difference instead of a subtraction and a product.  The luma weight
+//    is produced pre-scaled, wl' = sqrt(2 log2 e) wl (its exponent carries
+//    log2 sqrt(2 log2 e)), so the chroma exponent's -2 log2(e) wl^2 is one
+//    FMA -wl' wl' and the direction table holds w / sqrt(2 log2 e).
 #include "gcam_internal.hpp"
 #include "burst_tables.hpp"
 
@@ -202,8 +205,10 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
         h8 fp[PV];
         float r1[PV], pr[PV], kl[PV], k2[PV], cp[PV], sb[PV], sr[PV], sbc[PV], src[PV];
         float fY[PV], fCb[PV], fCr[PV], nY[PV], nC[PV];
-        const float kw = -2.0f * kLog2e, kroot = sqrtf(2.0f * kLog2e);
+        const float kroot = sqrtf(2.0f * kLog2e), lkroot = 0.5f * log2f(2.0f * kLog2e);
         const float rcb = __builtin_amdgcn_rcpf(p.chromaBoost), rscale = sref > 0.0f ? sref : 1.0f;
+        const float rthr0 = __builtin_amdgcn_rcpf(p.thr[0]), rthr1 = __builtin_amdgcn_rcpf(p.thr[1]);
+        const float rthr2 = __builtin_amdgcn_rcpf(p.thr[2]);
 #pragma unroll
         for (int k = 0; k < PV; k++) {
             const int y = min(yb + k, out.h - 1);
@@ -218,21 +223,21 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
                 const float d = sqrtf(dx * dx + dy * dy) / sqrtf(cx * cx + cy * cy);
                 lens = 1.0f + p.lensShading * d * d;
             }
-            const float s0 = sqrtf(nlf->a[0] + nlf->b[0] * lens * v.x);
-            const float s1 = sqrtf(nlf->a[1] + nlf->b[1] * lens * v.x);
-            const float s2 = sqrtf(nlf->a[2] + nlf->b[2] * lens * v.x);
-            const float dm0 = __builtin_amdgcn_rcpf(p.thr[0] * s0);
-            const float dm1 = __builtin_amdgcn_rcpf(p.thr[1] * s1), dm2 = __builtin_amdgcn_rcpf(p.thr[2] * s2);
+            // 1 / sigma per channel (hardware reciprocal square root)
+            const float is0 = __builtin_amdgcn_rsqf(nlf->a[0] + nlf->b[0] * lens * v.x);
+            const float is1 = __builtin_amdgcn_rsqf(nlf->a[1] + nlf->b[1] * lens * v.x);
+            const float is2 = __builtin_amdgcn_rsqf(nlf->a[2] + nlf->b[2] * lens * v.x);
+            const float dm0 = is0 * rthr0, dm1 = is1 * rthr1, dm2 = is2 * rthr2;
             float mag = 0.0f;
             if (grad.ptr) {
                 const float2 gg = grad.at(x, y);
                 mag = sqrtf(gg.x * gg.x + gg.y * gg.y);
             }
-            const float edge = smoothstepf(2.0f, 16.0f, p.gradientThreshold * mag / s0);
+            const float edge = smoothstepf(2.0f, 16.0f, p.gradientThreshold * mag * is0);
             const float lm = dm0 * rscale * __builtin_amdgcn_rcpf(1.0f + p.gradientBoost * edge);
             kl[k] = -2.0f * kLog2e * lm * lm;  // per unit of scaled squared distance
             k2[k] = -2.0f * kl[k];
-            cp[k] = k2[k] * v.w;               // kl |f_p|^2 (v.w = -|f_p|^2 / 2)
+            cp[k] = fmaf(k2[k], v.w, lkroot);  // kl |f_p|^2 (v.w = -|f_p|^2 / 2) + log2 sqrt(2 log2 e)
             r1[k] = sqrtf(-kl[k]);             // kl t0^2 = -(r1 t0)^2
             pr[k] = -f0p * r1[k];
             sb[k] = kroot * (dm1 * rcb);
@@ -264,12 +269,13 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
                     acc = P::dot2(h2{fq[4], fq[5]}, h2{fp[k][4], fp[k][5]}, acc);
                     acc = P::dot2(h2{fq[6], fq[7]}, h2{fp[k][6], fp[k][7]}, acc);
                     const float t0 = fmaf(f0q, r1[k], pr[k]);
+                    // wl' = sqrt(2 log2 e) wl; wd holds w / sqrt(2 log2 e)
                     const float wl = __builtin_amdgcn_exp2f(fmaf(k2[k], acc, fmaf(-t0, t0, cp[k])));
                     const float wY = wd * wl;
                     const float tb = fmaf(sb[k], qy.y, -sbc[k]), tr = fmaf(sr[k], qy.z, -src[k]);
                     float e = fmaf(-tb, tb, lwd);
                     e = fmaf(-tr, tr, e);
-                    e = fmaf(kw * wl, wl, e);
+                    e = fmaf(-wl, wl, e);
                     const float wC = __builtin_amdgcn_exp2f(e);
                     fY[k] = fmaf(wY, qy.x, fY[k]);
                     nY[k] += wY;
@@ -343,12 +349,16 @@ gcam_status launch_lp_r(View<const float4> in, View<const float2> grad, const fl
     }
 }
 
+// w: the direction weight divided by sqrt(2 log2 e) (the luma weight enters
+// pre-scaled by that factor); lw: log2 of the direction weight itself.
 DirTable make_dir_table() {
     DirTable t{};
+    const float kroot = std::sqrt(2.0f * kLog2e);
     for (int d2 = 0; d2 < (int)(sizeof(t.w) / sizeof(float)); d2++) {
         const float a = 0.1f * std::sqrt((float)d2);
-        t.w[d2] = std::exp(-2.0f * a * a);
-        t.lw[d2] = std::log2(t.w[d2]);
+        const float w = std::exp(-2.0f * a * a);
+        t.w[d2] = w / kroot;
+        t.lw[d2] = std::log2(w);
     }
     return t;
 }
@@ -437,7 +447,7 @@ gcam_status launch_burst_block_match_lp(const BurstTable<BurstBmFrame>* dev, int
 gcam_status launch_block_match_lp(View<const float4> in, View<const float2> grad, const float* basis,
                                   const NlfDev* nlf_level, const BmParams& p, int radius, int rows_per_thread,
                                   int precision, View<float4> out, hipStream_t s) {
-    const int rows = rows_per_thread == 0 ? (radius <= 6 ? 2 : 1) : rows_per_thread;
+    const int rows = rows_per_thread == 0 ? default_bm_rows(radius, out.w, out.h) : rows_per_thread;
     if (precision == GCAM_PRECISION_FP16) return dispatch<_Float16>(in, grad, basis, nlf_level, p, radius, rows, out, s);
     if (precision == GCAM_PRECISION_BF16) return dispatch<__bf16>(in, grad, basis, nlf_level, p, radius, rows, out, s);
     return set_error(GCAM_ERR_INVALID_ARGUMENT, "low-precision block matching needs FP16 or BF16");

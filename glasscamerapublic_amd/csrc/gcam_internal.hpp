@@ -105,6 +105,16 @@ gcam_status launch_pca_solve(double* partials, int nblocks, int64_t n, float* ba
                              double* eig_full, hipStream_t s);
 gcam_status launch_pca_project(View<const float4> in, const float* basis, View<float> feat8, hipStream_t s);
 
+// Default output rows per thread of block matching (rows_per_thread == 0):
+// two (32 x 16 output tiles) while that still gives at least 1024 workgroups,
+// else one (32 x 8 tiles: twice the workgroups for the small coarse levels);
+// one for radii above 6.
+inline int default_bm_rows(int radius, int w, int h) {
+    if (radius > 6) return 1;
+    const long tiles2 = (long)((w + 31) / 32) * ((h + 15) / 16);
+    return tiles2 >= 1024 ? 2 : 1;
+}
+
 gcam_status launch_block_match(View<const float4> in, View<const float2> grad, View<const float> feat8,
                                const NlfDev* nlf_level, const BmParams& p, int radius, int rows_per_thread,
                                View<float4> out, hipStream_t s);

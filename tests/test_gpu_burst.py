@@ -2,7 +2,7 @@
 
 A burst runs PyramidProcessor::denoise on every frame through batched
 launches, a persistent block-matching kernel and one captured graph.  With the
-block-matching tile fixed to the single-frame default, every output level of
+block-matching tile fixed to 32 x 16 outputs (two rows per thread), every output level of
 every frame must equal the single-frame result bit for bit (same kernels, same
 reduction order); with the autotuned tile the luma centring of the fused
 projection moves with the tile, so outputs agree to FP32 rounding.  The graph is
