@@ -410,25 +410,27 @@ __device__ void jacobi_sweeps(double* C, double* V, double2 (*Ab)[2 * slot::NB],
 //
 // The forward pass needs the 8 leading eigenvectors and the 25 eigenvalues.
 // Wave 0 reduces the covariance (scaled by its largest entry) to tridiagonal
-// form T = Q^T C Q with 23 Householder reflections H_k = I - u_k u_k^T; every
-// eigenvalue of T is bracketed by 20 rounds of 9-way multisection (8 lanes per
+// form T = Q^T C Q with 23 Householder reflections H_k = I - u_k u_k^T,
+// accumulating Q = H_0 H_1 .. H_22 in registers as it goes; every eigenvalue
+// of T is bracketed by 17 rounds of 9-way multisection (8 lanes per
 // eigenvalue; Sturm counts from the division-free recurrence of the leading
 // minors, p_i = (d_i - x) p_{i-1} - e_{i-1}^2 p_{i-2}, rescaled by powers of
-// two); 8 lanes run two steps of inverse iteration on T - lambda I (Gaussian
-// elimination with partial pivoting, from a fixed start vector) and the
-// vectors go back through the reflections.  The result is accepted when every
-// residual |C v - lambda v| and every |v_i . v_j - delta_ij| is at most 1e-10
-// of the matrix scale; otherwise (eigenvalues too close for inverse iteration
-// to separate, a degenerate matrix) the caller runs the Jacobi sweeps.
+// two); 8 lanes run one solve of inverse iteration on T - lambda I (Gaussian
+// elimination with partial pivoting, from a fixed start vector, all in
+// registers) and the vectors are mapped back as v = Q y.  The result is
+// accepted when every residual |C v - lambda v| and every
+// |v_i . v_j - delta_ij| is at most 1e-10 of the matrix scale; otherwise
+// (eigenvalues too close for inverse iteration to separate, a degenerate
+// matrix) the caller runs the Jacobi sweeps.
 struct Top8 {
-    double W[kPatch][kPatch + 1];  // working matrix, reduced in place
-    double U[kPatch - 2][kPatch];  // u_k at entries k + 1 .. 24 (zero below)
-    double bx[64], bu[64], bq[64];  // per-lane vectors of a reduction step
+    double W[kPatch][kPatch + 1];  // working matrix on entry; Q after the reduction
+    double bu[64], bp[64];         // u and p = W u of a reduction step, one entry per lane
     double d[kPatch], e[kPatch];
     double2 de2[kPatch];           // (d_i, e_{i-1}^2), e_{-1} = 0
     double lam[32];                // eigenvalues of T, descending
-    double X[kComponents][kPatch];
-    double f0r[kComponents][kPatch], f1[kComponents][kPatch], f2[kComponents][kPatch];  // U of T - lambda I
+    double Y[kComponents][kPatch + 1];  // eigenvectors of T (the eliminated b before)
+    double ur[kComponents][kPatch], u1[kComponents][kPatch], u2[kComponents][kPatch];  // U of T - lambda I
+    double X[kComponents][kPatch];      // eigenvectors of C
     double red[4];
     double lo, hi;
 };
@@ -437,6 +439,14 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The value of v held by lane l (l wave-uniform), in every lane.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Eigenvalues of T below x: sign changes along p_0 = 1, p_1, .., p_25
@@ -467,6 +477,13 @@ __device__ __forceinline__ double rcp_refined(double x) {
     return fma(y, fma(-x, y, 1.0), y);
 }
 
+// 1 / sqrt(x) for x > 0 to full double precision (two Newton steps).
+__device__ __forceinline__ double rsq_full(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(x * y, -0.5 * y, 1.5);
+    return y * fma(x * y, -0.5 * y, 1.5);
+}
+
 // Writes S.lam (scaled back: eigenvalues of C, descending) and S.X (the 8
 // leading eigenvectors); returns whether they pass the checks.  A0: the
 // covariance (leading dimension kLD).  Every thread of the workgroup calls it.
@@ -485,69 +502,92 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
     for (int e = tid; e < N * N; e += 256) S.W[e / N][e % N] = A0[(e / N) * LD + e % N] * rscale;
     __syncthreads();
 
-    // Householder tridiagonalisation (wave 0, lane r holding row r of W in
-    // registers, every step unrolled).  Step k: x = W[k+1 .., k],
-    // v = x - alpha e_1 with alpha = -sign(x_0) |x|, u = v sqrt(2 / |v|^2),
-    // p = W' u, q = p - (u.p / 2) u, W' -= u q^T + q u^T on the trailing block;
-    // the vectors x, u, u.p and q are exchanged through LDS.
+    // Householder tridiagonalisation on wave 0, every step unrolled; lane r
+    // holds row r of W (w) and row r of Q (q) in registers.  Step k: lane k
+    // holds x = W[k][k+1 ..] (= column k below the diagonal) and forms
+    // alpha = -sign(x_0) |x|, v_0 = x_0 - alpha and f = sqrt(2 / |v|^2),
+    // broadcast with readlane; lane r > k forms u_r = v_r f from its own
+    // W[r][k].  Two exchanges through LDS: u, then p = W u; with K = u . p,
+    // W <- W - u p^T - p u^T + K u u^T (row r: w_c -= u_r p_c + (p_r - K u_r)
+    // u_c) and Q <- Q - (Q u) u^T.
     if (wave == 0) {
-        double w[N];
+        double w[N], q[N];
 #pragma unroll
-        for (int c = 0; c < N; c++) w[c] = lane < N ? S.W[lane][c] : 0.0;
+        for (int c = 0; c < N; c++) {
+            w[c] = lane < N ? S.W[lane][c] : 0.0;
+            q[c] = lane == c ? 1.0 : 0.0;
+        }
 #pragma unroll
         for (int k = 0; k < N - 2; k++) {
+            double na = 0.0, nb = 0.0, nc = 0.0, nd = 0.0;
+#pragma unroll
+            for (int c = k + 1; c < N; c += 4) {
+                na = fma(w[c], w[c], na);
+                if (c + 1 < N) nb = fma(w[c + 1], w[c + 1], nb);
+                if (c + 2 < N) nc = fma(w[c + 2], w[c + 2], nc);
+                if (c + 3 < N) nd = fma(w[c + 3], w[c + 3], nd);
+            }
+            const double n2 = (na + nb) + (nc + nd), x0 = w[k + 1];
+            const double nrm = n2 > 0.0 ? n2 * rsq_full(n2) : 0.0;
+            const double al = x0 < 0.0 ? nrm : -nrm;
+            const double v0l = x0 - al;
+            const double nv = fma(v0l, v0l, fma(-x0, x0, n2));
+            const double fl = nv > 0.0 ? 1.4142135623730951 * rsq_full(nv) : 0.0;
+            const double alpha = readlane_f64(al, k), v0 = readlane_f64(v0l, k), f = readlane_f64(fl, k);
             const bool below = lane > k && lane < N;
-            const double x = below ? w[k] : 0.0;
-            S.bx[lane] = x;
-            wave_sync();
-            double n2a = 0.0, n2b = 0.0;
-#pragma unroll
-            for (int c = k + 1; c < N; c += 2) {
-                const double a = S.bx[c];
-                n2a = fma(a, a, n2a);
-                if (c + 1 < N) {
-                    const double b2 = S.bx[c + 1];
-                    n2b = fma(b2, b2, n2b);
-                }
-            }
-            const double n2 = n2a + n2b, x0 = S.bx[k + 1];
-            const double nrm = sqrt(n2);
-            const double alpha = x0 < 0.0 ? nrm : -nrm;
-            const double v0 = x0 - alpha;
-            const double nv = fma(v0, v0, fma(-x0, x0, n2));
-            const double f = nv > 0.0 ? sqrt(2.0 / nv) : 0.0;
-            const double u = (lane == k + 1 ? v0 : x) * f;
+            const double u = below ? (lane == k + 1 ? v0 : w[k]) * f : 0.0;
             S.bu[lane] = u;
-            if (lane < N) S.U[k][lane] = u;
-            wave_sync();
-            double pa = 0.0, pb = 0.0;
-#pragma unroll
-            for (int c = k + 1; c < N; c += 2) {
-                pa = fma(w[c], S.bu[c], pa);
-                if (c + 1 < N) pb = fma(w[c + 1], S.bu[c + 1], pb);
-            }
-            const double p = below ? pa + pb : 0.0;
-            S.bx[lane] = u * p;
-            wave_sync();
-            double ka = 0.0, kb = 0.0;
-#pragma unroll
-            for (int c = k + 1; c < N; c += 2) {
-                ka += S.bx[c];
-                if (c + 1 < N) kb += S.bx[c + 1];
-            }
-            const double q = below ? fma(-0.5 * (ka + kb), u, p) : 0.0;
-            S.bq[lane] = q;
             if (lane == k) S.d[k] = w[k];
             if (lane == 0) S.e[k] = alpha;
             wave_sync();
+            double uc[N];
+            double pa = 0.0, pb = 0.0, pc = 0.0, pd = 0.0, qa = 0.0, qb = 0.0;
 #pragma unroll
-            for (int c = k + 1; c < N; c++) w[c] = fma(-u, S.bq[c], fma(-q, S.bu[c], w[c]));
+            for (int c = k + 1; c < N; c++) {
+                uc[c] = S.bu[c];
+                switch ((c - k - 1) & 3) {
+                    case 0: pa = fma(w[c], uc[c], pa); break;
+                    case 1: pb = fma(w[c], uc[c], pb); break;
+                    case 2: pc = fma(w[c], uc[c], pc); break;
+                    default: pd = fma(w[c], uc[c], pd); break;
+                }
+                if ((c - k - 1) & 1) qb = fma(q[c], uc[c], qb);
+                else qa = fma(q[c], uc[c], qa);
+            }
+            const double p = below ? (pa + pb) + (pc + pd) : 0.0;
+            const double qu = qa + qb;
+            S.bp[lane] = p;
+            wave_sync();
+            double pv[N];
+            double ka = 0.0, kb = 0.0, kc = 0.0, kd = 0.0;
+#pragma unroll
+            for (int c = k + 1; c < N; c++) {
+                pv[c] = S.bp[c];
+                switch ((c - k - 1) & 3) {
+                    case 0: ka = fma(uc[c], pv[c], ka); break;
+                    case 1: kb = fma(uc[c], pv[c], kb); break;
+                    case 2: kc = fma(uc[c], pv[c], kc); break;
+                    default: kd = fma(uc[c], pv[c], kd); break;
+                }
+            }
+            const double K = (ka + kb) + (kc + kd);
+            const double pp = fma(-K, u, p);
+#pragma unroll
+            for (int c = k + 1; c < N; c++) {
+                w[c] = fma(-u, pv[c], fma(-pp, uc[c], w[c]));
+                q[c] = fma(-qu, uc[c], q[c]);
+            }
             wave_sync();
         }
         if (lane == N - 2) S.d[N - 2] = w[N - 2];
         if (lane == N - 1) {
             S.d[N - 1] = w[N - 1];
             S.e[N - 2] = w[N - 2];
+        }
+        // Q replaces W in LDS (W is no longer needed).
+        if (lane < N) {
+#pragma unroll
+            for (int c = 0; c < N; c++) S.W[lane][c] = q[c];
         }
         wave_sync();
         if (lane == 0) {
@@ -600,18 +640,19 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
     // rounding, so one solve suffices), by Gaussian elimination with partial
     // pivoting applied to b as it goes (U has two superdiagonals; pivots
     // below 1e-15 |T| are raised to that size), then back substitution and
-    // normalisation.  U and the eliminated b live in LDS.
+    // normalisation, unrolled (U and the eliminated b in LDS, the rest in
+    // registers).
     if (wave == 0 && lane < kComponents) {
         const int j = lane;
         const double lam = S.lam[j];
         const double tiny = 1e-15 * fmax(fabs(S.lo), fabs(S.hi)) + 1e-300;
-        double* ur = S.f0r[j];
-        double* u1 = S.f1[j];
-        double* u2 = S.f2[j];
-        double* bb = S.X[j];
+        double* ur = S.ur[j];
+        double* u1 = S.u1[j];
+        double* u2 = S.u2[j];
+        double* bb = S.Y[j];
         auto start = [&](int i) { return 1.0 + 0.25 * (double)((i * 7 + j * 3) % 11) / 11.0; };
         double c0 = S.d[0] - lam, c1 = S.e[0], cb = start(0);
-#pragma unroll 1
+#pragma unroll
         for (int i = 0; i < N - 1; i++) {
             const double nl = S.e[i], n0 = S.d[i + 1] - lam, n1 = i + 1 < N - 1 ? S.e[i + 1] : 0.0;
             const double nb = start(i + 1);
@@ -634,7 +675,7 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
         bb[N - 1] = x1;
         bb[N - 2] = x0;
         double n2 = fma(x1, x1, x0 * x0);
-#pragma unroll 1
+#pragma unroll
         for (int i = N - 3; i >= 0; i--) {
             const double xi = fma(-u2[i], x1, fma(-u1[i], x0, bb[i])) * ur[i];
             bb[i] = xi;
@@ -642,38 +683,22 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
             x1 = x0;
             x0 = xi;
         }
-        const double rn = rsq_refined(n2);
-#pragma unroll 1
+        const double rn = rsq_full(n2);
+#pragma unroll
         for (int i = 0; i < N; i++) bb[i] *= rn;
     }
     __syncthreads();
 
-    // Back through the reflections: v = H_0 H_1 .. H_22 x (wave 0: vector
-    // lane / 8, entries lane % 8 + 8 t).
-    if (wave == 0) {
-        const int j = lane >> 3, g = lane & 7;
-        double v[4];
+    // Back to C's basis: X[j][r] = sum_c Q[r][c] Y[j][c] (threads 0 .. 199).
+    if (tid < kComponents * N) {
+        const int j = tid / N, r = tid % N;
+        double a = 0.0, b = 0.0;
 #pragma unroll
-        for (int t = 0; t < 4; t++) v[t] = g + 8 * t < N ? S.X[j][g + 8 * t] : 0.0;
-        for (int k = N - 3; k >= 0; k--) {
-            double dp = 0.0;
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int i = g + 8 * t;
-                if (i > k && i < N) dp = fma(S.U[k][i], v[t], dp);
-            }
-            dp += __shfl_xor(dp, 1);
-            dp += __shfl_xor(dp, 2);
-            dp += __shfl_xor(dp, 4);
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int i = g + 8 * t;
-                if (i > k && i < N) v[t] = fma(-S.U[k][i], dp, v[t]);
-            }
+        for (int c = 0; c < N; c += 2) {
+            a = fma(S.W[r][c], S.Y[j][c], a);
+            if (c + 1 < N) b = fma(S.W[r][c + 1], S.Y[j][c + 1], b);
         }
-#pragma unroll
-        for (int t = 0; t < 4; t++)
-            if (g + 8 * t < N) S.X[j][g + 8 * t] = v[t];
+        S.X[j][r] = a + b;
     }
     if (tid < N) S.lam[tid] *= scale;
     __syncthreads();
