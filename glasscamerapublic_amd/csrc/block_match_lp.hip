@@ -77,6 +77,12 @@ struct Prec<__bf16> {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Two 16-bit values in one word, the first in the low half.
+template <class H>
+__device__ __forceinline__ uint32_t pack2(H a, H b) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+
 __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
     float t = (x - e0) / (e1 - e0);
     t = fminf(fmaxf(t, 0.0f), 1.0f);
@@ -98,7 +104,7 @@ struct LpTile {
     static constexpr int LW = FW + 4, LH = FH + 4, LN = LW * LH;
     static constexpr int THREADS = TW * TH, WAVES = THREADS / 64, GROUPS = (FN + 15) / 16;
     static_assert(THREADS % 64 == 0, "whole wavefronts");
-    static_assert(LN * 4 + FN * 36 <= 64 * 1024, "LDS tile too large");
+    static_assert(LN * 8 + FN * 36 <= 64 * 1024, "LDS tile too large");
 };
 
 // One output tile with origin (x0, y0); every thread of the workgroup calls
@@ -113,85 +119,112 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
     using h2 = typename P::h2;
     using h8 = typename P::h8;
     constexpr bool kSplit = !std::is_same<H, _Float16>::value;
-    __shared__ H lum_hi[T::LN];                 // centred, scaled luma, rounded
-    __shared__ H lum_lo[kSplit ? T::LN : 1];    // BF16: its rounding remainder
+    // Luma pair planes: word e holds the centred, scaled, rounded luma at
+    // luma-region positions e and e + 1 of its row (low half first), so an
+    // im2col pair of neighbouring patch values is one 4-byte LDS read.
+    __shared__ uint32_t lum_pr[T::LN];
+    __shared__ uint32_t lum_pr_lo[kSplit ? T::LN : 1];  // BF16: the rounding remainders
     __shared__ float4 ycn[T::FN];  // Y, Cb, Cr, -|f_1..7|^2 / 2 (scaled, rounded)
     __shared__ float f0s[T::FN];   // component 0, FP32
     __shared__ h8 fh[T::FN];       // components 1..7 rounded, 0
     const int tid = threadIdx.y * TW + threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     // Level scale ~ 1/sigma of mid-grey luma; centre = the tile's first pixel.
     const float sref = sqrtf(nlf->a[0] + nlf->b[0] * 0.5f);
     const float scale = sref > 0.0f ? 1.0f / sref : 1.0f;
     const float mu = in.at(clampi(x0, 0, in.w - 1), clampi(y0, 0, in.h - 1)).x;
 
-    // One pass over the luma region (tile + (R+2)-pixel apron) stages both the
-    // 16-bit luma of the projection and, inside the R-pixel apron, the YCbCr.
+    // One pass over the luma region (tile + (R+2)-pixel apron) stages the
+    // luma pairs of the projection and, inside the R-pixel apron, the YCbCr.
     for (int e = tid; e < T::LN; e += T::THREADS) {
         const int ly = e / T::LW, lx = e % T::LW;
-        const float4 v = in.at(clampi(x0 - R - 2 + lx, 0, in.w - 1), clampi(y0 - R - 2 + ly, 0, in.h - 1));
-        const float yv = (v.x - mu) * scale;
-        const H hi = (H)yv;
-        lum_hi[e] = hi;
-        if (kSplit) lum_lo[e] = (H)(yv - (float)hi);
+        const int gy = clampi(y0 - R - 2 + ly, 0, in.h - 1);
+        const float4 v = in.at(clampi(x0 - R - 2 + lx, 0, in.w - 1), gy);
+        const float v1 = reinterpret_cast<const float*>(&in.at(clampi(x0 - R - 1 + lx, 0, in.w - 1), gy))[0];
+        const float yv = (v.x - mu) * scale, yv1 = (v1 - mu) * scale;
+        const H h0 = (H)yv, h1 = (H)yv1;
+        lum_pr[e] = pack2(h0, h1);
+        if (kSplit) lum_pr_lo[e] = pack2((H)(yv - (float)h0), (H)(yv1 - (float)h1));
         const int fx = lx - 2, fy = ly - 2;
         if (fx >= 0 && fx < T::FW && fy >= 0 && fy < T::FH) ycn[fy * T::FW + fx] = make_float4(v.x, v.y, v.z, 0.0f);
     }
-    // A operand of the projection (constant): the transposed basis, lane holds
-    // basis[k][i] for component i = lane % 16, k = 8 (lane / 16) + v; zero
-    // outside 25 x 8.  koff: LDS offset of patch value k from the patch's
-    // top-left corner (k >= 25 reads any valid value: its basis row is zero).
+    // A operand of the projection (constant): the transposed basis in the pair
+    // order of K.  Lane group kg = lane / 16 holds K slots 8 kg .. 8 kg + 7 =
+    // pairs d = 0 .. 3: pairs 0-2 are patch row kg, columns (0, 1), (2, 3),
+    // (4, 5), pair 3 is row 4, columns (2 kg, 2 kg + 1) for kg < 3 and zero
+    // for kg = 3 (column 5 has a zero basis row).  So every lane group reads
+    // words row + {0, 2, 4} and one more word.  Lane holds basis[k][i] for
+    // component i = lane % 16 (zero outside 25 x 8); row_off / xoff: LDS word
+    // offsets of the lane group's row and of its fourth pair from the patch's
+    // top-left corner.
     h8 aop;
-    int koff[8];
+    int row_off, xoff;
     {
         const int i = lane & 15, kg = lane >> 4;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
-            const int k = 8 * kg + v, kc = min(k, kPatch - 1);
-            aop[v] = (H)((k < kPatch && i < kComponents) ? basis[k * kComponents + i] : 0.0f);
-            koff[v] = (kc / 5) * T::LW + kc % 5;
+            const int d = v / 2, r = d < 3 ? kg : 4, c = (d < 3 ? 2 * d : 2 * kg) + (v & 1);
+            const bool valid = (d < 3 || kg < 3) && c < 5 && i < kComponents;
+            aop[v] = (H)(valid ? basis[(r * 5 + c) * kComponents + i] : 0.0f);
         }
+        row_off = kg * T::LW;
+        xoff = kg < 3 ? (4 - kg) * T::LW + 2 * kg : 0;
     }
     __syncthreads();
 
     // Projection C[component][pixel], 16 feature-region pixels per MFMA.  B
-    // operand: lane holds patch values k = 8 (lane / 16) + v of pixel
-    // lane % 16 of the group; the patch is centred on the pixel's clamped
-    // image position (clamp-to-edge, like the FP32 projection).  BF16 adds the
-    // rounding remainder of each luma value as a second product.
+    // operand: lane holds K slots 8 (lane / 16) .. + 7 of pixel lane % 16 of
+    // the group; the patch is centred on the pixel's clamped image position
+    // (clamp-to-edge, like the FP32 projection).  Interior tiles (feature
+    // region inside the image) step the pixel position incrementally.  BF16
+    // adds the rounding remainder of each luma value as a second product.
+    const bool interior =
+        x0 - R >= 0 && y0 - R >= 0 && x0 + TW + R <= in.w && y0 + TH * PV + R <= in.h;
+    constexpr int kStep = 16 * T::WAVES, kStepY = kStep / T::FW, kStepX = kStep % T::FW;
+    int ex = (16 * wave + (lane & 15)) % T::FW, ey = (16 * wave + (lane & 15)) / T::FW;
     for (int g = wave; g < T::GROUPS; g += T::WAVES) {
         const int j = lane & 15, kg = lane >> 4;
-        const int e = min(g * 16 + j, T::FN - 1);
-        const int cx = clampi(x0 - R + e % T::FW, 0, in.w - 1) - (x0 - R);
-        const int cy = clampi(y0 - R + e / T::FW, 0, in.h - 1) - (y0 - R);
-        const int base = cy * T::LW + cx;
-        h8 bhi, blo;
-#pragma unroll
-        for (int v = 0; v < 8; v++) {
-            bhi[v] = lum_hi[base + koff[v]];
-            if (kSplit) blo[v] = lum_lo[base + koff[v]];
+        const int er = g * 16 + j;
+        int base;
+        if (interior) {
+            base = er < T::FN ? ey * T::LW + ex : 0;
+        } else {
+            const int e = min(er, T::FN - 1);
+            const int cx = clampi(x0 - R + e % T::FW, 0, in.w - 1) - (x0 - R);
+            const int cy = clampi(y0 - R + e / T::FW, 0, in.h - 1) - (y0 - R);
+            base = cy * T::LW + cx;
+        }
+        ex += kStepX;
+        ey += kStepY;
+        if (ex >= T::FW) {
+            ex -= T::FW;
+            ey++;
+        }
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        u32x4 whi, wlo;
+        {
+            const uint32_t* ph = lum_pr + base + row_off;
+            whi = u32x4{ph[0], ph[2], ph[4], ph[xoff]};
+            if (kSplit) {
+                const uint32_t* pl = lum_pr_lo + base + row_off;
+                wlo = u32x4{pl[0], pl[2], pl[4], pl[xoff]};
+            }
         }
         f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-        c = P::mfma(aop, bhi, c);
-        if (kSplit) c = P::mfma(aop, blo, c);
+        c = P::mfma(aop, __builtin_bit_cast(h8, whi), c);
+        if (kSplit) c = P::mfma(aop, __builtin_bit_cast(h8, wlo), c);
         // C: lane holds components 4 (lane / 16) + r of pixel lane % 16.  Lanes
         // 0-15 keep component 0 in FP32 and write halves 0-3 = (0, c1, c2, c3);
-        // lanes 16-31 write halves 4-7 = (c4 .. c7).
-        typedef H h4 __attribute__((ext_vector_type(4)));
-        h4 hv;
-        float n = 0.0f;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const H hq = (kg == 0 && r == 0) ? (H)0.0f : (H)c[r];
-            hv[r] = hq;
-            const float fq = (float)hq;
-            n = fmaf(fq, fq, n);
-        }
-        n += __shfl_xor(n, 16);
-        const int er = g * 16 + j;
+        // lanes 16-31 write halves 4-7 = (c4 .. c7).  The squared norm of the
+        // rounded components: two dot2 per lane, then lane l + (l ^ 16).
+        const h2 p01 = {(kg == 0) ? (H)0.0f : (H)c[0], (H)c[1]}, p23 = {(H)c[2], (H)c[3]};
+        float n = P::dot2(p01, p01, P::dot2(p23, p23, 0.0f));
+        const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(n), __float_as_uint(n), false, false);
+        n = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
         if (er < T::FN && kg < 2) {
-            reinterpret_cast<h4*>(&fh[er])[kg] = hv;
+            reinterpret_cast<uint2*>(&fh[er])[kg] =
+                make_uint2(__builtin_bit_cast(uint32_t, p01), __builtin_bit_cast(uint32_t, p23));
             if (kg == 0) {
                 f0s[er] = c[0];
                 ycn[er].w = -0.5f * n;
