@@ -45,14 +45,14 @@ BURST_SIZES = [(3264, 2448), (4032, 3024), (5472, 3648), (8064, 6048)]
 # (9x9 window), levels 0..4: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
 # what the counter reads for a plain 256 MiB copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
 MEASURED_BM_TRAFFIC = {
-    ("12mp", "fp16"): [58.1, 52.1, 40.3, 40.8, 45.0],
-    ("12mp", "fp32"): [111.6, 103.7, 81.8, 80.1, 80.1],
+    ("12mp", "fp16"): [58.1, 52.1, 40.3, 40.7, 43.8],
+    ("12mp", "fp32"): [111.6, 103.7, 81.8, 79.7, 78.7],
 }
 # Wave-level VALU instructions per block-matching dispatch on the same workload, levels 0..4:
 # rocprofv3 --pmc SQ_INSTS_VALU, in millions.
 MEASURED_BM_VALU = {
-    ("12mp", "fp16"): [482.805, 120.839, 30.759, 7.722, 1.946],
-    ("12mp", "fp32"): [534.553, 133.651, 33.950, 8.532, 2.155],
+    ("12mp", "fp16"): [409.964, 102.731, 26.210, 7.081, 1.794],
+    ("12mp", "fp32"): [534.553, 133.651, 33.950, 8.639, 2.171],
 }
 DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}
 # Vector ALU: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz (a wave64 instruction takes 4 cycles).
