@@ -9,15 +9,17 @@
 //
 // 1. Luma of the tile plus an (R+2)-pixel apron is staged in LDS as 16-bit
 //    values (BF16: high part and rounding remainder), centred on the tile's
-//    first pixel and scaled by the level scale s (~1/sigma of the level);
+//    first pixel and scaled by the level scale s (~1/sigma of the level), in
+//    pair planes (word e = the values at positions e and e + 1 of a row);
 //    the same pass stages the YCbCr of the tile plus its R-pixel apron.
 //    Feature distances are invariant to the centring; the scale divides the
 //    luma coefficient by s^2.
 // 2. Projection on MFMA, transposed (F^T = B^T P^T): v_mfma_f32_16x16x32_{f16,
-//    bf16}, (8 components padded to 16) x (25 patch values padded to K = 32)
-//    x 16 pixels, FP32 accumulation, so each lane ends with 4 consecutive
-//    components of one pixel.  With BF16 (8-bit mantissa) each luma value
-//    enters as a high part plus its rounding remainder (two MFMAs).
+//    bf16}, (8 components padded to 16) x (25 patch values as 15 column pairs
+//    and a zero pair, K = 32) x 16 pixels, FP32 accumulation, so each lane
+//    ends with 4 consecutive components of one pixel; a lane's im2col is
+//    three LDS reads.  With BF16 (8-bit mantissa) each luma value enters as
+//    a high part plus its rounding remainder (two MFMAs).
 // 3. Component 0 (the largest-variance one, in practice the patch mean) spans
 //    the widest range across a tile, so its rounding error would dominate the
 //    distances: it stays FP32.  Components 1..7 are rounded to FP16/BF16 and

@@ -1,4 +1,4 @@
-// PCA patch basis on the device: patch sampling, FP64 moments, FP64 Jacobi
+// PCA patch basis on the device: patch sampling, FP64 moments, FP64
 // eigensolve and the 25 -> 8 projection, for gfx950.
 //
 //   collect_patches  MetalPipeline/demosaic.metal:755-772 (grid demosaic_kernels.hpp:388)
@@ -9,9 +9,11 @@
 // host synchronization per level (pyramid_processor.cpp:146-151).  Here the
 // basis never leaves the device: the patches are reduced straight from the
 // image into FP64 first and second moments (sum x, sum x x^T) by a fixed grid
-// of workgroups, one workgroup turns them into the covariance and runs a
-// parallel-ordered cyclic Jacobi, and the projection reads the basis from
-// device memory, so a whole denoise call stays stream-ordered.
+// of workgroups, one workgroup turns them into the covariance and finds its
+// leading eigenpairs (Householder tridiagonalisation, multisection and
+// inverse iteration, with parallel-ordered cyclic Jacobi as the checked
+// fallback), and the projection reads the basis from device memory, so a
+// whole denoise call stays stream-ordered.
 #include "gcam_internal.hpp"
 #include "burst_tables.hpp"
 
