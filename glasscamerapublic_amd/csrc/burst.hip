@@ -209,6 +209,7 @@ void fill_tables(gcam_burst* b, BurstTables* t, const gcam_denoise_params* param
                 sb.sharpening = params[f * L + i].sharpening;
                 sb.thr_y = params[f * L + i].luma;
                 sb.out = View<float4>{p->sub[i], w, h, w};
+                sb.patches = p->patches;
                 sb.bx = (w + 63) / 64;
                 sb.block0 = sub;
                 sub += blocks64x4(w, h);
@@ -216,8 +217,10 @@ void fill_tables(gcam_burst* b, BurstTables* t, const gcam_denoise_params* param
             BurstMomFrame& m = t->mom[i].frame[f];
             if (i < L - 1) {
                 m.in = View<const float4>{p->sub[i], w, h, w};
+                m.patches = p->patches;
             } else {
                 m.in = noisy;
+                m.patches = nullptr;
             }
             m.partials = p->moment_partials;
             m.nblocks = moment_block_count((int64_t)(w / 8) * (h / 8));

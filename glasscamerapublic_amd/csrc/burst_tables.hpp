@@ -30,11 +30,13 @@ struct BurstSubFrame {
     const NlfDev* nlf;  // this frame's noise model of the level
     float sharpening, thr_y;
     View<float4> out;
+    float* patches;  // PCA sample of out, (w / 8) x (h / 8) x 25 (nullptr: none)
     int bx, block0;
 };
 
 struct BurstMomFrame {
     View<const float4> in;
+    const float* patches;  // the sample of `in` written by the subtract (nullptr: sample `in`)
     double* partials;  // kMomentSlots x kMoments of this frame
     int nblocks, block0;
     double n;          // patch slots, w * h / 64

@@ -80,10 +80,13 @@ gcam_status set_error(gcam_status st, const std::string& msg);
 gcam_status launch_downsample4(View<const float4> in, View<float4> out, hipStream_t s);
 gcam_status launch_downsample2(View<const float2> in, View<float2> out, hipStream_t s);
 // nlf_level: device noise model of this level (or nullptr to use nlf_x/nlf_y as given);
-// when set, nlf = (a.x * thr_y, b.x * thr_y).
+// when set, nlf = (a.x * thr_y, b.x * thr_y).  patches (nullable): also write
+// the PCA sample of the output, (w / 8) * (h / 8) patches x 25 luma values
+// (patch py * (w / 8) + px around pixel (8 px, 8 py), clamp-to-edge).
 gcam_status launch_subtract_noise(View<const float4> in, View<const float4> in1, View<const float4> den1,
                                   View<const float2> grad, float luma_weight, float sharpening, float nlf_x,
-                                  float nlf_y, const NlfDev* nlf_level, float thr_y, View<float4> out, hipStream_t s);
+                                  float nlf_y, const NlfDev* nlf_level, float thr_y, View<float4> out, float* patches,
+                                  hipStream_t s);
 gcam_status launch_noise_stats(View<const float4> in, View<float4> out, hipStream_t s);
 gcam_status launch_nlf_fit(View<const float4> stats, float exposure_multiplier, double* partials,
                            NlfFitState* state, NlfDev* nlf_out, hipStream_t s);

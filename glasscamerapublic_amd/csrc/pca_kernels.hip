@@ -822,8 +822,11 @@ __global__ __launch_bounds__(256) void k_burst_moments(const BurstTable<BurstMom
     const View<const float4> in = f.in;
     const int gw = in.w / 8, gh = in.h / 8;
     const int64_t npatch = (int64_t)gw * gh;
-    moments_body(lb, f.nblocks, npatch, f.partials + (size_t)lb * kMoments,
-                 [&](int64_t k, int c) { return image_patch_value(in, gw, npatch, k, c); });
+    const float* patches = f.patches;
+    moments_body(lb, f.nblocks, npatch, f.partials + (size_t)lb * kMoments, [&](int64_t k, int c) {
+        if (patches) return (k < npatch && c < kPatch) ? patches[k * kPatch + c] : 0.0f;
+        return image_patch_value(in, gw, npatch, k, c);
+    });
 }
 
 __global__ __launch_bounds__(256) void k_burst_moments_reduce(const BurstTable<BurstMomFrame>* __restrict__ tab) {

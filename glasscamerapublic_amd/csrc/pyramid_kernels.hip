@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void k_downsample2(View<const float2> in, View
 // subtractNoiseImage: Laplacian-style reconstruct of level i from the noisy
 // and the denoised level i+1 (both upsampled bilinearly), with optional
 // gradient-driven sharpening.
-__device__ __forceinline__ void subtract_noise_px(const View<const float4>& in, const View<const float4>& in1,
+__device__ __forceinline__ float subtract_noise_px(const View<const float4>& in, const View<const float4>& in1,
                                                   const View<const float4>& den1, const View<const float2>& grad,
                                                   float luma_weight, float sharpening, float nlf_x, float nlf_y,
                                                   const NlfDev* nlf_level, float thr_y, const View<float4>& out,
@@ -140,16 +140,51 @@ __device__ __forceinline__ void subtract_noise_px(const View<const float4>& in, 
     o.z = d1.z + (dp.z - d1.z) * alpha;
     o.w = ip.w;
     out.at(ox, oy) = o;
+    return o.x;
+}
+
+// The PCA sample of the level written as it is produced: patch k = py * gw + px
+// (gw = w / 8, gh = h / 8) holds the 5 x 5 luma around (8 px, 8 py), value
+// f = (j + 2) * 5 + (i + 2) at pixel (8 px + i, 8 py + j) clamped to the
+// image -- the collectPatches sample, so moments of this buffer equal those
+// of the image.  Along x, a pixel with ox mod 8 = r in {0, 1, 2} is offset
+// i = r of patch column ox / 8 and one with r in {6, 7} offset r - 8 of the
+// next column; column 0 is also the clamped offsets -2 and -1 of patch
+// column 0; rows likewise.  Offsets +1, +2 never clamp (8 (gw - 1) + 2 <= w - 6).
+__device__ __forceinline__ int patch_axis_slots(int o, int g, int* cell, int* off) {
+    int n = 0;
+    const int r = o & 7;
+    if (r <= 2 && (o >> 3) < g) cell[n] = o >> 3, off[n++] = r;
+    if (r >= 6 && (o >> 3) + 1 < g) cell[n] = (o >> 3) + 1, off[n++] = r - 8;
+    if (o == 0 && g > 0) {
+        cell[n] = 0, off[n++] = -2;
+        cell[n] = 0, off[n++] = -1;
+    }
+    return n;
+}
+
+__device__ __forceinline__ void write_patch_slots(float* __restrict__ patches, int gw, int gh, int ox, int oy,
+                                                  float v) {
+    int cx[3], ix[3], cy[3], jy[3];
+    const int ny = patch_axis_slots(oy, gh, cy, jy);
+    if (ny == 0) return;
+    const int nx = patch_axis_slots(ox, gw, cx, ix);
+    for (int b = 0; b < ny; b++)
+        for (int a = 0; a < nx; a++)
+            patches[((size_t)cy[b] * gw + cx[a]) * kPatch + (jy[b] + 2) * 5 + (ix[a] + 2)] = v;
 }
 
 __global__ __launch_bounds__(256) void k_subtract_noise(View<const float4> in, View<const float4> in1,
                                                         View<const float4> den1, View<const float2> grad,
                                                         float luma_weight, float sharpening, float nlf_x,
                                                         float nlf_y, const NlfDev* nlf_level, float thr_y,
-                                                        View<float4> out) {
+                                                        View<float4> out, float* __restrict__ patches) {
     const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
-    if (ox < out.w && oy < out.h)
-        subtract_noise_px(in, in1, den1, grad, luma_weight, sharpening, nlf_x, nlf_y, nlf_level, thr_y, out, ox, oy);
+    if (ox < out.w && oy < out.h) {
+        const float y = subtract_noise_px(in, in1, den1, grad, luma_weight, sharpening, nlf_x, nlf_y, nlf_level,
+                                          thr_y, out, ox, oy);
+        if (patches) write_patch_slots(patches, out.w / 8, out.h / 8, ox, oy, y);
+    }
 }
 
 // Burst forms: one launch for the level of every frame; workgroup b covers
@@ -170,8 +205,11 @@ __global__ __launch_bounds__(256) void k_burst_subtract(const BurstTable<BurstSu
     const BurstSubFrame& f = tab->frame[burst_frame_of(tab, b)];
     const int lb = b - f.block0;
     const int ox = (lb % f.bx) * 64 + threadIdx.x, oy = (lb / f.bx) * 4 + threadIdx.y;
-    if (ox < f.out.w && oy < f.out.h)
-        subtract_noise_px(f.in, f.in1, f.den1, f.grad, 1.0f, f.sharpening, 0.0f, 0.0f, f.nlf, f.thr_y, f.out, ox, oy);
+    if (ox < f.out.w && oy < f.out.h) {
+        const float y =
+            subtract_noise_px(f.in, f.in1, f.den1, f.grad, 1.0f, f.sharpening, 0.0f, 0.0f, f.nlf, f.thr_y, f.out, ox, oy);
+        if (f.patches) write_patch_slots(f.patches, f.out.w / 8, f.out.h / 8, ox, oy, y);
+    }
 }
 
 // basicNoiseStatistics: 5x5 mean of Y and population variances of Y, Cb, Cr.
@@ -405,9 +443,9 @@ gcam_status launch_downsample2(View<const float2> in, View<float2> out, hipStrea
 gcam_status launch_subtract_noise(View<const float4> in, View<const float4> in1, View<const float4> den1,
                                   View<const float2> grad, float luma_weight, float sharpening, float nlf_x,
                                   float nlf_y, const NlfDev* nlf_level, float thr_y, View<float4> out,
-                                  hipStream_t s) {
+                                  float* patches, hipStream_t s) {
     k_subtract_noise<<<grid64x4(out.w, out.h), dim3(64, 4), 0, s>>>(in, in1, den1, grad, luma_weight, sharpening,
-                                                                    nlf_x, nlf_y, nlf_level, thr_y, out);
+                                                                    nlf_x, nlf_y, nlf_level, thr_y, out, patches);
     GCAM_LAUNCH_CHECK("subtractNoiseImage");
     return GCAM_OK;
 }

@@ -140,6 +140,37 @@ def test_image_moments_match_collected_patches():
     np.testing.assert_allclose(fused_b.numpy(), rb, atol=1e-6)
 
 
+@pytest.mark.parametrize("h,w", [(256, 384), (211, 301)])
+def test_subtracted_level_sample_moments(h, w):
+    """Below the top level the PCA sample is written by the subtract kernel as
+    it produces the level: each level's basis and eigenvalues equal
+    collect_patches + build_pca_space of the subtracted level -- bit-exact where
+    the slot table is the patch grid in order (w, h multiples of 8: same
+    patches, same moment partition), to FP64 rounding otherwise -- and the
+    oracle's."""
+    L = 3
+    img = rand_img(h, w, 12)
+    p = gcam.PyramidProcessor(w, h, levels=L, search_radius=2)
+    nlf = [((1e-5,) * 3, (1e-3,) * 3)] * L
+    p.denoise([gcam.DenoiseParameters() for _ in range(L)], to_dev(img), None, nlf)
+    subs = p.subtractedImagePyramid
+    for i in range(L - 1):
+        fb, fe = (t.numpy() for t in p.pcaSpace(i))
+        sub = subs[i].contiguous()
+        cb, ce = (to_host(t) for t in gcam.build_pca_space(gcam.collect_patches(sub)))
+        lw, lh = w >> i, h >> i
+        if lw % 8 == 0 and lh % 8 == 0:
+            np.testing.assert_array_equal(fb, cb)
+            np.testing.assert_array_equal(fe, ce)
+        else:
+            np.testing.assert_allclose(fe[:8], ce[:8], rtol=1e-12)
+            np.testing.assert_allclose(fb, cb, atol=1e-6)
+        rb, rev = oracle.build_pca_space(oracle.collect_patches(to_host(sub)))
+        np.testing.assert_allclose(fe[:8], rev[:8], rtol=1e-9)
+        np.testing.assert_allclose(fb, rb, atol=1e-6)
+    p.close()
+
+
 @pytest.mark.parametrize("h,w", [(48, 64), (37, 29)])
 def test_pca_projection_parity(h, w):
     img = rand_img(h, w, 9)
