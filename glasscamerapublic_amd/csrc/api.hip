@@ -296,8 +296,7 @@ static gcam_status denoise_impl(gcam_pyramid* p, const gcam_denoise_params* para
         int nblocks = 0;
         // Levels below the top were sampled by their subtract.
         if (i < L - 1)
-            st = launch_buffer_moments(p->patches, (int64_t)(p->W[i] / 8) * (p->H[i] / 8), p->moment_partials,
-                                       &nblocks, s);
+            st = launch_sample_moments(p->patches, p->W[i], p->H[i], p->moment_partials, &nblocks, s);
         else
             st = launch_image_moments(layer, p->moment_partials, &nblocks, s);
         if (st != GCAM_OK) return st;

@@ -130,6 +130,24 @@ __global__ __launch_bounds__(256) void k_image_moments(View<const float4> in, do
                  [&](int64_t k, int f) { return image_patch_value(in, gw, npatch, k, f); });
 }
 
+// Value f of patch k (px = k % gw, py = k / gw) of a sample written by the
+// subtract kernel (row planes: value (i, j) at ((py * 5 + j) * gw + px) * 5 + i,
+// i, j in 0..4), the same value image_patch_value reads from the level.
+__device__ __forceinline__ float plane_patch_value(const float* __restrict__ s, int gw, int64_t npatch, int64_t k,
+                                                   int f) {
+    if (k >= npatch || f >= kPatch) return 0.0f;
+    const int kk = (int)k;  // npatch <= 16384^2 / 64
+    const int py = kk / gw, px = kk - py * gw;
+    const int j = f / 5, i = f - 5 * j;
+    return s[((size_t)(py * 5 + j) * gw + px) * 5 + i];
+}
+
+__global__ __launch_bounds__(256) void k_sample_moments(const float* sample, int gw, int gh, double* partials) {
+    const int64_t npatch = (int64_t)gw * gh;
+    moments_body(blockIdx.x, gridDim.x, npatch, partials + (size_t)blockIdx.x * kMoments,
+                 [&](int64_t k, int f) { return plane_patch_value(sample, gw, npatch, k, f); });
+}
+
 __global__ __launch_bounds__(256) void k_buffer_moments(const float* patches, int64_t n, double* partials) {
     moments_body(blockIdx.x, gridDim.x, n, partials + (size_t)blockIdx.x * kMoments, [&](int64_t k, int f) {
         return (k < n && f < kPatch) ? patches[k * kPatch + f] : 0.0f;
@@ -824,7 +842,7 @@ __global__ __launch_bounds__(256) void k_burst_moments(const BurstTable<BurstMom
     const int64_t npatch = (int64_t)gw * gh;
     const float* patches = f.patches;
     moments_body(lb, f.nblocks, npatch, f.partials + (size_t)lb * kMoments, [&](int64_t k, int c) {
-        if (patches) return (k < npatch && c < kPatch) ? patches[k * kPatch + c] : 0.0f;
+        if (patches) return plane_patch_value(patches, gw, npatch, k, c);
         return image_patch_value(in, gw, npatch, k, c);
     });
 }
@@ -894,6 +912,13 @@ static int moment_blocks(int64_t npatch) {
 gcam_status launch_image_moments(View<const float4> in, double* partials, int* nblocks, hipStream_t s) {
     *nblocks = moment_blocks((int64_t)(in.w / 8) * (in.h / 8));
     k_image_moments<<<*nblocks, 256, 0, s>>>(in, partials);
+    GCAM_LAUNCH_CHECK("patch moments");
+    return GCAM_OK;
+}
+
+gcam_status launch_sample_moments(const float* sample, int w, int h, double* partials, int* nblocks, hipStream_t s) {
+    *nblocks = moment_blocks((int64_t)(w / 8) * (h / 8));
+    k_sample_moments<<<*nblocks, 256, 0, s>>>(sample, w / 8, h / 8, partials);
     GCAM_LAUNCH_CHECK("patch moments");
     return GCAM_OK;
 }

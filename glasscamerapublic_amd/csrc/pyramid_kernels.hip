@@ -143,11 +143,12 @@ __device__ __forceinline__ float subtract_noise_px(const View<const float4>& in,
     return o.x;
 }
 
-// The PCA sample of the level written as it is produced: patch k = py * gw + px
-// (gw = w / 8, gh = h / 8) holds the 5 x 5 luma around (8 px, 8 py), value
-// f = (j + 2) * 5 + (i + 2) at pixel (8 px + i, 8 py + j) clamped to the
-// image -- the collectPatches sample, so moments of this buffer equal those
-// of the image.  Along x, a pixel with ox mod 8 = r in {0, 1, 2} is offset
+// The PCA sample of the level written as it is produced: patch (px, py)
+// (px < gw = w / 8, py < gh = h / 8) is the 5 x 5 luma around (8 px, 8 py),
+// offset (i, j) in -2..2 at pixel (8 px + i, 8 py + j) clamped to the image --
+// the collectPatches sample.  Stored in row planes: value (i, j) of patch
+// (px, py) at ((py * 5 + j + 2) * gw + px) * 5 + i + 2, so the 40 values a
+// wavefront's 64-pixel row holds are one contiguous 160-byte run.  Along x, a pixel with ox mod 8 = r in {0, 1, 2} is offset
 // i = r of patch column ox / 8 and one with r in {6, 7} offset r - 8 of the
 // next column; column 0 is also the clamped offsets -2 and -1 of patch
 // column 0; rows likewise.  Offsets +1, +2 never clamp (8 (gw - 1) + 2 <= w - 6).
@@ -171,7 +172,7 @@ __device__ __forceinline__ void write_patch_slots(float* __restrict__ patches, i
     const int nx = patch_axis_slots(ox, gw, cx, ix);
     for (int b = 0; b < ny; b++)
         for (int a = 0; a < nx; a++)
-            patches[((size_t)cy[b] * gw + cx[a]) * kPatch + (jy[b] + 2) * 5 + (ix[a] + 2)] = v;
+            patches[((size_t)(cy[b] * 5 + jy[b] + 2) * gw + cx[a]) * 5 + ix[a] + 2] = v;
 }
 
 __global__ __launch_bounds__(256) void k_subtract_noise(View<const float4> in, View<const float4> in1,
