@@ -165,6 +165,9 @@ __device__ void moments_reduce_body(int blk, const double* partials, int nblocks
     const int e = blk * kRedEntries + t % kRedEntries, sub = t / kRedEntries;
     if (sub < kRedLanes) {
         double acc = 0.0;
+        // Unrolled so that the loads of 8 partials are in flight together; the
+        // additions stay in index order.
+#pragma unroll 8
         for (int b = sub; b < nblocks; b += kRedLanes) acc += partials[(size_t)b * kMoments + e];
         part[sub][t % kRedEntries] = acc;
     }
