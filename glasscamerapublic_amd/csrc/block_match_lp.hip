@@ -51,6 +51,7 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 template <class H>
 struct Prec;
@@ -59,8 +60,8 @@ template <>
 struct Prec<_Float16> {
     using h2 = f16x2;
     using h8 = f16x8;
-    static __device__ __forceinline__ f32x4 mfma(h8 a, h8 b, f32x4 c) {
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    static __device__ __forceinline__ f32x16 mfma(h8 a, h8 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
     }
     static __device__ __forceinline__ float dot2(h2 a, h2 b, float c) { return __builtin_amdgcn_fdot2(a, b, c, false); }
 };
@@ -69,8 +70,8 @@ template <>
 struct Prec<__bf16> {
     using h2 = bf16x2;
     using h8 = bf16x8;
-    static __device__ __forceinline__ f32x4 mfma(h8 a, h8 b, f32x4 c) {
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    static __device__ __forceinline__ f32x16 mfma(h8 a, h8 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
     }
     static __device__ __forceinline__ float dot2(h2 a, h2 b, float c) {
         return __builtin_amdgcn_fdot2_f32_bf16(a, b, c, false);
@@ -104,7 +105,7 @@ template <int R, int PV, int TW, int TH>
 struct LpTile {
     static constexpr int FW = TW + 2 * R, FH = TH * PV + 2 * R, FN = FW * FH;
     static constexpr int LW = FW + 4, LH = FH + 4, LN = LW * LH;
-    static constexpr int THREADS = TW * TH, WAVES = THREADS / 64, GROUPS = (FN + 15) / 16;
+    static constexpr int THREADS = TW * TH, WAVES = THREADS / 64, GROUPS = (FN + 31) / 32;
     static_assert(THREADS % 64 == 0, "whole wavefronts");
     static_assert(LN * 8 + FN * 36 <= 64 * 1024, "LDS tile too large");
 };
@@ -151,43 +152,48 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
         const int fx = lx - 2, fy = ly - 2;
         if (fx >= 0 && fx < T::FW && fy >= 0 && fy < T::FH) ycn[fy * T::FW + fx] = make_float4(v.x, v.y, v.z, 0.0f);
     }
-    // A operand of the projection (constant): the transposed basis in the pair
-    // order of K.  Lane group kg = lane / 16 holds K slots 8 kg .. 8 kg + 7 =
-    // pairs d = 0 .. 3: pairs 0-2 are patch row kg, columns (0, 1), (2, 3),
-    // (4, 5), pair 3 is row 4, columns (2 kg, 2 kg + 1) for kg < 3 and zero
-    // for kg = 3 (column 5 has a zero basis row).  So every lane group reads
-    // words row + {0, 2, 4} and one more word.  Lane holds basis[k][i] for
-    // component i = lane % 16 (zero outside 25 x 8); row_off / xoff: LDS word
-    // offsets of the lane group's row and of its fourth pair from the patch's
-    // top-left corner.
-    h8 aop;
-    int row_off, xoff;
+    // A operands of the projection (constant): the transposed basis in the
+    // pair order of K, split over two v_mfma_f32_32x32x16 (K = 16 each).  K
+    // slice kg (of 4, 8 slots each) = pairs d = 0 .. 3: pairs 0-2 are patch
+    // row kg, columns (0, 1), (2, 3), (4, 5), pair 3 is row 4, columns (2 kg,
+    // 2 kg + 1) for kg < 3 and zero for kg = 3 (column 5 has a zero basis row).
+    // Lane half h = lane / 32 holds slice kg = h in the first product and
+    // kg = 2 + h in the second; lane holds basis[k][i] for component
+    // i = lane % 32 (zero outside 25 x 8).  ro / xo: LDS word offsets of the
+    // slice's row and of its fourth pair from the patch's top-left corner.
+    h8 aop[2];
+    int ro[2], xo[2];
     {
-        const int i = lane & 15, kg = lane >> 4;
+        const int i = lane & 31, h = lane >> 5;
 #pragma unroll
-        for (int v = 0; v < 8; v++) {
-            const int d = v / 2, r = d < 3 ? kg : 4, c = (d < 3 ? 2 * d : 2 * kg) + (v & 1);
-            const bool valid = (d < 3 || kg < 3) && c < 5 && i < kComponents;
-            aop[v] = (H)(valid ? basis[(r * 5 + c) * kComponents + i] : 0.0f);
+        for (int q = 0; q < 2; q++) {
+            const int kg = 2 * q + h;
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                const int d = v / 2, r = d < 3 ? kg : 4, c = (d < 3 ? 2 * d : 2 * kg) + (v & 1);
+                const bool valid = (d < 3 || kg < 3) && c < 5 && i < kComponents;
+                aop[q][v] = (H)(valid ? basis[(r * 5 + c) * kComponents + i] : 0.0f);
+            }
+            ro[q] = kg * T::LW;
+            xo[q] = kg < 3 ? 4 * T::LW + 2 * kg : kg * T::LW;  // kg = 3: a zero basis pair, any word
         }
-        row_off = kg * T::LW;
-        xoff = kg < 3 ? (4 - kg) * T::LW + 2 * kg : 0;
     }
     __syncthreads();
 
-    // Projection C[component][pixel], 16 feature-region pixels per MFMA.  B
-    // operand: lane holds K slots 8 (lane / 16) .. + 7 of pixel lane % 16 of
-    // the group; the patch is centred on the pixel's clamped image position
-    // (clamp-to-edge, like the FP32 projection).  Interior tiles (feature
-    // region inside the image) step the pixel position incrementally.  BF16
-    // adds the rounding remainder of each luma value as a second product.
+    // Projection C[component][pixel], 32 feature-region pixels per pair of
+    // v_mfma_f32_32x32x16.  B operands: lane holds the K slice of its half (as
+    // in the A operands) of pixel lane % 32 of the group; the patch is centred
+    // on the pixel's clamped image position (clamp-to-edge, like the FP32
+    // projection).  Interior tiles (feature region inside the image) step the
+    // pixel position incrementally.  BF16 adds the rounding remainder of each
+    // luma value as further products.
     const bool interior =
         x0 - R >= 0 && y0 - R >= 0 && x0 + TW + R <= in.w && y0 + TH * PV + R <= in.h;
-    constexpr int kStep = 16 * T::WAVES, kStepY = kStep / T::FW, kStepX = kStep % T::FW;
-    int ex = (16 * wave + (lane & 15)) % T::FW, ey = (16 * wave + (lane & 15)) / T::FW;
+    constexpr int kStep = 32 * T::WAVES, kStepY = kStep / T::FW, kStepX = kStep % T::FW;
+    int ex = (32 * wave + (lane & 31)) % T::FW, ey = (32 * wave + (lane & 31)) / T::FW;
     for (int g = wave; g < T::GROUPS; g += T::WAVES) {
-        const int j = lane & 15, kg = lane >> 4;
-        const int er = g * 16 + j;
+        const int j = lane & 31, h = lane >> 5;
+        const int er = g * 32 + j;
         int base;
         if (interior) {
             base = er < T::FN ? ey * T::LW + ex : 0;
@@ -204,30 +210,31 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
             ey++;
         }
         typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-        u32x4 whi, wlo;
-        {
-            const uint32_t* ph = lum_pr + base + row_off;
-            whi = u32x4{ph[0], ph[2], ph[4], ph[xoff]};
+        f32x16 c = {};
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const uint32_t* ph = lum_pr + base + ro[q];
+            const u32x4 whi = u32x4{ph[0], ph[2], ph[4], lum_pr[base + xo[q]]};
+            c = P::mfma(aop[q], __builtin_bit_cast(h8, whi), c);
             if (kSplit) {
-                const uint32_t* pl = lum_pr_lo + base + row_off;
-                wlo = u32x4{pl[0], pl[2], pl[4], pl[xoff]};
+                const uint32_t* pl = lum_pr_lo + base + ro[q];
+                const u32x4 wlo = u32x4{pl[0], pl[2], pl[4], lum_pr_lo[base + xo[q]]};
+                c = P::mfma(aop[q], __builtin_bit_cast(h8, wlo), c);
             }
         }
-        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-        c = P::mfma(aop, __builtin_bit_cast(h8, whi), c);
-        if (kSplit) c = P::mfma(aop, __builtin_bit_cast(h8, wlo), c);
-        // C: lane holds components 4 (lane / 16) + r of pixel lane % 16.  Lanes
-        // 0-15 keep component 0 in FP32 and write halves 0-3 = (0, c1, c2, c3);
-        // lanes 16-31 write halves 4-7 = (c4 .. c7).  The squared norm of the
-        // rounded components: two dot2 per lane, then lane l + (l ^ 16).
-        const h2 p01 = {(kg == 0) ? (H)0.0f : (H)c[0], (H)c[1]}, p23 = {(H)c[2], (H)c[3]};
+        // C: registers 0-3 of lane l hold components 4 (l / 32) + r of pixel
+        // l % 32 (rows 8..31 are the zero basis rows).  Lanes 0-31 keep
+        // component 0 in FP32 and write halves 0-3 = (0, c1, c2, c3); lanes
+        // 32-63 write halves 4-7 = (c4 .. c7).  The squared norm of the rounded
+        // components: two dot2 per lane, then lane l + (l ^ 32).
+        const h2 p01 = {(h == 0) ? (H)0.0f : (H)c[0], (H)c[1]}, p23 = {(H)c[2], (H)c[3]};
         float n = P::dot2(p01, p01, P::dot2(p23, p23, 0.0f));
-        const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(n), __float_as_uint(n), false, false);
+        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(n), __float_as_uint(n), false, false);
         n = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
-        if (er < T::FN && kg < 2) {
-            reinterpret_cast<uint2*>(&fh[er])[kg] =
+        if (er < T::FN) {
+            reinterpret_cast<uint2*>(&fh[er])[h] =
                 make_uint2(__builtin_bit_cast(uint32_t, p01), __builtin_bit_cast(uint32_t, p23));
-            if (kg == 0) {
+            if (h == 0) {
                 f0s[er] = c[0];
                 ycn[er].w = -0.5f * n;
             }
