@@ -45,13 +45,13 @@ BURST_SIZES = [(3264, 2448), (4032, 3024), (5472, 3648), (8064, 6048)]
 # (9x9 window), levels 0..4: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
 # what the counter reads for a plain 256 MiB copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
 MEASURED_BM_TRAFFIC = {
-    ("12mp", "fp16"): [58.1, 52.1, 40.3, 40.7, 43.8],
+    ("12mp", "fp16"): [58.1, 52.1, 40.3, 40.7, 43.9],
     ("12mp", "fp32"): [111.6, 103.7, 81.8, 79.7, 78.7],
 }
 # Wave-level VALU instructions per block-matching dispatch on the same workload, levels 0..4:
 # rocprofv3 --pmc SQ_INSTS_VALU, in millions.
 MEASURED_BM_VALU = {
-    ("12mp", "fp16"): [409.964, 102.731, 26.210, 7.081, 1.794],
+    ("12mp", "fp16"): [395.661, 99.062, 25.232, 6.754, 1.704],
     ("12mp", "fp32"): [534.553, 133.651, 33.950, 8.639, 2.171],
 }
 DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}

@@ -14,12 +14,14 @@
 //    the same pass stages the YCbCr of the tile plus its R-pixel apron.
 //    Feature distances are invariant to the centring; the scale divides the
 //    luma coefficient by s^2.
-// 2. Projection on MFMA, transposed (F^T = B^T P^T): v_mfma_f32_16x16x32_{f16,
-//    bf16}, (8 components padded to 16) x (25 patch values as 15 column pairs
-//    and a zero pair, K = 32) x 16 pixels, FP32 accumulation, so each lane
-//    ends with 4 consecutive components of one pixel; a lane's im2col is
-//    three LDS reads.  With BF16 (8-bit mantissa) each luma value enters as
-//    a high part plus its rounding remainder (two MFMAs).
+// 2. Projection on MFMA, transposed (F^T = B^T P^T): two
+//    v_mfma_f32_32x32x16_{f16,bf16}, (8 components padded to 32) x (25 patch
+//    values as 15 column pairs and a zero pair, K = 2 x 16) x 32 pixels, FP32
+//    accumulation, so every lane ends with 4 consecutive components of one
+//    pixel (with 16 x 16 x 32 products, 16 pixels per product, half the lanes
+//    would hold only the padding rows); a lane's im2col is four LDS reads per
+//    product.  With BF16 (8-bit mantissa) each luma value enters as a high
+//    part plus its rounding remainder (twice the products).
 // 3. Component 0 (the largest-variance one, in practice the patch mean) spans
 //    the widest range across a tile, so its rounding error would dominate the
 //    distances: it stays FP32.  Components 1..7 are rounded to FP16/BF16 and
