@@ -3,23 +3,28 @@
 
 One step = one whole ``PyramidProcessor::denoise`` call (5-level pyramid,
 5x5 patches -> 8 PCA components, 9x9 search window) over one synthetic 12 MP
-frame already resident in HBM.  The reported form is the precision form
-(FP16 MFMA projection and FP16 distance products, FP32 accumulation); the
-plain FP32 form is timed beside it on the same frame, and the PSNR loss of the
-reported form against the FP64 oracle is measured on that frame.  Prints ONE
-JSON line with the throughput in Mpix/s of the base level (K timed steps with
-no instrumentation), the roofline of the dominant kernel (block matching,
+frame already resident in HBM, at the iPhone 14 Wide high-ISO denoise preset
+(chroma multiplier 0.25), where the denoise raises the frame's PSNR.  The
+reported form is the precision form (FP16 MFMA projection and FP16 distance
+products, FP32 accumulation); the plain FP32 form is timed beside it on the
+same frame, and the PSNR loss of the reported form against the FP64 oracle is
+measured on that frame.  Prints ONE JSON line with the throughput in Mpix/s of
+the base level (K timed steps with no instrumentation; per-step median, min and
+max beside the mean), the roofline of the dominant kernel (block matching,
 timed over 5 further steps with HIP events the library records around each
-stage on the stream it launches on) and the CPU oracle timed on this host's
-cores on the same frame.
+stage on the stream it launches on), matrix-core utilisation, and the CPU
+oracle timed on this host's cores on the same frame.  Every timed leg starts
+after a time-based warm-up, and comparisons between forms use median steps.
 
     python bench.py [--steps 20] [--warmup 3] [--precision fp16] [--no-cpu-baseline]
-    python bench.py --workload burst32      # the 32-frame mixed-size burst (batched form)
+    python bench.py --workload burst32      # only the 32-frame mixed-size burst (batched form)
+    python bench.py --workload copy         # 256 MiB device copy: the HBM counter calibration
 
-The raw front end that produces the denoise's inputs from the frame's Bayer
-mosaic (RawFrontEnd) is timed beside it under "front_end" (--no-front-end to
-skip), and burst fusion (registerAndFuse of the frame, FindHomography on 1500
-matches) under "registration" (--no-registration to skip).
+Beside the denoise the default run times the raw front end that produces its
+inputs from the frame's Bayer mosaic (RawFrontEnd, "front_end"; --no-front-end
+to skip), burst fusion (registerAndFuse of the frame, FindHomography on 1500
+matches: "registration"; --no-registration) and the 32-frame burst of the
+batched form ("burst32"; --no-burst).
 """
 from __future__ import annotations
 
@@ -54,12 +59,45 @@ MEASURED_BM_VALU = {
     ("12mp", "fp16"): [395.661, 99.062, 25.232, 6.754, 1.704],
     ("12mp", "fp32"): [534.553, 133.651, 33.950, 8.639, 2.171],
 }
+# Matrix-core work per dispatch on the same workload, levels 0..4 (rocprofv3 --pmc
+# SQ_INSTS_VALU_MFMA_MOPS_F16 / _F64, which count flops / 512): the FP16 projection inside the precision
+# form's block matching (v_mfma_f32_32x32x16_f16) and the FP64 patch moments (v_mfma_f64_16x16x4f64,
+# levels 0-3 in k_sample_moments, level 4 in k_image_moments).
+MEASURED_MFMA_MOPS = {
+    ("12mp", "fp16"): {"block_match_f16": [91445760, 22982400, 5898240, 1966080, 491520],
+                       "moments_f64": [786432, 286080, 71424, 18048, 4608]},
+}
+MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0, "f64": 78.6}
 DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}
 # Vector ALU: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz (a wave64 instruction takes 4 cycles).
 VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 # Issue slots per (pixel, window position) of the reference's inner loop: ~39 VALU operations and
 # two exponentials at quarter rate.
 MODEL_SLOTS_PER_PAIR = 47
+# Host CPU threads one GPU's job may use on the benchmark machines (their CPU share per GPU).
+BOX_CPU_SHARE = 16
+
+
+def mfma_block(workload, precision, stages_mean5, levels):
+    """Matrix-core utilisation: measured MFMA flops per dispatch over the live per-level stage durations
+    (block matching for the FP16 projection, the moment kernel for the FP64 moments), against the dense
+    MFMA peaks."""
+    m = MEASURED_MFMA_MOPS.get((workload, precision))
+    if m is None:
+        return None
+    out = {}
+    for name, key, dt in (("block_match_f16_projection", "block_match", "f16"), ("moments_f64", "moments", "f64")):
+        mops = m["block_match_f16" if dt == "f16" else "moments_f64"]
+        ms = stages_mean5.get(key, [0.0] * levels)
+        flops = [512.0 * x for x in mops[:levels]]
+        rates = [f / (t / 1e3) / 1e12 if t > 0 else None for f, t in zip(flops, ms)]
+        tot_t = sum(ms[:levels])
+        agg = sum(flops) / (tot_t / 1e3) / 1e12 if tot_t > 0 else None
+        out[name] = {"tflops_per_level": [round(r, 3) if r else None for r in rates],
+                     "tflops": round(agg, 3) if agg else None, "peak_tflops": MFMA_PEAK_TFLOPS[dt],
+                     "frac": round(agg / MFMA_PEAK_TFLOPS[dt], 5) if agg else None,
+                     "gflop_per_frame": round(sum(flops) / 1e9, 3)}
+    return out
 
 
 def algorithmic_bytes_per_base_pixel(levels: int) -> float:
@@ -82,6 +120,25 @@ def psnr(a, b) -> float:
     return 10.0 * math.log10(1.0 / mse)
 
 
+def step_stats(step_ms):
+    return {"median_ms": round(statistics.median(step_ms), 4), "min_ms": round(min(step_ms), 4),
+            "max_ms": round(max(step_ms), 4), "mean_ms": round(sum(step_ms) / len(step_ms), 4)}
+
+
+def settle(fn, warmup, min_s=0.3):
+    """W warmup calls, then more until at least min_s of wall time has passed with the GPU busy, so that
+    a timed leg that follows an idle gap (allocation, host work) does not start at ramping clocks."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    while time.perf_counter() - t < min_s:
+        for _ in range(4):
+            fn()
+        torch.cuda.synchronize()
+
+
 def time_single(gcam, torch, w, h, levels, radius, precision, bm_rows, ycc, params, nlf, steps, warmup, dev):
     """K timed steps of one processor, then a few steps with the library's stage
     events on (HIP events it records around every stage on its stream) for the
@@ -89,8 +146,7 @@ def time_single(gcam, torch, w, h, levels, radius, precision, bm_rows, ycc, para
     stream = torch.cuda.current_stream()
     proc = gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius, precision=precision, device=dev,
                                  bm_rows_per_thread=bm_rows)
-    for _ in range(warmup):
-        proc.denoise(params, ycc, None, nlf)
+    settle(lambda: proc.denoise(params, ycc, None, nlf), warmup)
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -102,19 +158,19 @@ def time_single(gcam, torch, w, h, levels, radius, precision, bm_rows, ycc, para
     wall = time.perf_counter() - t0
     step_ms = [evs[k].elapsed_time(evs[k + 1]) for k in range(steps)]
     proc.set_profiling(True)
-    bm = []
+    runs = []
     for _ in range(5):
         proc.denoise(params, ycc, None, nlf)
-        st = proc.stage_times()
-        bm.append(st["block_match"])
-    stages = proc.stage_times()
-    stages["block_match_mean5"] = [sum(x[i] for x in bm) / len(bm) for i in range(levels)]
+        runs.append(proc.stage_times())
+    stages = runs[-1]
+    mean5 = {k: [sum(r[k][i] for r in runs) / len(runs) for i in range(levels)] for k in stages}
     proc.set_profiling(False)
     out = proc.denoisedImagePyramid[0].clone()
     proc.close()
     total_ms = evs[0].elapsed_time(evs[-1])
-    return {"ms_per_step": total_ms / steps, "median_ms": statistics.median(step_ms),
-            "wall_ms_per_step": wall * 1e3 / steps, "stages": stages, "out": out}
+    return {"ms_per_step": total_ms / steps, "step_stats": step_stats(step_ms),
+            "wall_ms_per_step": wall * 1e3 / steps, "stages": stages, "stages_mean5": mean5,
+            "stage_sum_ms": sum(sum(v) for k, v in mean5.items() if k != "moments"), "out": out}
 
 
 # HBM bytes per pixel of the front end's stage DAG, each stage's inputs read
@@ -237,23 +293,40 @@ def time_registration(gcam, torch, ycc, w, h, steps, warmup, cpu_baseline):
     return out
 
 
-def single_frame(args) -> dict:
-    import numpy as np
-    import torch
+def preset_params(gcam, levels):
+    """iPhone 14 Wide denoise preset at high ISO (iPhone14WideCalibration.cpp:75-111 at nlf_alpha = 1: luma
+    3, 1.5, 1, 1, 1 x 2; chromaBoost 8, 4, 2, 2, 2; gradientBoost 2, 1, 0..; gradientThreshold 2, 2, 1..;
+    sharpening 1) with the chroma multiplier at 0.25: the synthetic scene's chroma carries the same
+    high-frequency texture as its luma (independent simplex octaves per RGB channel), which the preset's
+    chroma multiplier of 1 would average away.  At this point the denoise raises the frame's PSNR on all
+    three channels (no gradient image: the gradient terms are inert)."""
+    lm, cb = [3.0, 1.5, 1.0, 1.0, 1.0], [8.0, 4.0, 2.0, 2.0, 2.0]
+    gb, gt = [2.0, 1.0, 0.0, 0.0, 0.0], [2.0, 2.0, 1.0, 1.0, 1.0]
+    return [gcam.DenoiseParameters(luma=2.0 * lm[min(i, 4)], chroma=0.25, chromaBoost=cb[min(i, 4)],
+                                   gradientBoost=gb[min(i, 4)], gradientThreshold=gt[min(i, 4)], sharpening=1.0)
+            for i in range(levels)]
 
-    import glasscamerapublic_amd as gcam
 
-    w, h, levels, radius, a, b = WORKLOADS[args.workload]
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
+PRESET_NAME = ("iPhone 14 Wide high-ISO preset (luma 6, 3, 2, 2, 2; chromaBoost 8, 4, 2, 2, 2), chroma 0.25, "
+               "no gradient image")
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def calibrate(gcam, torch, w, h, levels, radius, params, ycc, dev):
+    """MeasureYCbCrNLF on every level of the frame (the in-pipeline calibration); returns the noise model
+    and the fit's own time (a calibrating call minus a plain one)."""
     stream = torch.cuda.current_stream()
-
-    ycc, raw, clean = gcam.synth_frame(w, h, a, b, scene_seed=1234, noise_seed=5678, device=dev, want_raw=True,
-                                       want_clean=True)
-    params = [gcam.DenoiseParameters() for _ in range(levels)]
     nlf = [((1e-8,) * 3, (1e-8,) * 3) for _ in range(levels)]
-
-    # Noise model: MeasureYCbCrNLF on every level, once per frame, outside the timed region.
     cal = gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius, device=dev)
     cal.denoise(params, ycc, None, nlf, calibrateFromImage=True)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -266,8 +339,26 @@ def single_frame(args) -> dict:
     cal.denoise(params, ycc, None, nlf)
     e1.record(stream)
     torch.cuda.synchronize()
-    nlf_fit_ms = max(0.0, calib_ms - e0.elapsed_time(e1))
+    fit_ms = max(0.0, calib_ms - e0.elapsed_time(e1))
     cal.close()
+    return nlf, fit_ms
+
+
+def single_frame(args) -> dict:
+    import numpy as np
+    import torch
+
+    import glasscamerapublic_amd as gcam
+
+    w, h, levels, radius, a, b = WORKLOADS[args.workload]
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+
+    ycc, raw, clean = gcam.synth_frame(w, h, a, b, scene_seed=1234, noise_seed=5678, device=dev, want_raw=True,
+                                       want_clean=True)
+    params = preset_params(gcam, levels)
+    # Noise model: MeasureYCbCrNLF on every level, once per frame, outside the timed region.
+    nlf, nlf_fit_ms = calibrate(gcam, torch, w, h, levels, radius, params, ycc, dev)
 
     prim = time_single(gcam, torch, w, h, levels, radius, args.precision, args.bm_rows, ycc, params, nlf,
                        args.steps, args.warmup, dev)
@@ -283,7 +374,7 @@ def single_frame(args) -> dict:
     # Dominant kernel: block matching, one launch per level.  Algorithmic bytes
     # per launch = 24 B (3 ch x FP32 in + 3 ch x FP32 out) per level pixel.
     level_px = [(w >> i) * (h >> i) for i in range(levels)]
-    bm_ms = prim["stages"].pop("block_match_mean5")
+    bm_ms = prim["stages_mean5"]["block_match"]
     bm_avg_ms = sum(bm_ms) / levels
     bm_bytes_per_launch = 24.0 * sum(level_px) / levels
     bm_gbs = bm_bytes_per_launch / (bm_avg_ms / 1e3) / 1e9
@@ -312,23 +403,31 @@ def single_frame(args) -> dict:
         host_nlf = np.array([[list(A), list(B)] for A, B in nlf], np.float64)
         pr = [dict(luma=p.luma, chroma=p.chroma, chromaBoost=p.chromaBoost, gradientBoost=p.gradientBoost,
                    gradientThreshold=p.gradientThreshold, sharpening=p.sharpening) for p in params]
-        threads = args.cpu_threads or min(cpu_cores(), 16)
+        host_cpus = cpu_cores()
+        threads = args.cpu_threads or min(host_cpus, BOX_CPU_SHARE)
         oracle.set_threads(threads)
         t = time.perf_counter()
         oracle.pyramid_denoise(host_img, pr, host_nlf, levels=levels, radius=radius)
         cpu_s = time.perf_counter() - t
         cpu = {"value": round(mpix / cpu_s, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
+               "host_cpus": host_cpus, "cpu_model": cpu_model(),
                "sample": f"one full {w}x{h} frame (the benchmark frame), {levels} levels, "
-                         f"{2 * radius + 1}x{2 * radius + 1} window, FP32 C++ oracle, {threads} threads, "
+                         f"{2 * radius + 1}x{2 * radius + 1} window, FP32 C++ oracle, {threads} threads "
+                         f"(the host's CPU share for one GPU: at most {BOX_CPU_SHARE} of its {host_cpus} CPUs), "
                          f"{cpu_s:.1f} s"}
         if not args.no_quality:
             ref64 = oracle.pyramid_denoise(host_img, pr, host_nlf, levels=levels, radius=radius,
                                            dtype=np.float64).denoised[0]
             got = prim["out"].cpu().numpy()
             p64, pgpu = psnr(ref64, host_clean), psnr(got, host_clean)
-            quality = {"psnr_fp64_oracle_db": round(p64, 4), f"psnr_{args.precision}_db": round(pgpu, 4),
-                       "psnr_loss_db": round(p64 - pgpu, 4), "psnr_noisy_db": round(psnr(host_img, host_clean), 4),
+            quality = {"params": PRESET_NAME, "psnr_noisy_db": round(psnr(host_img, host_clean), 4),
+                       "psnr_fp64_oracle_db": round(p64, 4), f"psnr_{args.precision}_db": round(pgpu, 4),
+                       "psnr_loss_db": round(p64 - pgpu, 4),
                        "max_abs_vs_fp64": float(np.abs(got[..., :3] - ref64[..., :3]).max())}
+            if ref32 is not None:
+                got32 = ref32["out"].cpu().numpy()
+                quality["psnr_fp32_db"] = round(psnr(got32, host_clean), 4)
+                quality["fp32_max_abs_vs_fp64"] = float(np.abs(got32[..., :3] - ref64[..., :3]).max())
 
     out = {
         "metric": METRIC,
@@ -348,7 +447,7 @@ def single_frame(args) -> dict:
                         f"-> YCbCr, {levels}-level pyramid, 5x5 patches -> 8 PCA comps, "
                         f"{2 * radius + 1}x{2 * radius + 1} window",
             "width": w, "height": h, "levels": levels, "search_radius": radius, "precision": args.precision,
-            "parallelism": "single-gpu",
+            "denoise_parameters": PRESET_NAME, "parallelism": "single-gpu",
         },
         "roofline": {
             "bound": "hbm",
@@ -364,18 +463,25 @@ def single_frame(args) -> dict:
         "valu": valu,
         "pipeline": {"B_alg_per_base_px": round(b_alg, 3), "achieved_GBs": round(pipeline_gbs, 2),
                      "hbm_frac": round(pipeline_gbs / HBM_PEAK_GBS, 5)},
-        "median_ms": round(prim["median_ms"], 4),
+        "step_stats": prim["step_stats"],
         "wall_ms_per_step": round(prim["wall_ms_per_step"], 4),
+        "stage_sum_ms": round(prim["stage_sum_ms"], 4),
         "nlf_fit_ms": round(nlf_fit_ms, 3),
-        "stage_ms_last_step": {k: [round(x, 4) for x in v] for k, v in prim["stages"].items()},
+        "stage_ms_mean5": {k: [round(x, 4) for x in v] for k, v in prim["stages_mean5"].items()},
         "cpu_baseline": cpu,
     }
+    mf = mfma_block(args.workload, args.precision, prim["stages_mean5"], levels)
+    if mf is not None:
+        out["mfma"] = mf
     if ref32 is not None:
-        v32 = mpix / (ref32["ms_per_step"] / 1e3)
-        ref32["stages"].pop("block_match_mean5")
-        out["fp32_form"] = {"value": round(v32, 3), "ms_per_step": round(ref32["ms_per_step"], 4),
-                            "speedup_of_reported_form": round(value / v32, 4),
-                            "stage_ms_last_step": {k: [round(x, 4) for x in v] for k, v in ref32["stages"].items()}}
+        med16, med32 = prim["step_stats"]["median_ms"], ref32["step_stats"]["median_ms"]
+        out["fp32_form"] = {"value": round(mpix / (ref32["ms_per_step"] / 1e3), 3),
+                            "value_median": round(mpix / (med32 / 1e3), 3),
+                            "ms_per_step": round(ref32["ms_per_step"], 4), "step_stats": ref32["step_stats"],
+                            "stage_sum_ms": round(ref32["stage_sum_ms"], 4),
+                            "speedup_of_reported_form": round(med32 / med16, 4),
+                            "speedup_basis": "median step times",
+                            "stage_ms_mean5": {k: [round(x, 4) for x in v] for k, v in ref32["stages_mean5"].items()}}
     if quality is not None:
         out["quality"] = quality
     if not args.no_front_end:
@@ -387,10 +493,21 @@ def single_frame(args) -> dict:
     if not args.no_registration:
         out["registration"] = time_registration(gcam, torch, ycc, w, h, args.steps, args.warmup,
                                                 not args.no_cpu_baseline)
+    del ycc, raw, clean
+    torch.cuda.empty_cache()
+    if not args.no_burst and args.workload == "12mp":
+        bst = burst(args, steps=min(args.steps, 10), warmup=min(args.warmup, 2))
+        bst.pop("metric")
+        for k in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):
+            bst.pop(k, None)
+        if ref32 is not None:
+            bst["vs_fp32_form"] = round(bst["value_median"] / out["fp32_form"]["value_median"], 4)
+            bst["vs_fp32_form_basis"] = "burst median Mpix/s over the FP32 form's median Mpix/s on the 12 MP frame"
+        out["burst32"] = bst
     return out
 
 
-def burst(args) -> dict:
+def burst(args, steps=None, warmup=None) -> dict:
     """The batched form: 32 frames, sizes drawn uniformly (seed 42) from four camera sizes, noise gain a
     log-uniform in [1e-3, 1e-2]; one step = one gcam_burst_denoise call (one graph launch)."""
     import numpy as np
@@ -398,15 +515,15 @@ def burst(args) -> dict:
 
     import glasscamerapublic_amd as gcam
 
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     levels, radius, b = 5, 4, 2e-3
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
-    rng = np.random.default_rng(42)
-    n = args.burst_frames
-    sizes = [BURST_SIZES[int(k)] for k in rng.integers(0, len(BURST_SIZES), n)]
-    gains = np.exp(rng.uniform(np.log(1e-3), np.log(1e-2), n))
-    params = [gcam.DenoiseParameters() for _ in range(levels)]
+    sizes, gains = burst_plan(args.burst_frames)
+    n = len(sizes)
+    params = preset_params(gcam, levels)
     frames, nlfs = [], []
     cal = {}
     for f, ((w, h), a) in enumerate(zip(sizes, gains)):
@@ -427,19 +544,18 @@ def burst(args) -> dict:
     bp.denoise(params, frames, nlfs)  # autotune + graph capture + first run
     torch.cuda.synchronize()
     first_call_s = time.perf_counter() - t
-    for _ in range(args.warmup):
-        bp.denoise(params, frames, nlfs)
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    settle(lambda: bp.denoise(params, frames, nlfs), warmup)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     evs[0].record(stream)
-    for k in range(args.steps):
+    for k in range(steps):
         bp.denoise(params, frames, nlfs)
         evs[k + 1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    step_ms = [evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)]
-    ms_per_step = evs[0].elapsed_time(evs[-1]) / args.steps
+    step_ms = [evs[k].elapsed_time(evs[k + 1]) for k in range(steps)]
+    ms_per_step = evs[0].elapsed_time(evs[-1]) / steps
     mpix = sum(w * h for w, h in sizes) / 1e6
     value = mpix / (ms_per_step / 1e3)
     tun = bp.tuning
@@ -448,14 +564,17 @@ def burst(args) -> dict:
     level_px = [sum((w >> i) * (h >> i) for w, h in sizes) for i in range(levels)]
     bm_gbs = [24.0 * level_px[i] / (tun["level_ms"][i] / 1e3) / 1e9 if tun["level_ms"][i] > 0 else None
               for i in range(levels)]
+    stats = step_stats(step_ms)
     out = {
         "metric": "Mpix/s, burst of 32 mixed-size frames, 5-level PCA denoise, one graph launch",
         "value": round(value, 3),
+        "value_median": round(mpix / (stats["median_ms"] / 1e3), 3),
         "unit": "Mpix/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": round(ms_per_step, 3),
+        "step_stats": stats,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -465,7 +584,7 @@ def burst(args) -> dict:
                                f"{', '.join(f'{w}x{h}' for w, h in BURST_SIZES)}, a log-uniform [1e-3, 1e-2], "
                                f"b={b:g}, {levels} levels, {2 * radius + 1}x{2 * radius + 1} window",
                    "frames": n, "total_mpix": round(mpix, 3), "precision": args.precision,
-                   "parallelism": "single-gpu"},
+                   "denoise_parameters": PRESET_NAME, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "kernel": f"k_burst_block_match_lp<{radius}> (level 0)",
                      "achieved": round(bm_gbs[0], 2) if bm_gbs[0] else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bm_gbs[0] / HBM_PEAK_GBS, 5) if bm_gbs[0] else None, "traffic": None},
@@ -473,12 +592,49 @@ def burst(args) -> dict:
                      "hbm_frac": round(pipeline_gbs / HBM_PEAK_GBS, 5)},
         "tuning": tun,
         "first_call_s": round(first_call_s, 3),
-        "median_ms": round(statistics.median(step_ms), 3),
-        "wall_ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "wall_ms_per_step": round(wall * 1e3 / steps, 3),
         "cpu_baseline": None,
     }
     bp.close()
+    del frames
+    torch.cuda.empty_cache()
     return out
+
+
+def burst_plan(n):
+    """Frame sizes (uniform over the four camera sizes) and noise gains (log-uniform in [1e-3, 1e-2]) of the
+    n-frame burst, seed 42."""
+    import numpy as np
+    rng = np.random.default_rng(42)
+    sizes = [BURST_SIZES[int(k)] for k in rng.integers(0, len(BURST_SIZES), n)]
+    gains = np.exp(rng.uniform(np.log(1e-3), np.log(1e-2), n))
+    return sizes, gains
+
+
+def copy_calibration(args) -> dict:
+    """A plain device-to-device copy of 256 MiB (torch's copy kernel on the current stream), the reference
+    point that scales the FETCH_SIZE / WRITE_SIZE counters of the denoise kernels to bytes: under
+    rocprofv3 --pmc each dispatch reads and writes exactly 256 MiB.  Also reports the copy's bandwidth."""
+    import torch
+    n = 256 * (1 << 20) // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    settle(lambda: dst.copy_(src), args.warmup, min_s=0.1)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        dst.copy_(src)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbs = 2 * n * 4 / (ms / 1e3) / 1e9
+    return {"metric": "GB/s, device copy of 256 MiB (counter calibration)", "value": round(gbs, 1), "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "copy 256 MiB", "bytes_read": n * 4, "bytes_written": n * 4},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": 2 * n * 4}}
 
 
 def main() -> int:
@@ -486,7 +642,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="12mp", choices=sorted(WORKLOADS) + ["burst32"])
+    ap.add_argument("--workload", default="12mp", choices=sorted(WORKLOADS) + ["burst32", "copy"])
     ap.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16"])
     ap.add_argument("--bm-rows", type=int, default=0, help="block-matching output rows per thread (0 = default)")
     ap.add_argument("--burst-frames", type=int, default=32)
@@ -496,14 +652,18 @@ def main() -> int:
     ap.add_argument("--no-registration", action="store_true",
                     help="skip timing burst registration (registerAndFuse, FindHomography) beside the denoise")
     ap.add_argument("--no-quality", action="store_true", help="skip the PSNR check against the FP64 oracle")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: min(cores, 16))")
+    ap.add_argument("--no-burst", action="store_true", help="skip the 32-frame burst leg of the default run")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help=f"oracle threads (default: the host's CPUs, at most {BOX_CPU_SHARE})")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "run with --gpus 1", file=sys.stderr)
         return 2
-    if args.workload == "burst32":
+    if args.workload == "copy":
+        out = copy_calibration(args)
+    elif args.workload == "burst32":
         if args.precision == "fp32":
             print("the burst form runs the precision form: --precision fp16 or bf16", file=sys.stderr)
             return 2

@@ -23,7 +23,7 @@ GCAM_PIXEL_R16U = 5
 
 GCAM_PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 GCAM_LEVEL_DENOISED, GCAM_LEVEL_SUBTRACTED, GCAM_LEVEL_NOISY, GCAM_LEVEL_FEATURES = 0, 1, 2, 3
-STAGES = ("downsample", "nlf", "subtract", "pca", "block_match")
+STAGES = ("downsample", "nlf", "subtract", "pca", "block_match", "moments")
 
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTED = (

@@ -5,7 +5,9 @@ batch of crops with shared per-level, per-channel thresholds (levels x 3:
 Y, Cb, Cr) and returns the stacked (N, H, W, 3) outputs; autograd carries
 gradients back to ``thresholds`` through the hand-written HIP adjoint (match
 weights recomputed, the PCA basis differentiated too).  The images are inputs
-of the forward only: no gradient flows to them.
+of the forward only: no gradient flows to them.  Each processor keeps the state
+of its last forward only, so backward raises if a processor ran another
+forward between this forward and its backward.
 """
 from __future__ import annotations
 
@@ -24,6 +26,7 @@ class _DenoiseThresholds(torch.autograd.Function):
             out = proc.denoise_fwd(params, thresholds.detach().cpu(), img, nl)
             outs.append(out[..., :3].clone())
         ctx.processors = processors
+        ctx.generations = [proc.generation for proc in processors]
         ctx.thr_device = thresholds.device
         return torch.stack(outs)
 
@@ -33,7 +36,7 @@ class _DenoiseThresholds(torch.autograd.Function):
         for k, proc in enumerate(ctx.processors):
             g = torch.zeros(grad_out.shape[1:3] + (4,), dtype=torch.float32, device=grad_out.device)
             g[..., :3] = grad_out[k]
-            gk = proc.denoise_bwd(g)
+            gk = proc.denoise_bwd(g, generation=ctx.generations[k])
             total = gk if total is None else total + gk
         return total.to(ctx.thr_device), None, None, None, None
 

@@ -295,11 +295,13 @@ static gcam_status denoise_impl(gcam_pyramid* p, const gcam_denoise_params* para
         const int64_t n = ((int64_t)p->W[i] * p->H[i]) / 64;  // :142
         int nblocks = 0;
         // Levels below the top were sampled by their subtract.
+        GCAM_MARK(i, GCAM_STAGE_MOMENTS, 0);
         if (i < L - 1)
             st = launch_sample_moments(p->patches, p->W[i], p->H[i], p->moment_partials, &nblocks, s);
         else
             st = launch_image_moments(layer, p->moment_partials, &nblocks, s);
         if (st != GCAM_OK) return st;
+        GCAM_MARK(i, GCAM_STAGE_MOMENTS, 1);
         st = launch_pca_solve(p->moment_partials, nblocks, n, p->basis + 200 * i, p->eigvals + 25 * i,
                               keep_eig ? p->eig_full + kEigStride * i : nullptr, s);
         if (st != GCAM_OK) return st;

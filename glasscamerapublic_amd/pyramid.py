@@ -104,6 +104,9 @@ class PyramidProcessor:
             check(lib().gcam_pyramid_create(self.width, self.height, self.levels, ctypes.byref(cfg),
                                             ctypes.byref(handle)))
         self._h = handle
+        # Counts the calls that overwrite the handle's per-call state (denoise, denoise_fwd); a reverse
+        # pass records the count of its forward and refuses to run after another forward.
+        self.generation = 0
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -147,6 +150,7 @@ class PyramidProcessor:
         img = as_image(image, GCAM_PIXEL_RGBA32F)
         grad = as_image(gradientImage, GCAM_PIXEL_RG32F)
         out = Image()
+        self.generation += 1
         with torch.cuda.device(self.device):
             check(lib().gcam_pyramid_denoise(self._h, P, ctypes.byref(img),
                                              ctypes.byref(grad) if grad is not None else None, N,
@@ -196,14 +200,22 @@ class PyramidProcessor:
         P, N = self._params_nlf(denoiseParameters, nlfParameters)
         th = (ctypes.c_float * (3 * self.levels))(*[float(v) for v in torch.as_tensor(thresholds).reshape(-1)])
         out = Image()
+        self.generation += 1
         with torch.cuda.device(self.device):
             check(lib().gcam_pyramid_denoise_fwd(self._h, P, th, ctypes.byref(as_image(image, GCAM_PIXEL_RGBA32F)),
                                                  N, ctypes.c_void_p(_stream(stream)), ctypes.byref(out)))
         self._fwd_image = image
         return _wrap(out, 4, self.device)
 
-    def denoise_bwd(self, grad_output: torch.Tensor, stream=None) -> torch.Tensor:
-        """dL/dthresholds (levels x 3, float32 CPU tensor) of the last denoise_fwd, given dL/d(output)."""
+    def denoise_bwd(self, grad_output: torch.Tensor, stream=None, generation=None) -> torch.Tensor:
+        """dL/dthresholds (levels x 3, float32 CPU tensor) of the last denoise_fwd, given dL/d(output).
+
+        generation: the processor's ``generation`` right after the forward being differentiated; if
+        another denoise or denoise_fwd ran on this processor since, the saved forward state is gone and
+        this raises instead of differentiating the later call."""
+        if generation is not None and generation != self.generation:
+            raise RuntimeError(f"denoise_bwd: the processor ran {self.generation - generation} more forward "
+                               "call(s) since the forward being differentiated; its saved state is overwritten")
         g = (ctypes.c_float * (3 * self.levels))()
         grad_output = grad_output.contiguous()
         with torch.cuda.device(self.device):

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define GCAM_ABI_VERSION 1
+#define GCAM_ABI_VERSION 2
 #define GCAM_MAX_LEVELS 8
 #define GCAM_MAX_SEARCH_RADIUS 10
 #define GCAM_MAX_BURST 64
@@ -152,7 +152,8 @@ typedef enum gcam_stage {
     GCAM_STAGE_SUBTRACT = 2,    /* subtractNoiseImage                       */
     GCAM_STAGE_PCA = 3,         /* patch moments + eigensolve + projection  */
     GCAM_STAGE_BLOCK_MATCH = 4, /* blockMatchingDenoiseImage                */
-    GCAM_STAGE_COUNT = 5
+    GCAM_STAGE_MOMENTS = 5,     /* the patch-moment kernel alone (inside GCAM_STAGE_PCA) */
+    GCAM_STAGE_COUNT = 6
 } gcam_stage;
 
 gcam_status gcam_pyramid_set_profiling(gcam_pyramid* pyramid, int32_t enable);

@@ -19,14 +19,15 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libgcam_oracle.so")
+# GCAM_ORACLE_LIB selects another build of the same sources (the Makefile's sanitizer build).
+_LIB_PATH = os.environ.get("GCAM_ORACLE_LIB") or os.path.join(_HERE, "libgcam_oracle.so")
 _lib = None
 
 
 def build() -> str:
     """Compile the oracle with its Makefile; returns the library path."""
     subprocess.run(["make", "-s", "-C", _HERE, "-B", "libgcam_oracle.so"], check=True)
-    return _LIB_PATH
+    return os.path.join(_HERE, "libgcam_oracle.so")
 
 
 def lib() -> ctypes.CDLL:

@@ -48,3 +48,12 @@ def nlf_for(levels, a=4e-3, b=2e-3):
         out[i, 0] = np.array([b * b, 0.3 * b * b, 0.3 * b * b]) * s + 1e-8
         out[i, 1] = np.array([0.6 * a, 0.25 * a, 0.25 * a]) * s
     return out
+
+
+def preset_params(levels):
+    """The benchmark's denoise parameters as dicts: the iPhone 14 Wide high-ISO preset
+    (iPhone14WideCalibration.cpp:75-111 at nlf_alpha = 1) with the chroma multiplier at 0.25."""
+    lm, cb = [3.0, 1.5, 1.0, 1.0, 1.0], [8.0, 4.0, 2.0, 2.0, 2.0]
+    gb, gt = [2.0, 1.0, 0.0, 0.0, 0.0], [2.0, 2.0, 1.0, 1.0, 1.0]
+    return [dict(luma=2.0 * lm[min(i, 4)], chroma=0.25, chromaBoost=cb[min(i, 4)], gradientBoost=gb[min(i, 4)],
+                 gradientThreshold=gt[min(i, 4)], sharpening=1.0) for i in range(levels)]

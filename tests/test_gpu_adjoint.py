@@ -111,6 +111,24 @@ def test_autograd_batch_of_crops():
     np.testing.assert_allclose(thr.grad.numpy(), expect, rtol=1e-5, atol=1e-6 * np.abs(expect).max())
 
 
+def test_backward_after_another_forward_raises():
+    """Each processor keeps only its last forward: a second forward (a validation pass, say) between a
+    forward and its backward must make the backward raise, not differentiate the later call."""
+    w, h, levels = 256, 256, 3
+    frames = [gcam.synth_frame(w, h, 5e-3, 2e-3, scene_seed=400 + k, noise_seed=500 + k) for k in range(2)]
+    nlf = [[(tuple(x[0]), tuple(x[1])) for x in nlf_for(levels, 5e-3, 2e-3)] for _ in range(2)]
+    procs = [gcam.PyramidProcessor(w, h, levels=levels, search_radius=2) for _ in range(2)]
+    thr = torch.tensor(THR5[:levels], requires_grad=True)
+    out = denoise_thresholds(thr, frames, procs, params_for(levels), nlf)
+    with torch.no_grad():
+        denoise_thresholds(thr, frames[::-1], procs, params_for(levels), nlf)  # overwrites both processors
+    with pytest.raises(RuntimeError, match="more forward"):
+        out.sum().backward()
+    out = denoise_thresholds(thr, frames, procs, params_for(levels), nlf)
+    out.sum().backward()  # a backward right after its forward still works
+    assert torch.isfinite(thr.grad).all()
+
+
 def test_unsupported_settings_are_rejected():
     proc = gcam.PyramidProcessor(256, 256, levels=3, search_radius=2)
     ycc = gcam.synth_frame(256, 256, 5e-3, 2e-3)

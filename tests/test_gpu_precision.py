@@ -33,8 +33,9 @@ def camera_params(levels):
                  sharpening=1.0) for i in range(levels)]
 
 
+@pytest.mark.parametrize("radius", [1, 4, 5, 7, 10])
 @pytest.mark.parametrize("precision,tol", [("fp16", 5e-3), ("bf16", 3e-2)])
-def test_block_match_lp_close_to_fp64(precision, tol):
+def test_block_match_lp_close_to_fp64(precision, tol, radius):
     w, h = 96, 64
     ycc = gcam.synth_frame(w, h, 4e-3, 2e-3, scene_seed=3, noise_seed=4)
     img = host(ycc)
@@ -43,18 +44,43 @@ def test_block_match_lp_close_to_fp64(precision, tol):
     args = dict(var_a=[4e-6, 1e-6, 1e-6], var_b=[2.4e-3, 1e-3, 1e-3], thresholdMultipliers=[4.0, 2.0, 2.0],
                 chromaBoost=2.0)
     ref = oracle.block_match(img, feat, args["var_a"], args["var_b"], args["thresholdMultipliers"],
-                             args["chromaBoost"], 0.0, 1.0, 0.0, 4)
-    got = host(gcam.block_matching_denoise_image_lp(ycc, torch.from_numpy(basis.astype(np.float32)), search_radius=4,
-                                                    precision=precision, **args))
+                             args["chromaBoost"], 0.0, 1.0, 0.0, radius)
+    got = host(gcam.block_matching_denoise_image_lp(ycc, torch.from_numpy(basis.astype(np.float32)),
+                                                    search_radius=radius, precision=precision, **args))
     err = np.abs(got[..., :3] - ref[..., :3])
-    print(f"{precision}: max-abs {err.max():.3g} mean-abs {err.mean():.3g}")
+    print(f"{precision} r{radius}: max-abs {err.max():.3g} mean-abs {err.mean():.3g}")
     assert err.max() < tol
     assert err.mean() < tol / 10
 
 
+@pytest.mark.parametrize("rows", [1, 2, 4])
+@pytest.mark.parametrize("radius", [1, 5, 7, 10])
+@pytest.mark.parametrize("precision,tol", [("fp16", 5e-3), ("bf16", 3e-2)])
+def test_lp_tile_shapes_vs_fp64(precision, tol, radius, rows, oracle_threads):
+    """Every tile shape of the precision-form kernel (one, two or four output rows per thread for
+    radius <= 6; 32 x 4 threads, one row each, for radius 7..10, which ignores rows) through the
+    whole two-level pyramid, against the FP64 oracle on both levels."""
+    w, h, levels = 288, 200, 2
+    ycc = gcam.synth_frame(w, h, 4e-3, 2e-3, scene_seed=11 + radius, noise_seed=12 + rows)
+    prm = camera_params(levels)
+    nlf = [(tuple(x[0]), tuple(x[1])) for x in nlf_for(levels)]
+    proc = gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius, precision=precision,
+                                 bm_rows_per_thread=rows)
+    proc.denoise([gcam.DenoiseParameters(**p) for p in prm], ycc, None, list(nlf))
+    got = [host(t) for t in proc.denoisedImagePyramid]
+    proc.close()
+    ref = oracle.pyramid_denoise(host(ycc), prm, np.array(nlf), levels=levels, radius=radius, dtype=np.float64)
+    for i in range(levels):
+        err = np.abs(got[i][..., :3] - ref.denoised[i][..., :3])
+        print(f"{precision} r{radius} rows {rows} level {i}: max-abs {err.max():.3g} mean-abs {err.mean():.3g}")
+        assert err.max() < tol
+        assert err.mean() < tol / 10
+
+
+@pytest.mark.parametrize("radius", [4, 10])
 @pytest.mark.parametrize("precision,max_loss_db", [("fp16", 0.05), ("bf16", 0.5)])
-def test_pyramid_psnr_loss_vs_fp64(precision, max_loss_db, oracle_threads):
-    w, h, levels, radius = 1024, 768, 5, 4
+def test_pyramid_psnr_loss_vs_fp64(precision, max_loss_db, radius, oracle_threads):
+    w, h, levels = 1024, 768, 5
     ycc, clean = gcam.synth_frame(w, h, 4e-3, 2e-3, scene_seed=61, noise_seed=62, want_clean=True)
     prm = camera_params(levels)
     nlf = [(tuple(x[0]), tuple(x[1])) for x in nlf_for(levels)]
