@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgcam_denoise.so")
 
 GCAM_OK = 0
+GCAM_ERR_INVALID_ARGUMENT = 1
 GCAM_PIXEL_RGBA32F = 0
 GCAM_PIXEL_RG32F = 1
 GCAM_PIXEL_R32F = 2
@@ -31,7 +32,7 @@ EXPORTED = (
     "gcam_pyramid_create", "gcam_pyramid_destroy", "gcam_pyramid_denoise", "gcam_pyramid_level",
     "gcam_pyramid_basis", "gcam_pyramid_set_profiling", "gcam_pyramid_stage_times",
     "gcam_pyramid_denoise_fwd", "gcam_pyramid_denoise_bwd", "gcam_downsample_image", "gcam_subtract_noise_image", "gcam_collect_patches",
-    "gcam_build_pca_space", "gcam_pca_projection", "gcam_block_matching_denoise_image",
+    "gcam_build_pca_space", "gcam_build_pca_space_noalloc", "gcam_pca_projection", "gcam_block_matching_denoise_image",
     "gcam_block_matching_denoise_image_lp", "gcam_denoise_image",
     "gcam_basic_noise_statistics", "gcam_measure_ycbcr_nlf", "gcam_synth_frame",
     "gcam_burst_create", "gcam_burst_destroy", "gcam_burst_denoise", "gcam_burst_output", "gcam_burst_tune",
@@ -131,6 +132,7 @@ def _declare(L: ctypes.CDLL) -> None:
     L.gcam_subtract_noise_image.argtypes = [_IP, _IP, _IP, _IP, _F, _F, _F, _F, _IP, _P]
     L.gcam_collect_patches.argtypes = [_IP, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), _P]
     L.gcam_build_pca_space.argtypes = [_P, ctypes.c_int64, _P, _P, _P]
+    L.gcam_build_pca_space_noalloc.argtypes = [_P, ctypes.c_int64, _P, _P, _P]
     L.gcam_pca_projection.argtypes = [_IP, _P, _IP, _P]
     L.gcam_block_matching_denoise_image.argtypes = [_IP, _IP, _IP, _P, _P, _P, _F, _F, _F, _F, _I, _IP, _P]
     L.gcam_block_matching_denoise_image_lp.argtypes = [_IP, _IP, _P, _P, _P, _P, _F, _F, _F, _F, _I, _I, _IP, _P]

@@ -259,6 +259,19 @@ gcam_status gcam_collect_patches(const gcam_image* input, float* patches, int64_
  * device 25 x 8 floats; eigenvalues: device 25 doubles or NULL. */
 gcam_status gcam_build_pca_space(const float* patches, int64_t n, float* basis, double* eigenvalues, void* stream);
 
+/* No counterpart in the reference: gcam_build_pca_space for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results; the
+ * moment partials live in a fixed device array of the library (2.87 MB per
+ * GPU, held until the process ends) instead of a per-call allocation, and no
+ * call allocates.  n must be 2 .. GCAM_PCA_NOALLOC_MAX_PATCHES.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other streams or
+ * host threads) race on it. */
+#define GCAM_PCA_NOALLOC_MAX_PATCHES ((int64_t)1 << 24)
+#define GCAM_PCA_NOALLOC_SCRATCH_BYTES ((size_t)1025 * 350 * 8)
+gcam_status gcam_build_pca_space_noalloc(const float* patches, int64_t n, float* basis, double* eigenvalues,
+                                         void* stream);
+
 /* pcaProjectionKernel (demosaic_kernels.hpp:393-408), demosaic.metal:774-796.
  * basis: device 25 x 8 floats.  features: GCAM_PIXEL_PCA8_F32. */
 gcam_status gcam_pca_projection(const gcam_image* input, const float* basis, gcam_image* features, void* stream);
