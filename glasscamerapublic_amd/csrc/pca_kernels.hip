@@ -30,19 +30,23 @@ __device__ __forceinline__ double sign_ref(int r) { return 1.0 + (double)((r * r
 
 // Row/column of moment entry e: entries 0..24 are sum x_r (r = e, s = -1),
 // entries 25.. enumerate the upper triangle r <= s row by row.
+// Row r of triangle entry t = e - 25 is the largest r with start(r) <= t,
+// start(r) = 25 r - r (r - 1) / 2 (the entries of rows 0 .. r - 1): the root
+// of the quadratic, corrected by one step either way.
 __device__ __forceinline__ void moment_rs(int e, int& r, int& s) {
     if (e < kPatch) {
         r = e;
         s = -1;
         return;
     }
-    int t = e - kPatch;
-    r = 0;
-    while (t >= kPatch - r) {
-        t -= kPatch - r;
-        r++;
-    }
-    s = r + t;
+    const int t = e - kPatch;
+    auto start = [](int q) { return kPatch * q - q * (q - 1) / 2; };
+    int q = (int)((51.0f - sqrtf(2601.0f - 8.0f * (float)t)) * 0.5f);
+    q = q < 0 ? 0 : (q > kPatch - 1 ? kPatch - 1 : q);
+    if (q < kPatch - 1 && start(q + 1) <= t) q++;
+    if (start(q) > t) q--;
+    r = q;
+    s = q + (t - start(q));
 }
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -761,7 +765,8 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
 // The 8 largest eigenpairs, largest first (PCA.cpp:49-55), are written as
 // basis[r * 8 + c]; ties in eigenvalue keep index order; each vector is
 // oriented to a non-negative dot product with sign_ref.
-__device__ void pca_solve_body(const double* moments, double n, float* basis, double* eigvals, double* eig_full) {
+__device__ __forceinline__ void pca_solve_body(const double* moments, double n, float* basis, double* eigvals,
+                                               double* eig_full) {
     constexpr int N = kPatch;
     constexpr int LD = kLD;
     __shared__ double mom[kMoments];

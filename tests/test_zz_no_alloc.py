@@ -35,7 +35,8 @@ def run(entry, p, n):
 
 def test_noalloc_equals_allocating_entry_point():
     L = _lib.lib()
-    for n, seed in ((200_000, 1), (3_001, 2)):  # the second call has fewer patches
+    # each call has fewer patches than the one before
+    for n, seed in ((200_000, 1), (3_001, 2), (700, 4)):
         p = patches(n, seed)
         st_a, basis_a, ev_a = run(L.gcam_build_pca_space, p, n)
         st_b, basis_b, ev_b = run(L.gcam_build_pca_space_noalloc, p, n)
