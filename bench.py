@@ -23,8 +23,9 @@ after a time-based warm-up, and comparisons between forms use median steps.
 Beside the denoise the default run times the raw front end that produces its
 inputs from the frame's Bayer mosaic (RawFrontEnd, "front_end"; --no-front-end
 to skip), burst fusion (registerAndFuse of the frame, FindHomography on 1500
-matches: "registration"; --no-registration) and the 32-frame burst of the
-batched form ("burst32"; --no-burst).
+matches: "registration"; --no-registration), the reference-CPU configuration
+(1024x1024, 4 levels, 7x7: both forms and the CPU oracle, "c1"; --no-c1) and
+the 32-frame burst of the batched form ("burst32"; --no-burst).
 """
 from __future__ import annotations
 
@@ -495,6 +496,8 @@ def single_frame(args) -> dict:
                                                 not args.no_cpu_baseline)
     del ycc, raw, clean
     torch.cuda.empty_cache()
+    if not args.no_c1 and args.workload == "12mp":
+        out["c1"] = c1_block(args)
     if not args.no_burst and args.workload == "12mp":
         bst = burst(args, steps=min(args.steps, 10), warmup=min(args.warmup, 2))
         bst.pop("metric")
@@ -504,6 +507,53 @@ def single_frame(args) -> dict:
             bst["vs_fp32_form"] = round(bst["value_median"] / out["fp32_form"]["value_median"], 4)
             bst["vs_fp32_form_basis"] = "burst median Mpix/s over the FP32 form's median Mpix/s on the 12 MP frame"
         out["burst32"] = bst
+    return out
+
+
+def c1_block(args) -> dict:
+    """The reference-CPU configuration: one 1024x1024 frame (a=2e-3, b=1e-3), 4 levels, 7x7 window, both GPU
+    forms and the CPU oracle on the same frame (the whole frame, the host's CPU share)."""
+    import numpy as np
+    import torch
+
+    import glasscamerapublic_amd as gcam
+
+    w, h, levels, radius, a, b = WORKLOADS["1mp"]
+    dev = torch.device("cuda", 0)
+    ycc, clean = gcam.synth_frame(w, h, a, b, scene_seed=1234, noise_seed=5678, device=dev, want_clean=True)
+    params = preset_params(gcam, levels)
+    nlf, _ = calibrate(gcam, torch, w, h, levels, radius, params, ycc, dev)
+    mpix = w * h / 1e6
+    out = {"workload": f"{w}x{h} RGGB 12-bit simplex scene, Poisson-Gaussian a={a:g} b={b:g}, {levels} levels, "
+                       f"{2 * radius + 1}x{2 * radius + 1} window", "unit": "Mpix/s"}
+    for prec in ("fp16", "fp32"):
+        r = time_single(gcam, torch, w, h, levels, radius, prec, 0, ycc, params, nlf, args.steps, args.warmup, dev)
+        out[prec] = {"value_median": round(mpix / (r["step_stats"]["median_ms"] / 1e3), 2),
+                     "step_stats": r["step_stats"]}
+        if prec == "fp16":
+            b_alg = algorithmic_bytes_per_base_pixel(levels)
+            gbs = b_alg * w * h / (r["step_stats"]["median_ms"] / 1e3) / 1e9
+            out["pipeline"] = {"B_alg_per_base_px": round(b_alg, 3), "achieved_GBs": round(gbs, 2),
+                               "hbm_frac": round(gbs / HBM_PEAK_GBS, 5)}
+    if not args.no_cpu_baseline:
+        import oracle
+        host_img = ycc.cpu().numpy()
+        host_nlf = np.array([[list(A), list(B)] for A, B in nlf], np.float64)
+        pr = [dict(luma=p.luma, chroma=p.chroma, chromaBoost=p.chromaBoost, gradientBoost=p.gradientBoost,
+                   gradientThreshold=p.gradientThreshold, sharpening=p.sharpening) for p in params]
+        host_cpus = cpu_cores()
+        threads = args.cpu_threads or min(host_cpus, BOX_CPU_SHARE)
+        oracle.set_threads(threads)
+        reps, t = 0, time.perf_counter()
+        while True:  # at least one frame and about a second of CPU work
+            oracle.pyramid_denoise(host_img, pr, host_nlf, levels=levels, radius=radius)
+            reps += 1
+            if time.perf_counter() - t > 1.0:
+                break
+        cpu_s = (time.perf_counter() - t) / reps
+        out["cpu_baseline"] = {"value": round(mpix / cpu_s, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
+                               "host_cpus": host_cpus, "cpu_model": cpu_model(),
+                               "sample": f"the whole {w}x{h} frame x {reps}, FP32 C++ oracle, {threads} threads"}
     return out
 
 
@@ -653,6 +703,8 @@ def main() -> int:
                     help="skip timing burst registration (registerAndFuse, FindHomography) beside the denoise")
     ap.add_argument("--no-quality", action="store_true", help="skip the PSNR check against the FP64 oracle")
     ap.add_argument("--no-burst", action="store_true", help="skip the 32-frame burst leg of the default run")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the 1024x1024, 4-level, 7x7 leg (both forms and the CPU oracle) of the default run")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help=f"oracle threads (default: the host's CPUs, at most {BOX_CPU_SHARE})")
     args = ap.parse_args()
