@@ -21,14 +21,16 @@
 //   orchestration                 MetalPipeline/raw_converter.cpp:148-214, 70-78
 //
 // The kernels are stencils over FP32 images, HBM-bound except the gradient
-// blur (up to 29 bilinear taps of a 4-channel image per pixel: vector-ALU and
-// LDS-bound).  The Sobel filter is fused into that blur (the Sobel tile of a
-// workgroup lives in LDS and never reaches HBM), and the last demosaic pass,
-// the highlight blend and the YCbCr transform run as one pass.  Arithmetic is FP32 evaluated in source
-// order (no contraction): out-of-bounds reads clamp to the edge, the linear
-// sampler interpolates at exact sub-texel positions, so every stage except the
-// gradient direction (atan2f) reproduces a plain FP32 evaluation of the
-// formulas bit for bit.
+// blur (a 4-tap and, below the noise threshold, a 9 x 9 Gaussian of a
+// 4-channel image per pixel: vector-ALU and LDS-bound).  The Sobel filter is
+// fused into that blur (the Sobel tile of a workgroup lives in LDS and never
+// reaches HBM), and the last demosaic pass, the highlight blend and the YCbCr
+// transform run as one pass.  Arithmetic is FP32 evaluated in source order (no
+// contraction): out-of-bounds reads clamp to the edge, the linear sampler
+// interpolates at exact sub-texel positions, so every stage except the
+// gradient direction (atan2f) and the wide blur reproduces a plain FP32
+// evaluation of the formulas bit for bit.  The wide blur runs separably
+// (SepKernel), equal to the reference's bilinear taps up to FP32 rounding.
 #pragma clang fp contract(off)
 
 #include "gcam_internal.hpp"
@@ -149,6 +151,24 @@ struct BlurTaps {
     Tap t2[kMaxTaps2];
 };
 
+// The wide kernel evaluated separably.  The bilinear taps of a k x k Gaussian
+// (k = 2h + 1) cover the texel kernel G(dx, dy) = g(dx) g(dy), |dx|, |dy| <= h,
+// except the corner tap, which the reference places at (h + 1/2, h + 1/2), so
+// that G(h, h) is spread as G(h, h) / 4 over the texels {h, h + 1}^2.  Hence
+// sum = sum_dy g(dy) sum_dx g(dx) S(dx, dy)
+//     + G(h, h) (S(h+1, h) + S(h, h+1) + S(h+1, h+1) - 3 S(h, h)) / 4,
+// divided by the taps' weight sum: 2h + 2 horizontal and 2h + 2 vertical
+// weights per output instead of (h + 1)^2 bilinear taps of four texels.
+// Equal to the tap evaluation up to FP32 rounding of the folded taps (~1e-7
+// relative).
+constexpr int kMaxSepHalf = 5;
+struct SepKernel {
+    int h;
+    float g[kMaxSepHalf + 1];  // g(d), d = 0 .. h
+    float gc;                  // G(h, h)
+    float norm;                // sum of the bilinear taps' weights, in tap order
+};
+
 // Output tile of a workgroup and its apron: taps reach texels x - 5 .. x + 6
 // for offsets in [-5, 5.5].
 constexpr int kBTW = 64, kBTH = 16, kBApL = 5, kBApR = 6;
@@ -191,14 +211,54 @@ __device__ __forceinline__ void blur_taps(const float4 (*sb)[kBAW], int lx, int 
     }
 }
 
+// The separable wide kernel for kBRows consecutive outputs starting at tile
+// row ly: each Sobel row's horizontal sum is formed once (in registers) and
+// added into every output whose window holds the row.
+template <int HW>
+__device__ __forceinline__ void blur_sep(const float4 (*sb)[kBAW], int lx, int ly, const SepKernel& k,
+                                         float4 (&sum)[kBRows]) {
+#pragma unroll
+    for (int r = 0; r < kBRows; r++) sum[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int j = -HW; j <= kBRows - 1 + HW; j++) {
+        float4 hr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int dx = -HW; dx <= HW; dx++) {
+            const float g = k.g[dx < 0 ? -dx : dx];
+            const float4 v = sb[ly + j][lx + dx];
+            hr = make_float4(fmaf(g, v.x, hr.x), fmaf(g, v.y, hr.y), fmaf(g, v.z, hr.z), fmaf(g, v.w, hr.w));
+        }
+#pragma unroll
+        for (int r = 0; r < kBRows; r++) {
+            const int d = j - r;
+            if (d < -HW || d > HW) continue;
+            const float g = k.g[d < 0 ? -d : d];
+            sum[r] = make_float4(fmaf(g, hr.x, sum[r].x), fmaf(g, hr.y, sum[r].y), fmaf(g, hr.z, sum[r].z),
+                                 fmaf(g, hr.w, sum[r].w));
+        }
+    }
+    const float c4 = 0.25f * k.gc;
+#pragma unroll
+    for (int r = 0; r < kBRows; r++) {
+        const float4 a = sb[ly + r + HW][lx + HW], b = sb[ly + r + HW][lx + HW + 1];
+        const float4 c = sb[ly + r + HW + 1][lx + HW], d = sb[ly + r + HW + 1][lx + HW + 1];
+        sum[r].x = fmaf(c4, (b.x + c.x) + (d.x - 3.0f * a.x), sum[r].x);
+        sum[r].y = fmaf(c4, (b.y + c.y) + (d.y - 3.0f * a.y), sum[r].y);
+        sum[r].z = fmaf(c4, (b.z + c.z) + (d.z - 3.0f * a.z), sum[r].z);
+        sum[r].w = fmaf(c4, (b.w + c.w) + (d.w - 3.0f * a.w), sum[r].w);
+    }
+}
+
 // kFromRaw: the Sobel tile is computed here from the scaled raw (the fused
 // front end); otherwise it is read from `sobel` (the per-operation entry
 // point, like the reference's two-texture kernel).  A tile entry outside the
 // image holds the Sobel value of the nearest image pixel, which is what the
-// clamp-to-edge sampler reads there.
-template <bool kFromRaw>
+// clamp-to-edge sampler reads there.  The narrow kernel, which decides
+// between the two radii per pixel, is evaluated tap by tap in the reference's
+// order; the wide one separably (HW = its half width).
+template <bool kFromRaw, int HW>
 __global__ __launch_bounds__(256) void k_blur_sobel(View<const float> raw, View<const float4> sobel, BlurTaps taps,
-                                                    float var_a, float var_b, View<float2> out) {
+                                                    SepKernel wide_k, float var_a, float var_b, View<float2> out) {
     __shared__ float4 sb[kBAH][kBAW];
     constexpr int RW = kBAW + 2, RH = kBAH + 2;
     __shared__ float rw[kFromRaw ? RH : 1][kFromRaw ? RW : 1];
@@ -232,7 +292,7 @@ __global__ __launch_bounds__(256) void k_blur_sobel(View<const float> raw, View<
     if (x >= W) return;
     const int r0 = threadIdx.y * kBRows, lx = threadIdx.x + kBApL;
     float4 nar[kBRows], wid[kBRows];
-    float nn, nw;
+    float nn;
     blur_taps(sb, lx, r0 + kBApL, taps.t1, taps.n1, nar, nn);
     bool wide[kBRows], any = false;
 #pragma unroll
@@ -243,7 +303,8 @@ __global__ __launch_bounds__(256) void k_blur_sobel(View<const float> raw, View<
         wide[k] = sqrtf(nar[k].x * nar[k].x + nar[k].y * nar[k].y) < 4.0f * sigma;
         any = any || wide[k];
     }
-    if (any) blur_taps(sb, lx, r0 + kBApL, taps.t2, taps.n2, wid, nw);
+    if (any) blur_sep<HW>(sb, lx, r0 + kBApL, wide_k, wid);
+    const float nw = wide_k.norm;
 #pragma unroll
     for (int k = 0; k < kBRows; k++) {
         const int y = y0 + r0 + k;
@@ -930,6 +991,23 @@ bool make_taps(float radius, Tap* out, int cap, int* n) {
     return true;
 }
 
+// The wide kernel's separable form (see SepKernel), from the same FP32
+// Gaussian the taps are folded from; norm = the taps' weights summed in tap
+// order, as the tap-by-tap evaluation divides by.
+SepKernel make_sep_kernel(float radius, const Tap* t, int n) {
+    SepKernel k{};
+    int ks = std::max((int)std::ceil(2 * radius), 2);
+    if (ks % 2 == 0) ks++;
+    k.h = ks / 2;
+    for (int d = 0; d <= k.h && d <= kMaxSepHalf; d++)
+        k.g[d] = std::exp(-((float)(d * d) / (2 * radius * radius)));
+    k.gc = std::exp(-((float)(2 * k.h * k.h) / (2 * radius * radius)));
+    float norm = 0.0f;
+    for (int i = 0; i < n; i++) norm = norm + t[i].w;
+    k.norm = norm;
+    return k;
+}
+
 dim3 grid_for(int w, int h, dim3 b) { return dim3((w + b.x - 1) / b.x, (h + b.y - 1) / b.y); }
 
 bool valid_image(const gcam_image* im, int fmt) {
@@ -1041,19 +1119,37 @@ gcam_status gcam_raw_image_sobel(const gcam_image* scaled, gcam_image* sobel, vo
     return launch_status("rawImageSobel");
 }
 
+}  // extern "C"
+
+template <int HW>
+static void launch_blur_sobel(const gcam_image* scaled, const gcam_image* sobel, const BlurTaps& taps,
+                              const SepKernel& wk, const float raw_variance[2], gcam_image* gradient, hipStream_t s) {
+    const dim3 g((gradient->width + kBTW - 1) / kBTW, (gradient->height + kBTH - 1) / kBTH);
+    const dim3 b(kBTW, kBTH / kBRows);
+    if (sobel) {
+        k_blur_sobel<false, HW><<<g, b, 0, s>>>(view<const float>(*scaled), view<const float4>(*sobel), taps, wk,
+                                                raw_variance[0], raw_variance[1], view<float2>(*gradient));
+    } else {
+        k_blur_sobel<true, HW><<<g, b, 0, s>>>(view<const float>(*scaled), View<const float4>{nullptr, 0, 0, 0}, taps,
+                                               wk, raw_variance[0], raw_variance[1], view<float2>(*gradient));
+    }
+}
+
+extern "C" {
+
 static gcam_status blur_sobel(const gcam_image* scaled, const gcam_image* sobel, float radius1, float radius2,
                               const float raw_variance[2], gcam_image* gradient, hipStream_t s) {
     BlurTaps taps{};
     if (!make_taps(radius1, taps.t1, kMaxTaps1, &taps.n1) || !make_taps(radius2, taps.t2, kMaxTaps2, &taps.n2))
         return bad("gaussianBlurSobelImage: radius1 must be in (0, 1.5] and radius2 in (0, 5.5]");
-    const dim3 g((gradient->width + kBTW - 1) / kBTW, (gradient->height + kBTH - 1) / kBTH);
-    const dim3 b(kBTW, kBTH / kBRows);
-    if (sobel) {
-        k_blur_sobel<false><<<g, b, 0, s>>>(view<const float>(*scaled), view<const float4>(*sobel), taps,
-                                            raw_variance[0], raw_variance[1], view<float2>(*gradient));
-    } else {
-        k_blur_sobel<true><<<g, b, 0, s>>>(view<const float>(*scaled), View<const float4>{nullptr, 0, 0, 0}, taps,
-                                           raw_variance[0], raw_variance[1], view<float2>(*gradient));
+    const SepKernel wk = make_sep_kernel(radius2, taps.t2, taps.n2);
+    switch (wk.h) {
+        case 1: launch_blur_sobel<1>(scaled, sobel, taps, wk, raw_variance, gradient, s); break;
+        case 2: launch_blur_sobel<2>(scaled, sobel, taps, wk, raw_variance, gradient, s); break;
+        case 3: launch_blur_sobel<3>(scaled, sobel, taps, wk, raw_variance, gradient, s); break;
+        case 4: launch_blur_sobel<4>(scaled, sobel, taps, wk, raw_variance, gradient, s); break;
+        case 5: launch_blur_sobel<5>(scaled, sobel, taps, wk, raw_variance, gradient, s); break;
+        default: return bad("gaussianBlurSobelImage: radius2 must be in (0, 5.5]");
     }
     return launch_status("gaussianBlurSobelImage");
 }

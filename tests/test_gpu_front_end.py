@@ -1,13 +1,19 @@
 """Raw front end on the GPU against the FP32 oracle.
 
-Every stage except the demosaic's direction weights evaluates the same FP32
-operations in the same order as the oracle (no contraction, correctly rounded
-division and square root), so scale, Sobel, both blur paths (with the Sobel
-tile fused or read from an image), the highlight blend, the transform and the
-despeckle are compared bit for bit.  The demosaic takes the gradient direction
-from atan2f, whose device and host implementations differ in the last bits;
-the direction enters continuously (through mix weights), so the demosaic is
-held to 1e-5 absolute on values in [0.1, 1].
+Scale, Sobel, the highlight blend, the transform and the despeckle evaluate
+the same FP32 operations in the same order as the oracle (no contraction,
+correctly rounded division and square root) and are compared bit for bit.  The
+gradient blur evaluates its narrow kernel (which decides per pixel between the
+two radii) tap by tap in the oracle's order, so the decision and every
+narrow-kernel pixel are bit-exact; the wide kernel is evaluated separably
+(one horizontal pass per Sobel row, one vertical pass, a corner correction),
+which equals the tap evaluation to FP32 rounding: magnitudes within 1e-5
+relative, and the sign (that of the blurred signed gradient) may differ only
+where that sum is within rounding of zero, which the demosaic never sees (it
+reads |gx|, |gy|).  The demosaic takes the gradient direction from atan2f,
+whose device and host implementations differ in the last bits; the direction
+enters continuously (through mix weights), so the demosaic is held to 1e-5
+absolute on values in [0.1, 1].
 """
 import numpy as np
 import pytest
@@ -33,6 +39,19 @@ def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
+def check_gradient(got, want, narrow_px=None):
+    """The blurred gradient against the oracle's: bit-exact on the narrow-kernel pixels, magnitudes of the
+    wide-kernel ones within 1e-5 relative (+1e-7 of the largest), at most a handful of sign flips."""
+    if narrow_px is not None:
+        np.testing.assert_array_equal(got[narrow_px], want[narrow_px])
+    scale = float(np.abs(want).max())
+    np.testing.assert_allclose(np.abs(got), np.abs(want), rtol=1e-5, atol=1e-7 * scale)
+    flips = int(np.sum(np.sign(got) != np.sign(want)))
+    assert flips <= 2 + got.size // 1_000_000, f"{flips} sign flips"
+    err = float(np.abs(np.abs(got) - np.abs(want)).max())
+    print(f"gradient: max |magnitude| difference {err:.3g} (scale {scale:.3g}), sign flips {flips}")
+
+
 @pytest.mark.parametrize("bayer,lens", [(0, 0.0), (1, 0.4), (2, 0.0), (3, 0.25)])
 def test_scale_raw_is_bit_exact(bayer, lens):
     rng = np.random.default_rng(bayer)
@@ -45,7 +64,7 @@ def test_scale_raw_is_bit_exact(bayer, lens):
 
 
 @pytest.mark.parametrize("w,h", [(256, 192), (250, 130), (66, 18)])
-def test_sobel_and_gradient_blur_are_bit_exact(w, h):
+def test_sobel_bit_exact_and_gradient_blur(w, h):
     raw_d, raw = raw_frame(w, h, 11)
     scaled = oracle.scale_raw(raw, 2, [SCALE_12BIT] * 4)
     sob = oracle.raw_sobel(scaled)
@@ -55,10 +74,11 @@ def test_sobel_and_gradient_blur_are_bit_exact(w, h):
     want = oracle.blur_sobel(scaled, sob, rv[1])
     given = gcam.gaussianBlurSobelImage(dev(scaled), dev(sob), rv[1]).cpu().numpy()
     fused = gcam.gaussianBlurSobelImage(dev(scaled), None, rv[1]).cpu().numpy()
-    np.testing.assert_array_equal(given, want)
-    np.testing.assert_array_equal(fused, want)
-    narrow = oracle.blur_sobel(scaled, sob, (0.0, 0.0))
-    wide_frac = float(np.mean(np.any(want != narrow, axis=-1)))
+    narrow = oracle.blur_sobel(scaled, sob, (0.0, 0.0))  # sigma 0: every pixel keeps the narrow kernel
+    narrow_px = np.all(want == narrow, axis=-1)
+    np.testing.assert_array_equal(given, fused)  # Sobel tile read from an image or computed in the tile
+    check_gradient(fused, want, narrow_px)
+    wide_frac = float(np.mean(~narrow_px))
     print(f"{w}x{h}: wide-kernel pixels {wide_frac:.3f}")
     assert 0.05 < wide_frac < 0.995  # both branches of the blur are exercised
 
@@ -116,7 +136,7 @@ def test_front_end_matches_the_oracle(w, h, despeckle):
     ycc, grad = fe.run(raw_d, p)
     torch.cuda.synchronize()
     ycc_o, grad_o, _ = oracle_front(raw, p)
-    np.testing.assert_array_equal(grad.cpu().numpy(), grad_o)
+    check_gradient(grad.cpu().numpy(), grad_o)
     err = float(np.abs(ycc.cpu().numpy() - ycc_o).max())
     print(f"{w}x{h} despeckle={despeckle}: YCbCr max abs {err:.3g}")
     assert err <= 2 * DEMOSAIC_ATOL  # the transform's rows sum to at most ~1.5 in magnitude
