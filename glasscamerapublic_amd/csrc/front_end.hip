@@ -71,6 +71,20 @@ __device__ __forceinline__ P fetch(const View<const P>& v, int x, int y) {
     return v.at(clampi(x, 0, v.w - 1), clampi(y, 0, v.h - 1));
 }
 
+// An image region held in LDS: cell (x, y) of the image plane at
+// p[(y - oy) * pitch + (x - ox)].  A cell outside the image holds the value at
+// the clamped position, so fetch needs no clamp.
+template <class P>
+struct Tile {
+    P* p;
+    int ox, oy, pitch;
+    __device__ __forceinline__ P& at(int x, int y) const { return p[(y - oy) * pitch + (x - ox)]; }
+};
+template <class P>
+__device__ __forceinline__ P fetch(const Tile<P>& t, int x, int y) {
+    return t.at(x, y);
+}
+
 // Metal's smoothstep / mix / clamp.
 __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
     float t = (x - e0) / (e1 - e0);
@@ -320,8 +334,10 @@ struct Var2 {
     float a, b;
 };
 
-// interpolateGreen at one pixel (demosaic.metal:214-300).
-__device__ float green_at(const View<const float>& R, const View<const float2>& Gr, Quad q, Var2 gv, int x, int y) {
+// interpolateGreen at one pixel (demosaic.metal:214-300).  R reads the scaled
+// raw (an image or an LDS tile), Gr the gradient.
+template <class RawR>
+__device__ __forceinline__ float green_at(const RawR& R, const View<const float2>& Gr, Quad q, Var2 gv, int x, int y) {
     const bool red = (q.o[kRed].x & 1) == (x & 1) && (q.o[kRed].y & 1) == (y & 1);
     const bool blue = (q.o[kBlue].x & 1) == (x & 1) && (q.o[kBlue].y & 1) == (y & 1);
     if (!(red || blue)) return R.at(x, y);
@@ -365,8 +381,9 @@ __global__ __launch_bounds__(256) void k_interpolate_green(View<const float> raw
 // interpolateRedBluePixel (demosaic.metal:309-380): the missing colour at a
 // red (red_site) or blue site from the colour differences of its diagonal
 // neighbours, steered by the gradient direction.
-__device__ float4 red_blue_site(const View<const float>& R, const View<const float>& Gn, const View<const float2>& Gr,
-                                int x, int y, bool red_site, Var2 rv, Var2 bv) {
+template <class RawR, class GreenR>
+__device__ __forceinline__ float4 red_blue_site(const RawR& R, const GreenR& Gn, const View<const float2>& Gr, int x,
+                                                int y, bool red_site, Var2 rv, Var2 bv) {
     const float g = Gn.at(x, y), c1 = R.at(x, y);
     const float g_tl = fetch(Gn, x - 1, y - 1), g_tr = fetch(Gn, x + 1, y - 1);
     const float g_bl = fetch(Gn, x - 1, y + 1), g_br = fetch(Gn, x + 1, y + 1);
@@ -412,9 +429,11 @@ __global__ __launch_bounds__(256) void k_interpolate_red_blue(View<const float> 
     }
 }
 
-// interpolateRedBlueAtGreenPixel (demosaic.metal:420-502).
-__device__ float4 red_blue_at_green_site(const View<const float4>& C, const View<const float2>& Gr, int x, int y,
-                                         Var2 rv, Var2 bv) {
+// interpolateRedBlueAtGreenPixel (demosaic.metal:420-502).  C reads the
+// interpolateRedBlue output as (red, green, blue, 0).
+template <class RgbR>
+__device__ __forceinline__ float4 red_blue_at_green_site(const RgbR& C, const View<const float2>& Gr, int x, int y,
+                                                         Var2 rv, Var2 bv) {
     const float gc = C.at(x, y).y;
     const float4 l = fetch(C, x - 1, y), r = fetch(C, x + 1, y), u = fetch(C, x, y - 1), d = fetch(C, x, y + 1);
     const float4 l3 = fetch(C, x - 3, y), r3 = fetch(C, x + 3, y), u3 = fetch(C, x, y - 3), d3 = fetch(C, x, y + 3);
@@ -472,22 +491,16 @@ __device__ __forceinline__ float4 transform3(const Mat3& t, float4 p) {
                        t.m[6] * p.x + t.m[7] * p.y + t.m[8] * p.z, 0.0f);
 }
 
-// interpolateRedBlueAtGreen, one quad per thread; with kTail the quad's four
-// pixels then go through blendHighlightsImage and transformImage (the fused
-// front end) instead of being written as linear RGB.
-template <bool kTail>
+// interpolateRedBlueAtGreen, one quad per thread.
 __global__ __launch_bounds__(256) void k_interpolate_red_blue_at_green(View<const float4> in, View<const float2> grad,
-                                                                       View<float4> out, Quad q, Var2 rv, Var2 bv,
-                                                                       float clip, Mat3 t) {
+                                                                       View<float4> out, Quad q, Var2 rv, Var2 bv) {
     const int qx = blockIdx.x * blockDim.x + threadIdx.x, qy = blockIdx.y * blockDim.y + threadIdx.y;
     if (qx >= out.w / 2 || qy >= out.h / 2) return;
     const int x = 2 * qx, y = 2 * qy;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const int px = x + q.o[c].x, py = y + q.o[c].y;
-        float4 v = (c == kGreen || c == kGreen2) ? red_blue_at_green_site(in, grad, px, py, rv, bv) : in.at(px, py);
-        if (kTail) v = transform3(t, blend_highlight(v, clip));
-        out.at(px, py) = v;
+        out.at(px, py) = (c == kGreen || c == kGreen2) ? red_blue_at_green_site(in, grad, px, py, rv, bv) : in.at(px, py);
     }
 }
 
@@ -525,9 +538,12 @@ __device__ __forceinline__ float median9(float v[9]) {
     return a1;
 }
 
-__global__ __launch_bounds__(256) void k_despeckle(View<const float4> in, View<float4> out, Var2 yv) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= out.w || y >= out.h) return;
+// despeckleLumaMedianChromaImage at one pixel (demosaic.metal:1262-1283): the
+// luma clamped into its neighbourhood's second extremes (softly, by how far
+// the extreme stands out of the noise), the chroma replaced by its median.
+// `in` reads the YCbCr image (an image or an LDS tile).
+template <class YccR>
+__device__ __forceinline__ float4 despeckle_at(const YccR& in, int x, int y, Var2 yv) {
     // The reference's reading order: the right two columns top to bottom,
     // then (-1, 1), (-1, 0), (-1, -1).
     const int ox[9] = {0, 1, 0, 1, 0, 1, -1, -1, -1}, oy[9] = {-1, -1, 0, 0, 1, 1, 1, 0, -1};
@@ -550,9 +566,151 @@ __global__ __launch_bounds__(256) void k_despeckle(View<const float4> in, View<f
     const float sigma = sqrtf(yv.a + yv.b * centre);
     const float lo = mixf(min2, min1, smoothstepf(sigma, 4 * sigma, min2 - min1));
     const float hi = mixf(max2, max1, smoothstepf(sigma, 4 * sigma, max1 - max2));
-    out.at(x, y) = make_float4(clampf(centre, lo, hi), median9(cb), median9(cr), 0.0f);
+    return make_float4(clampf(centre, lo, hi), median9(cb), median9(cr), 0.0f);
 }
 
+__global__ __launch_bounds__(256) void k_despeckle(View<const float4> in, View<float4> out, Var2 yv) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    out.at(x, y) = despeckle_at(in, x, y, yv);
+}
+
+// ---- the demosaic chain in one pass ----------------------------------------
+//
+// k_demosaic_tile runs interpolateGreen, interpolateRedBlue,
+// interpolateRedBlueAtGreen with blendHighlightsImage and transformImage, and
+// (kDespeckle) the despeckle for one TW x TH tile of the output, with every
+// intermediate image in LDS: the scaled raw and the gradient are read once
+// and the YCbCr image written once.  Each stage covers the tile grown by the
+// reach of the stages after it (the despeckle reads +-1, red/blue at green
+// +-3, red/blue +-2, green +-2), and a cell outside the image holds the value
+// at the clamped position, which is what the per-stage kernels' clamped reads
+// return there: each value is the same FP32 evaluation as theirs, bit for bit.
+
+// interpolateRedBlue's output read as (red, green, blue, 0) from the fused
+// kernel's green plane and (red, blue) plane; green sites hold (0, 0).
+struct RgbTile {
+    Tile<float> g;
+    Tile<float2> rb;
+    __device__ __forceinline__ float4 at(int x, int y) const {
+        const float2 v = rb.at(x, y);
+        return make_float4(v.x, g.at(x, y), v.y, 0.0f);
+    }
+};
+__device__ __forceinline__ float4 fetch(const RgbTile& t, int x, int y) { return t.at(x, y); }
+
+// The YCbCr image as three LDS planes of one geometry, `plane` floats apart.
+struct YccTile {
+    float* p;
+    int ox, oy, pitch, plane;
+    __device__ __forceinline__ int index(int x, int y) const { return (y - oy) * pitch + (x - ox); }
+    __device__ __forceinline__ float4 at(int x, int y) const {
+        const int i = index(x, y);
+        return make_float4(p[i], p[i + plane], p[i + 2 * plane], 0.0f);
+    }
+    __device__ __forceinline__ void store(int x, int y, float4 v) const {
+        const int i = index(x, y);
+        p[i] = v.x;
+        p[i + plane] = v.y;
+        p[i + 2 * plane] = v.z;
+    }
+};
+__device__ __forceinline__ float4 fetch(const YccTile& t, int x, int y) { return t.at(x, y); }
+
+__device__ __forceinline__ bool is_site(const Quad& q, int c, int x, int y) {
+    return (q.o[c].x & 1) == (x & 1) && (q.o[c].y & 1) == (y & 1);
+}
+
+constexpr int kTileThreads = 256;
+
+// Calls f(x, y) for every position of the w x h region at (ox, oy), w and h
+// even, one 2x2 block per thread and pass: in each of the four calls of a
+// pass the threads of a wave hold positions of one Bayer colour (away from
+// the image border), so the colour branches do not diverge.
+template <class F>
+__device__ __forceinline__ void for_each_quad(int ox, int oy, int w, int h, F&& f) {
+    const int bw = w / 2, n = bw * (h / 2);
+    for (int b = threadIdx.x; b < n; b += kTileThreads) {
+        const int x = ox + 2 * (b % bw), y = oy + 2 * (b / bw);
+        f(x, y);
+        f(x + 1, y);
+        f(x, y + 1);
+        f(x + 1, y + 1);
+    }
+}
+
+template <int TW, int TH, bool kDespeckle>
+struct TileGeometry {
+    static constexpr int kH3 = kDespeckle ? 1 : 0;  // YCbCr
+    static constexpr int kH2 = kH3 + 3;             // red / blue
+    static constexpr int kH1 = kH2 + 2;             // green
+    static constexpr int kH0 = kH1 + 2;             // scaled raw
+    static constexpr int kW0 = TW + 2 * kH0, kR0 = TH + 2 * kH0;
+    static constexpr int kW1 = TW + 2 * kH1, kR1 = TH + 2 * kH1;
+    static constexpr int kW2 = TW + 2 * kH2, kR2 = TH + 2 * kH2;
+    static constexpr int kW3 = TW + 2 * kH3, kR3 = TH + 2 * kH3;
+    static constexpr int kGreenFloats = kW1 * kR1, kRbFloats = 2 * kW2 * kR2;
+    // The raw plane is dead once red / blue is done; the YCbCr planes reuse it.
+    static constexpr int kRawFloats = kW0 * kR0, kYccFloats = 3 * kW3 * kR3;
+    static constexpr int kFloats = kGreenFloats + kRbFloats + (kRawFloats > kYccFloats ? kRawFloats : kYccFloats);
+    static_assert(TW % 2 == 0 && TH % 2 == 0, "tiles hold whole 2x2 quads");
+    static_assert(kGreenFloats % 2 == 0, "the (red, blue) plane is float2-aligned");
+};
+
+// One workgroup per tile.  The dispatcher deals workgroups to the 8 XCDs in
+// turn, so workgroup b runs on XCD b % 8; tile (b % 8) * per_xcd + b / 8 gives
+// each XCD a contiguous run of tiles in row order, whose halos then meet in
+// that XCD's L2.
+template <int TW, int TH, bool kDespeckle>
+__global__ __launch_bounds__(kTileThreads) void k_demosaic_tile(View<const float> raw, View<const float2> grad,
+                                                                View<float4> out, Quad q, Var2 rv, Var2 gv, Var2 bv,
+                                                                float clip, Mat3 t, Var2 yv, int tiles_x, int tiles,
+                                                                int per_xcd) {
+    using G = TileGeometry<TW, TH, kDespeckle>;
+    __shared__ float lds[G::kFloats];
+    const int tile = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tile >= tiles) return;
+    const int x0 = (tile % tiles_x) * TW, y0 = (tile / tiles_x) * TH;
+    const int W = out.w, H = out.h;
+    const Tile<float> green{lds, x0 - G::kH1, y0 - G::kH1, G::kW1};
+    const Tile<float2> rb{reinterpret_cast<float2*>(lds + G::kGreenFloats), x0 - G::kH2, y0 - G::kH2, G::kW2};
+    float* reused = lds + G::kGreenFloats + G::kRbFloats;
+    const Tile<float> sraw{reused, x0 - G::kH0, y0 - G::kH0, G::kW0};
+    const YccTile ycc{reused, x0 - G::kH3, y0 - G::kH3, G::kW3, G::kW3 * G::kR3};
+
+    for (int i = threadIdx.x; i < G::kW0 * G::kR0; i += kTileThreads) {
+        const int x = sraw.ox + i % G::kW0, y = sraw.oy + i / G::kW0;
+        sraw.p[i] = raw.at(clampi(x, 0, W - 1), clampi(y, 0, H - 1));
+    }
+    __syncthreads();
+    for_each_quad(green.ox, green.oy, G::kW1, G::kR1, [&](int x, int y) {
+        green.at(x, y) = green_at(sraw, grad, q, gv, clampi(x, 0, W - 1), clampi(y, 0, H - 1));
+    });
+    __syncthreads();
+    for_each_quad(rb.ox, rb.oy, G::kW2, G::kR2, [&](int x, int y) {
+        const int cx = clampi(x, 0, W - 1), cy = clampi(y, 0, H - 1);
+        const bool red = is_site(q, kRed, cx, cy);
+        float2 v = make_float2(0.0f, 0.0f);
+        if (red || is_site(q, kBlue, cx, cy)) {
+            const float4 c = red_blue_site(sraw, green, grad, cx, cy, red, rv, bv);
+            v = make_float2(c.x, c.z);
+        }
+        rb.at(x, y) = v;
+    });
+    __syncthreads();
+    const RgbTile rgb{green, rb};
+    for_each_quad(ycc.ox, ycc.oy, G::kW3, G::kR3, [&](int x, int y) {
+        const int cx = clampi(x, 0, W - 1), cy = clampi(y, 0, H - 1);
+        const bool site = is_site(q, kRed, cx, cy) || is_site(q, kBlue, cx, cy);
+        const float4 v = site ? rgb.at(cx, cy) : red_blue_at_green_site(rgb, grad, cx, cy, rv, bv);
+        ycc.store(x, y, transform3(t, blend_highlight(v, clip)));
+    });
+    __syncthreads();
+    for (int i = threadIdx.x; i < TW * TH; i += kTileThreads) {
+        const int x = x0 + i % TW, y = y0 + i / TW;
+        if (x < W && y < H) out.at(x, y) = kDespeckle ? despeckle_at(ycc, x, y, yv) : ycc.at(x, y);
+    }
+}
 
 // ---- raw noise model: basicRawNoiseStatistics + MeasureRawNLF ------------
 
@@ -1043,6 +1201,20 @@ gcam_status raw_nlf_device(View<const float> scaled, const Quad& q, View<float4>
     k_raw_nlf_finalize<3><<<1, 256, 0, s>>>(partials, kRawNlfBlocks, state, exposure, npx, out);
     return launch_status("MeasureRawNLF");
 }
+
+// The demosaic chain of the front end (interpolateGreen through the YCbCr
+// transform, then the despeckle when asked) as one k_demosaic_tile pass.
+template <bool kDespeckle>
+gcam_status demosaic_tiles(View<const float> scaled, View<const float2> grad, View<float4> out, const Quad& q,
+                           const float rv[6], float clip, const Mat3& t, Var2 yv, hipStream_t s) {
+    constexpr int TW = 64, TH = 32;
+    const int tiles_x = (out.w + TW - 1) / TW, tiles = tiles_x * ((out.h + TH - 1) / TH);
+    const int per_xcd = (tiles + 7) / 8;
+    k_demosaic_tile<TW, TH, kDespeckle><<<8 * per_xcd, kTileThreads, 0, s>>>(
+        scaled, grad, out, q, Var2{rv[0], rv[1]}, Var2{rv[2], rv[3]}, Var2{rv[4], rv[5]}, clip, t, yv, tiles_x, tiles,
+        per_xcd);
+    return launch_status(kDespeckle ? "demosaic + despeckle" : "demosaic");
+}
 }  // namespace
 
 }  // namespace gcam
@@ -1166,8 +1338,7 @@ gcam_status gcam_gaussian_blur_sobel_image(const gcam_image* scaled, const gcam_
 }
 
 static gcam_status demosaic(const gcam_image* scaled, const gcam_image* gradient, gcam_image* green,
-                            gcam_image* rgb_tmp, gcam_image* rgb_out, int32_t bayer, const float rv[6], bool tail,
-                            float clip, const Mat3& t, hipStream_t s) {
+                            gcam_image* rgb_tmp, gcam_image* rgb_out, int32_t bayer, const float rv[6], hipStream_t s) {
     const Quad q = quad_of(bayer);
     const Var2 red{rv[0], rv[1]}, gv{rv[2], rv[3]}, blue{rv[4], rv[5]};
     const int w = scaled->width, h = scaled->height;
@@ -1180,15 +1351,8 @@ static gcam_status demosaic(const gcam_image* scaled, const gcam_image* gradient
     k_interpolate_red_blue<<<gq, kBlock, 0, s>>>(view<const float>(*scaled), view<const float>(*green),
                                                  view<const float2>(*gradient), view<float4>(*rgb_tmp), q, red, blue);
     if ((st = launch_status("interpolateRedBlue")) != GCAM_OK) return st;
-    if (tail) {
-        k_interpolate_red_blue_at_green<true><<<gq, kBlock, 0, s>>>(view<const float4>(*rgb_tmp),
-                                                                    view<const float2>(*gradient),
-                                                                    view<float4>(*rgb_out), q, red, blue, clip, t);
-    } else {
-        k_interpolate_red_blue_at_green<false><<<gq, kBlock, 0, s>>>(view<const float4>(*rgb_tmp),
-                                                                     view<const float2>(*gradient),
-                                                                     view<float4>(*rgb_out), q, red, blue, clip, t);
-    }
+    k_interpolate_red_blue_at_green<<<gq, kBlock, 0, s>>>(view<const float4>(*rgb_tmp), view<const float2>(*gradient),
+                                                          view<float4>(*rgb_out), q, red, blue);
     return launch_status("interpolateRedBlueAtGreen");
 }
 
@@ -1203,8 +1367,7 @@ gcam_status gcam_demosaic_image(const gcam_image* scaled, const gcam_image* grad
     if (rgb_tmp->data == rgb_out->data) return bad("gcam_demosaic_image: rgb_tmp and rgb_out must differ");
     if ((scaled->width | scaled->height) & 1) return bad("gcam_demosaic_image: width and height must be even");
     if (bayer < 0 || bayer > 3) return bad("gcam_demosaic_image: bad Bayer pattern");
-    return demosaic(scaled, gradient, green, rgb_tmp, rgb_out, bayer, raw_variance, false, 1.0f, Mat3{},
-                    (hipStream_t)stream);
+    return demosaic(scaled, gradient, green, rgb_tmp, rgb_out, bayer, raw_variance, (hipStream_t)stream);
 }
 
 gcam_status gcam_blend_highlights_image(const gcam_image* in, float clip, gcam_image* out, void* stream) {
@@ -1348,9 +1511,6 @@ gcam_status gcam_local_tone_mapping_mask_image(const gcam_image* image, const gc
 struct gcam_front_end {
     int w, h, device;
     float* scaled;
-    float* green;
-    float4* rgb;  // interpolateRedBlue output
-    float4* ycc;  // before the despeckle
     char* nlf_scratch;      // raw statistics images, fit partials, state, output
     float raw_a[4], raw_b[4];  // the raw noise model of the last run
 };
@@ -1365,8 +1525,7 @@ gcam_status gcam_front_end_create(int32_t width, int32_t height, gcam_front_end*
     f->h = height;
     hipGetDevice(&f->device);
     const size_t n = (size_t)width * height;
-    if (hipMalloc(&f->scaled, n * sizeof(float)) != hipSuccess || hipMalloc(&f->green, n * sizeof(float)) != hipSuccess ||
-        hipMalloc(&f->rgb, n * sizeof(float4)) != hipSuccess || hipMalloc(&f->ycc, n * sizeof(float4)) != hipSuccess) {
+    if (hipMalloc(&f->scaled, n * sizeof(float)) != hipSuccess) {
         gcam_front_end_destroy(f);
         return set_error(GCAM_ERR_OUT_OF_MEMORY, "gcam_front_end_create: device allocation failed");
     }
@@ -1377,9 +1536,6 @@ gcam_status gcam_front_end_create(int32_t width, int32_t height, gcam_front_end*
 void gcam_front_end_destroy(gcam_front_end* f) {
     if (!f) return;
     hipFree(f->scaled);
-    hipFree(f->green);
-    hipFree(f->rgb);
-    hipFree(f->ycc);
     hipFree(f->nlf_scratch);
     delete f;
 }
@@ -1395,8 +1551,7 @@ gcam_status gcam_front_end_run(gcam_front_end* f, const gcam_image* raw, const g
     if (p->bayer_pattern < 0 || p->bayer_pattern > 3) return bad("gcam_front_end_run: bad Bayer pattern");
     const hipStream_t s = (hipStream_t)stream;
     const int w = f->w, h = f->h;
-    gcam_image sc{f->scaled, w, h, w, GCAM_PIXEL_R32F}, gr{f->green, w, h, w, GCAM_PIXEL_R32F};
-    gcam_image rgb{f->rgb, w, h, w, GCAM_PIXEL_RGBA32F}, pre{f->ycc, w, h, w, GCAM_PIXEL_RGBA32F};
+    gcam_image sc{f->scaled, w, h, w, GCAM_PIXEL_R32F};
     gcam_status st = gcam_scale_raw_data(raw, &sc, p->bayer_pattern, p->scale_mul, p->black_level, p->lens_shading, s);
     if (st != GCAM_OK) return st;
     for (int c = 0; c < 4; c++) {
@@ -1435,12 +1590,13 @@ gcam_status gcam_front_end_run(gcam_front_end* f, const gcam_image* raw, const g
     if ((st = blur_sobel(&sc, nullptr, 1.5f, 4.5f, rv + 2, gradient, s)) != GCAM_OK) return st;
     Mat3 t;
     gcam_cam_ycbcr(p->rgb_cam, t.m);
-    gcam_image* tail_out = p->despeckle ? &pre : ycbcr;
-    if ((st = demosaic(&sc, gradient, &gr, &rgb, tail_out, p->bayer_pattern, rv, true, p->highlight_clip, t, s)) !=
-        GCAM_OK)
-        return st;
-    if (p->despeckle) return gcam_despeckle_image(&pre, p->despeckle_nlf.a, p->despeckle_nlf.b, ycbcr, s);
-    return GCAM_OK;
+    const View<const float> scaled_v{f->scaled, w, h, w};
+    const Quad q = quad_of(p->bayer_pattern);
+    if (p->despeckle)
+        return demosaic_tiles<true>(scaled_v, view<const float2>(*gradient), view<float4>(*ycbcr), q, rv,
+                                    p->highlight_clip, t, Var2{p->despeckle_nlf.a[0], p->despeckle_nlf.b[0]}, s);
+    return demosaic_tiles<false>(scaled_v, view<const float2>(*gradient), view<float4>(*ycbcr), q, rv,
+                                 p->highlight_clip, t, Var2{}, s);
 }
 
 gcam_status gcam_front_end_raw_nlf(const gcam_front_end* f, float raw_nlf_a[4], float raw_nlf_b[4]) {

@@ -140,16 +140,36 @@ def test_front_end_matches_the_oracle(w, h, despeckle):
     err = float(np.abs(ycc.cpu().numpy() - ycc_o).max())
     print(f"{w}x{h} despeckle={despeckle}: YCbCr max abs {err:.3g}")
     assert err <= 2 * DEMOSAIC_ATOL  # the transform's rows sum to at most ~1.5 in magnitude
-    # the per-operation entry points compose to the same result
+    check_per_stage_chain(raw_d, p, ycc, grad)
+    fe.close()
+
+
+def check_per_stage_chain(raw_d, p, ycc, grad):
+    """The per-operation entry points compose to the front end's result bit for bit."""
     scaled = gcam.scaleRawData(raw_d, p.bayerPattern, p.scale_mul, p.black_level, p.lensShadingCorrection)
     rv = gcam.getRawVariance(p.rawNlf)
     g2 = gcam.gaussianBlurSobelImage(scaled, gcam.rawImageSobel(scaled), rv[1])
     assert torch.equal(g2, grad)
     _, _, rgb = gcam.demosaicImage(scaled, g2, p.bayerPattern, rv)
     y2 = gcam.transformImage(gcam.blendHighlightsImage(rgb, p.highlightClip), gcam.cam_ycbcr(p.rgb_cam))
-    if despeckle:
+    if p.despeckleNlf is not None:
         y2 = gcam.despeckleImage(y2, *p.despeckleNlf)
     assert torch.equal(y2, ycc)
+
+
+# The front end's demosaic runs as one pass over 64 x 32 tiles with every
+# intermediate in LDS; these sizes cover images smaller than a tile, partial
+# tiles at the right and bottom edges and every Bayer pattern.
+@pytest.mark.parametrize("w,h,bayer,despeckle", [
+    (2, 2, 0, True), (6, 4, 1, False), (66, 34, 3, True), (64, 32, 2, False),
+    (130, 98, 0, False), (1030, 770, 1, True), (4032, 96, 2, True), (200, 1000, 3, False)])
+def test_tiled_demosaic_equals_the_per_stage_chain(w, h, bayer, despeckle):
+    print(f"tiled demosaic {w}x{h} bayer {bayer} despeckle {despeckle}", flush=True)
+    raw_d, _ = raw_frame(w, h, 7 + bayer)
+    p = params_for(bayer=bayer, despeckle=despeckle)
+    fe = gcam.RawFrontEnd(w, h)
+    ycc, grad = fe.run(raw_d, p)
+    check_per_stage_chain(raw_d, p, ycc, grad)
     fe.close()
 
 
