@@ -184,8 +184,10 @@ struct SepKernel {
 };
 
 // Output tile of a workgroup and its apron: taps reach texels x - 5 .. x + 6
-// for offsets in [-5, 5.5].
-constexpr int kBTW = 64, kBTH = 16, kBApL = 5, kBApR = 6;
+// for offsets in [-5, 5.5].  64 x 32 tiles of 512 threads (four rows each)
+// measured fastest against 32-64 x 8-64 tiles of 1-8 rows per thread on the
+// 12 MP frame: taller tiles recompute less Sobel apron.
+constexpr int kBTW = 64, kBTH = 32, kBApL = 5, kBApR = 6;
 constexpr int kBAW = kBTW + kBApL + kBApR, kBAH = kBTH + kBApL + kBApR;
 
 // Rows of output a thread blurs: consecutive, so that the horizontal
@@ -270,9 +272,13 @@ __device__ __forceinline__ void blur_sep(const float4 (*sb)[kBAW], int lx, int l
 // clamp-to-edge sampler reads there.  The narrow kernel, which decides
 // between the two radii per pixel, is evaluated tap by tap in the reference's
 // order; the wide one separably (HW = its half width).
+constexpr int kBThreads = kBTW * (kBTH / kBRows);
+static_assert(kBTH % kBRows == 0 && kBThreads <= 1024, "whole rows per thread, one workgroup");
+
 template <bool kFromRaw, int HW>
-__global__ __launch_bounds__(256) void k_blur_sobel(View<const float> raw, View<const float4> sobel, BlurTaps taps,
-                                                    SepKernel wide_k, float var_a, float var_b, View<float2> out) {
+__global__ __launch_bounds__(kBThreads) void k_blur_sobel(View<const float> raw, View<const float4> sobel,
+                                                          BlurTaps taps, SepKernel wide_k, float var_a, float var_b,
+                                                          View<float2> out) {
     __shared__ float4 sb[kBAH][kBAW];
     constexpr int RW = kBAW + 2, RH = kBAH + 2;
     __shared__ float rw[kFromRaw ? RH : 1][kFromRaw ? RW : 1];
@@ -334,30 +340,39 @@ struct Var2 {
     float a, b;
 };
 
-// interpolateGreen at one pixel (demosaic.metal:214-300).  R reads the scaled
-// raw (an image or an LDS tile), Gr the gradient.
+// The gradient as the demosaic passes use it: t = 2 atan2(|gy|, |gx|) / pi
+// (0: horizontal gradient, 1: vertical) and its length.
+struct GradDir {
+    float t, len;
+};
+__device__ __forceinline__ GradDir grad_dir(float2 g) {
+    const float ax = fabsf(g.x), ay = fabsf(g.y);
+    return GradDir{2 * atan2f(ay, ax) / kPi, sqrtf(ax * ax + ay * ay)};
+}
+
+__device__ __forceinline__ bool is_site(const Quad& q, int c, int x, int y) {
+    return (q.o[c].x & 1) == (x & 1) && (q.o[c].y & 1) == (y & 1);
+}
+
+// interpolateGreen at a red or blue site (demosaic.metal:214-300).  R reads the
+// scaled raw (an image or an LDS tile), gdir is the site's gradient.
 template <class RawR>
-__device__ __forceinline__ float green_at(const RawR& R, const View<const float2>& Gr, Quad q, Var2 gv, int x, int y) {
-    const bool red = (q.o[kRed].x & 1) == (x & 1) && (q.o[kRed].y & 1) == (y & 1);
-    const bool blue = (q.o[kBlue].x & 1) == (x & 1) && (q.o[kBlue].y & 1) == (y & 1);
-    if (!(red || blue)) return R.at(x, y);
+__device__ __forceinline__ float green_at_site(const RawR& R, GradDir gdir, Var2 gv, int x, int y) {
     const float low_noise = 1.0f - smoothstepf(3.5e-4f, 2e-3f, gv.b);
     const float gl = fetch(R, x - 1, y), gr = fetch(R, x + 1, y), gu = fetch(R, x, y - 1), gd = fetch(R, x, y + 1);
     const float c = R.at(x, y);
     const float cl = fetch(R, x - 2, y), cr = fetch(R, x + 2, y), cu = fetch(R, x, y - 2), cd = fetch(R, x, y + 2);
     const float d_ave = (fetch(R, x - 1, y - 1) + fetch(R, x + 1, y - 1) + fetch(R, x - 1, y + 1) + fetch(R, x + 1, y + 1)) / 4;
     const float g_ave = (gl + gr + gu + gd) / 4;
-    const float2 g2 = Gr.at(x, y);
-    const float ax = fabsf(g2.x), ay = fabsf(g2.y);
     const float lf_h = (gl + gr) / 2, lf_v = (gu + gd) / 2;
     const float hf_h = ((cl + cr) - 2 * c) / 4, hf_v = ((cu + cd) - 2 * c) / 4;
     const float sd = sqrtf(gv.a + gv.b * g_ave);
-    const float glen = sqrtf(ax * ax + ay * ay);
+    const float glen = gdir.len;
     const float thr = smoothstepf(sd, 4 * sd, glen);
     const float low_thr = 1.0f - smoothstepf(2 * sd, 8 * sd, glen);
     const float sharpening = (0.5f + 0.5f * low_noise) * thr * (1 + low_noise * low_thr);
     const float highlights = 1.0f - smoothstepf(0.25f, 1.0f, fmaxf(c, fmaxf(fmaxf(cl, cr), fmaxf(cu, cd))));
-    float dir = 2 * atan2f(ay, ax) / kPi;
+    float dir = gdir.t;
     if (gv.b < 1e-3f) {
         dir = dir < 0.5f ? mixf(dir, 0.0f, 1.0f - smoothstepf(0.3f, 0.45f, dir))
                          : mixf(dir, 1.0f, smoothstepf(1.0f - 0.45f, 1.0f - 0.3f, dir));
@@ -371,6 +386,13 @@ __device__ __forceinline__ float green_at(const RawR& R, const View<const float2
     return clampf(g, mixf(1.0f, 0.5f, white) * gmin, mixf(1.0f, 1.5f, white) * gmax);
 }
 
+// interpolateGreen at any pixel: green sites keep their sample.
+template <class RawR>
+__device__ __forceinline__ float green_at(const RawR& R, const View<const float2>& Gr, Quad q, Var2 gv, int x, int y) {
+    if (!(is_site(q, kRed, x, y) || is_site(q, kBlue, x, y))) return R.at(x, y);
+    return green_at_site(R, grad_dir(Gr.at(x, y)), gv, x, y);
+}
+
 __global__ __launch_bounds__(256) void k_interpolate_green(View<const float> raw, View<const float2> grad,
                                                            View<float> green, Quad q, Var2 gv) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -380,10 +402,10 @@ __global__ __launch_bounds__(256) void k_interpolate_green(View<const float> raw
 
 // interpolateRedBluePixel (demosaic.metal:309-380): the missing colour at a
 // red (red_site) or blue site from the colour differences of its diagonal
-// neighbours, steered by the gradient direction.
+// neighbours, steered by the gradient direction (gdir: the site's gradient).
 template <class RawR, class GreenR>
-__device__ __forceinline__ float4 red_blue_site(const RawR& R, const GreenR& Gn, const View<const float2>& Gr, int x,
-                                                int y, bool red_site, Var2 rv, Var2 bv) {
+__device__ __forceinline__ float4 red_blue_site(const RawR& R, const GreenR& Gn, GradDir gdir, int x, int y,
+                                                bool red_site, Var2 rv, Var2 bv) {
     const float g = Gn.at(x, y), c1 = R.at(x, y);
     const float g_tl = fetch(Gn, x - 1, y - 1), g_tr = fetch(Gn, x + 1, y - 1);
     const float g_bl = fetch(Gn, x - 1, y + 1), g_br = fetch(Gn, x + 1, y + 1);
@@ -397,10 +419,8 @@ __device__ __forceinline__ float4 red_blue_site(const RawR& R, const GreenR& Gn,
     const float d2_bl = g2_bl - fetch(R, x - 2, y + 2), d2_br = g2_br - fetch(R, x + 2, y + 2);
     const Var2 v = red_site ? rv : bv;
     const float sd = sqrtf(v.a + v.b * c2_ave);
-    const float2 gg = Gr.at(x, y);
-    const float ax = fabsf(gg.x), ay = fabsf(gg.y);
-    const float dir = 1.0f - 2 * atan2f(ay, ax) / kPi;
-    const float thr = smoothstepf(sd, 4 * sd, sqrtf(ax * ax + ay * ay));
+    const float dir = 1.0f - gdir.t;
+    const float thr = smoothstepf(sd, 4 * sd, gdir.len);
     const float alpha = mixf(0.5f, dir, thr);
     const float highlights = 1.0f - smoothstepf(0.25f, 1.0f, fmaxf(g, fmaxf(fmaxf(g2_tr, g2_bl), fmaxf(g2_tl, g2_br))));
     const float centre = 2 * (g - c1);
@@ -419,10 +439,9 @@ __global__ __launch_bounds__(256) void k_interpolate_red_blue(View<const float> 
     const int qx = blockIdx.x * blockDim.x + threadIdx.x, qy = blockIdx.y * blockDim.y + threadIdx.y;
     if (qx >= rgb.w / 2 || qy >= rgb.h / 2) return;
     const int x = 2 * qx, y = 2 * qy;
-    rgb.at(x + q.o[kRed].x, y + q.o[kRed].y) =
-        red_blue_site(raw, green, grad, x + q.o[kRed].x, y + q.o[kRed].y, true, rv, bv);
-    rgb.at(x + q.o[kBlue].x, y + q.o[kBlue].y) =
-        red_blue_site(raw, green, grad, x + q.o[kBlue].x, y + q.o[kBlue].y, false, rv, bv);
+    const int rx = x + q.o[kRed].x, ry = y + q.o[kRed].y, bx = x + q.o[kBlue].x, by = y + q.o[kBlue].y;
+    rgb.at(rx, ry) = red_blue_site(raw, green, grad_dir(grad.at(rx, ry)), rx, ry, true, rv, bv);
+    rgb.at(bx, by) = red_blue_site(raw, green, grad_dir(grad.at(bx, by)), bx, by, false, rv, bv);
     for (int c : {kGreen, kGreen2}) {
         const int px = x + q.o[c].x, py = y + q.o[c].y;
         rgb.at(px, py) = make_float4(0.0f, green.at(px, py), 0.0f, 0.0f);
@@ -430,10 +449,10 @@ __global__ __launch_bounds__(256) void k_interpolate_red_blue(View<const float> 
 }
 
 // interpolateRedBlueAtGreenPixel (demosaic.metal:420-502).  C reads the
-// interpolateRedBlue output as (red, green, blue, 0).
+// interpolateRedBlue output as (red, green, blue, 0); gdir is the site's
+// gradient.
 template <class RgbR>
-__device__ __forceinline__ float4 red_blue_at_green_site(const RgbR& C, const View<const float2>& Gr, int x, int y,
-                                                         Var2 rv, Var2 bv) {
+__device__ __forceinline__ float4 red_blue_at_green_site(const RgbR& C, GradDir gdir, int x, int y, Var2 rv, Var2 bv) {
     const float gc = C.at(x, y).y;
     const float4 l = fetch(C, x - 1, y), r = fetch(C, x + 1, y), u = fetch(C, x, y - 1), d = fetch(C, x, y + 1);
     const float4 l3 = fetch(C, x - 3, y), r3 = fetch(C, x + 3, y), u3 = fetch(C, x, y - 3), d3 = fetch(C, x, y + 3);
@@ -442,10 +461,8 @@ __device__ __forceinline__ float4 red_blue_at_green_site(const RgbR& C, const Vi
     const float db_l = l.y - l.z, db_r = r.y - r.z, db_u = u.y - u.z, db_d = d.y - d.z;
     const float dr_l3 = l3.y - l3.x, dr_r3 = r3.y - r3.x, dr_u3 = u3.y - u3.x, dr_d3 = d3.y - d3.x;
     const float db_l3 = l3.y - l3.z, db_r3 = r3.y - r3.z, db_u3 = u3.y - u3.z, db_d3 = d3.y - d3.z;
-    const float2 gg = Gr.at(x, y);
-    const float ax = fabsf(gg.x), ay = fabsf(gg.y);
-    const float dir = 2 * atan2f(ay, ax) / kPi;
-    const float glen = sqrtf(ax * ax + ay * ay);
+    const float dir = gdir.t;
+    const float glen = gdir.len;
     const float rsd = sqrtf(rv.a + rv.b * red_ave), bsd = sqrtf(bv.a + bv.b * blue_ave);
     const float ralpha = mixf(0.5f, dir, smoothstepf(rsd, 4 * rsd, glen));
     const float balpha = mixf(0.5f, dir, smoothstepf(bsd, 4 * bsd, glen));
@@ -500,7 +517,9 @@ __global__ __launch_bounds__(256) void k_interpolate_red_blue_at_green(View<cons
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const int px = x + q.o[c].x, py = y + q.o[c].y;
-        out.at(px, py) = (c == kGreen || c == kGreen2) ? red_blue_at_green_site(in, grad, px, py, rv, bv) : in.at(px, py);
+        out.at(px, py) = (c == kGreen || c == kGreen2)
+                             ? red_blue_at_green_site(in, grad_dir(grad.at(px, py)), px, py, rv, bv)
+                             : in.at(px, py);
     }
 }
 
@@ -548,6 +567,13 @@ __device__ __forceinline__ float4 despeckle_at(const YccR& in, int x, int y, Var
     // then (-1, 1), (-1, 0), (-1, -1).
     const int ox[9] = {0, 1, 0, 1, 0, 1, -1, -1, -1}, oy[9] = {-1, -1, 0, 0, 1, 1, 1, 0, -1};
     float cb[9], cr[9];
+    // The reference's running extremes: v > max1 shifts max1 into max2, else
+    // v > max2 replaces max2 (and the mirror image for the minima), from the
+    // starts 0 and 65535.  As max2 <= max1 always, that is max2 = max(max2,
+    // min(v, max1)), max1 = max(max1, v): a NaN neighbour changes nothing in
+    // either form (the IEEE minimum / maximum propagate it into fmaxf / fminf,
+    // which drop it), and the two forms can differ only in the sign of a zero
+    // extreme, which the mix and smoothstep below turn into the same result.
     float max1 = 0.0f, max2 = 0.0f, min1 = 65535.0f, min2 = 65535.0f;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -555,12 +581,10 @@ __device__ __forceinline__ float4 despeckle_at(const YccR& in, int x, int y, Var
         cb[k] = p.y;
         cr[k] = p.z;
         const float v = p.x;
-        max2 = (v <= max1 && v > max2) ? v : max2;
-        max2 = v > max1 ? max1 : max2;
-        max1 = v > max1 ? v : max1;
-        min2 = (v >= min1 && v < min2) ? v : min2;
-        min2 = v < min1 ? min1 : min2;
-        min1 = v < min1 ? v : min1;
+        max2 = fmaxf(max2, __builtin_elementwise_minimum(v, max1));
+        max1 = fmaxf(max1, v);
+        min2 = fminf(min2, __builtin_elementwise_maximum(v, min1));
+        min1 = fminf(min1, v);
     }
     const float centre = in.at(x, y).x;
     const float sigma = sqrtf(yv.a + yv.b * centre);
@@ -617,20 +641,14 @@ struct YccTile {
 };
 __device__ __forceinline__ float4 fetch(const YccTile& t, int x, int y) { return t.at(x, y); }
 
-__device__ __forceinline__ bool is_site(const Quad& q, int c, int x, int y) {
-    return (q.o[c].x & 1) == (x & 1) && (q.o[c].y & 1) == (y & 1);
-}
-
-constexpr int kTileThreads = 256;
-
 // Calls f(x, y) for every position of the w x h region at (ox, oy), w and h
 // even, one 2x2 block per thread and pass: in each of the four calls of a
 // pass the threads of a wave hold positions of one Bayer colour (away from
 // the image border), so the colour branches do not diverge.
-template <class F>
+template <int NT, class F>
 __device__ __forceinline__ void for_each_quad(int ox, int oy, int w, int h, F&& f) {
     const int bw = w / 2, n = bw * (h / 2);
-    for (int b = threadIdx.x; b < n; b += kTileThreads) {
+    for (int b = threadIdx.x; b < n; b += NT) {
         const int x = ox + 2 * (b % bw), y = oy + 2 * (b / bw);
         f(x, y);
         f(x + 1, y);
@@ -650,19 +668,25 @@ struct TileGeometry {
     static constexpr int kW2 = TW + 2 * kH2, kR2 = TH + 2 * kH2;
     static constexpr int kW3 = TW + 2 * kH3, kR3 = TH + 2 * kH3;
     static constexpr int kGreenFloats = kW1 * kR1, kRbFloats = 2 * kW2 * kR2;
-    // The raw plane is dead once red / blue is done; the YCbCr planes reuse it.
-    static constexpr int kRawFloats = kW0 * kR0, kYccFloats = 3 * kW3 * kR3;
-    static constexpr int kFloats = kGreenFloats + kRbFloats + (kRawFloats > kYccFloats ? kRawFloats : kYccFloats);
+    // The gradient of the red and blue sites of the red / blue region (half its
+    // cells, a GradDir each), computed by the green pass and read by the
+    // red / blue pass.  It and the raw plane are dead once red / blue is done;
+    // the YCbCr planes reuse their space.
+    static constexpr int kRawFloats = kW0 * kR0, kDirFloats = kW2 * kR2, kYccFloats = 3 * kW3 * kR3;
+    static constexpr int kReusedFloats =
+        kRawFloats + kDirFloats > kYccFloats ? kRawFloats + kDirFloats : kYccFloats;
+    static constexpr int kFloats = kGreenFloats + kRbFloats + kReusedFloats;
     static_assert(TW % 2 == 0 && TH % 2 == 0, "tiles hold whole 2x2 quads");
-    static_assert(kGreenFloats % 2 == 0, "the (red, blue) plane is float2-aligned");
+    static_assert(kGreenFloats % 2 == 0 && kRawFloats % 2 == 0, "float2 planes are 8-byte aligned");
+    static_assert(sizeof(GradDir) == 2 * sizeof(float), "a GradDir is two floats");
 };
 
 // One workgroup per tile.  The dispatcher deals workgroups to the 8 XCDs in
 // turn, so workgroup b runs on XCD b % 8; tile (b % 8) * per_xcd + b / 8 gives
 // each XCD a contiguous run of tiles in row order, whose halos then meet in
 // that XCD's L2.
-template <int TW, int TH, bool kDespeckle>
-__global__ __launch_bounds__(kTileThreads) void k_demosaic_tile(View<const float> raw, View<const float2> grad,
+template <int TW, int TH, int NT, bool kDespeckle>
+__global__ __launch_bounds__(NT) void k_demosaic_tile(View<const float> raw, View<const float2> grad,
                                                                 View<float4> out, Quad q, Var2 rv, Var2 gv, Var2 bv,
                                                                 float clip, Mat3 t, Var2 yv, int tiles_x, int tiles,
                                                                 int per_xcd) {
@@ -676,37 +700,52 @@ __global__ __launch_bounds__(kTileThreads) void k_demosaic_tile(View<const float
     const Tile<float2> rb{reinterpret_cast<float2*>(lds + G::kGreenFloats), x0 - G::kH2, y0 - G::kH2, G::kW2};
     float* reused = lds + G::kGreenFloats + G::kRbFloats;
     const Tile<float> sraw{reused, x0 - G::kH0, y0 - G::kH0, G::kW0};
+    // dirs: one entry per red / blue site of the red / blue region; the sites
+    // of a row share the parity of x, so x / 2 indexes them.
+    GradDir* dirs = reinterpret_cast<GradDir*>(reused + G::kRawFloats);
+    auto dir_at = [&](int x, int y) -> GradDir& { return dirs[(y - rb.oy) * (G::kW2 / 2) + ((x - rb.ox) >> 1)]; };
     const YccTile ycc{reused, x0 - G::kH3, y0 - G::kH3, G::kW3, G::kW3 * G::kR3};
 
-    for (int i = threadIdx.x; i < G::kW0 * G::kR0; i += kTileThreads) {
+    for (int i = threadIdx.x; i < G::kW0 * G::kR0; i += NT) {
         const int x = sraw.ox + i % G::kW0, y = sraw.oy + i / G::kW0;
         sraw.p[i] = raw.at(clampi(x, 0, W - 1), clampi(y, 0, H - 1));
     }
     __syncthreads();
-    for_each_quad(green.ox, green.oy, G::kW1, G::kR1, [&](int x, int y) {
-        green.at(x, y) = green_at(sraw, grad, q, gv, clampi(x, 0, W - 1), clampi(y, 0, H - 1));
+    for_each_quad<NT>(green.ox, green.oy, G::kW1, G::kR1, [&](int x, int y) {
+        const int cx = clampi(x, 0, W - 1), cy = clampi(y, 0, H - 1);
+        float g;
+        if (is_site(q, kRed, cx, cy) || is_site(q, kBlue, cx, cy)) {
+            const GradDir d = grad_dir(grad.at(cx, cy));
+            g = green_at_site(sraw, d, gv, cx, cy);
+            // the red / blue pass reads the gradient of its in-image sites
+            if (x == cx && y == cy && x - rb.ox >= 0 && x - rb.ox < G::kW2 && y - rb.oy >= 0 && y - rb.oy < G::kR2)
+                dir_at(x, y) = d;
+        } else {
+            g = sraw.at(cx, cy);
+        }
+        green.at(x, y) = g;
     });
     __syncthreads();
-    for_each_quad(rb.ox, rb.oy, G::kW2, G::kR2, [&](int x, int y) {
+    for_each_quad<NT>(rb.ox, rb.oy, G::kW2, G::kR2, [&](int x, int y) {
         const int cx = clampi(x, 0, W - 1), cy = clampi(y, 0, H - 1);
         const bool red = is_site(q, kRed, cx, cy);
         float2 v = make_float2(0.0f, 0.0f);
         if (red || is_site(q, kBlue, cx, cy)) {
-            const float4 c = red_blue_site(sraw, green, grad, cx, cy, red, rv, bv);
+            const float4 c = red_blue_site(sraw, green, dir_at(cx, cy), cx, cy, red, rv, bv);
             v = make_float2(c.x, c.z);
         }
         rb.at(x, y) = v;
     });
     __syncthreads();
     const RgbTile rgb{green, rb};
-    for_each_quad(ycc.ox, ycc.oy, G::kW3, G::kR3, [&](int x, int y) {
+    for_each_quad<NT>(ycc.ox, ycc.oy, G::kW3, G::kR3, [&](int x, int y) {
         const int cx = clampi(x, 0, W - 1), cy = clampi(y, 0, H - 1);
         const bool site = is_site(q, kRed, cx, cy) || is_site(q, kBlue, cx, cy);
-        const float4 v = site ? rgb.at(cx, cy) : red_blue_at_green_site(rgb, grad, cx, cy, rv, bv);
+        const float4 v = site ? rgb.at(cx, cy) : red_blue_at_green_site(rgb, grad_dir(grad.at(cx, cy)), cx, cy, rv, bv);
         ycc.store(x, y, transform3(t, blend_highlight(v, clip)));
     });
     __syncthreads();
-    for (int i = threadIdx.x; i < TW * TH; i += kTileThreads) {
+    for (int i = threadIdx.x; i < TW * TH; i += NT) {
         const int x = x0 + i % TW, y = y0 + i / TW;
         if (x < W && y < H) out.at(x, y) = kDespeckle ? despeckle_at(ycc, x, y, yv) : ycc.at(x, y);
     }
@@ -1207,10 +1246,12 @@ gcam_status raw_nlf_device(View<const float> scaled, const Quad& q, View<float4>
 template <bool kDespeckle>
 gcam_status demosaic_tiles(View<const float> scaled, View<const float2> grad, View<float4> out, const Quad& q,
                            const float rv[6], float clip, const Mat3& t, Var2 yv, hipStream_t s) {
-    constexpr int TW = 64, TH = 32;
+    // 48 x 32 tiles, 512 threads: two workgroups (32 wavefronts) per CU; measured
+    // against 32-64 x 16-64 tiles of 256-1024 threads on the 12 MP frame.
+    constexpr int TW = 48, TH = 32, NT = 512;
     const int tiles_x = (out.w + TW - 1) / TW, tiles = tiles_x * ((out.h + TH - 1) / TH);
     const int per_xcd = (tiles + 7) / 8;
-    k_demosaic_tile<TW, TH, kDespeckle><<<8 * per_xcd, kTileThreads, 0, s>>>(
+    k_demosaic_tile<TW, TH, NT, kDespeckle><<<8 * per_xcd, NT, 0, s>>>(
         scaled, grad, out, q, Var2{rv[0], rv[1]}, Var2{rv[2], rv[3]}, Var2{rv[4], rv[5]}, clip, t, yv, tiles_x, tiles,
         per_xcd);
     return launch_status(kDespeckle ? "demosaic + despeckle" : "demosaic");

@@ -167,9 +167,10 @@ gcam_status gcam_matrix3_inverse(const float m[9], float out[9]);
 /* The front end as one call: RawConverter::demosaic from the raw frame to the
  * YCbCr image (raw_converter.cpp:179-214; the high-noise raw denoise of
  * :195-202 is not part of it), then RawConverter::denoise's despeckle
- * (:70-78).  The Sobel filter is fused into the gradient blur, and
- * interpolateRedBlueAtGreen, blendHighlightsImage and transformImage run as
- * one pass. */
+ * (:70-78).  Three passes: scaleRawData; the gradient blur with the Sobel
+ * filter fused in; interpolateGreen through the YCbCr transform and the
+ * despeckle as one tiled pass (intermediates in on-chip memory), equal bit for
+ * bit to the per-operation entry points in sequence. */
 typedef struct gcam_front_end_params {
     int32_t bayer_pattern;       /* gcam_bayer_pattern                               */
     float black_level;           /* normalised, as for gcam_scale_raw_data           */
@@ -190,8 +191,8 @@ typedef struct gcam_front_end_params {
 
 typedef struct gcam_front_end gcam_front_end;
 
-/* Allocates the intermediates (scaled raw, green, two RGBA images) for one
- * frame size on the current device. */
+/* Allocates the one intermediate image (the scaled raw) for one frame size on
+ * the current device. */
 gcam_status gcam_front_end_create(int32_t width, int32_t height, gcam_front_end** out);
 void gcam_front_end_destroy(gcam_front_end* front_end);
 

@@ -63,7 +63,7 @@ def test_scale_raw_is_bit_exact(bayer, lens):
     np.testing.assert_array_equal(got, want)
 
 
-@pytest.mark.parametrize("w,h", [(256, 192), (250, 130), (66, 18)])
+@pytest.mark.parametrize("w,h", [(256, 192), (250, 130), (66, 18), (64, 32)])
 def test_sobel_bit_exact_and_gradient_blur(w, h):
     raw_d, raw = raw_frame(w, h, 11)
     scaled = oracle.scale_raw(raw, 2, [SCALE_12BIT] * 4)
@@ -111,6 +111,17 @@ def test_highlights_transform_despeckle_are_bit_exact():
     va, vb = (2e-5, 1e-5, 1e-5), (1.5e-3, 5e-4, 5e-4)
     np.testing.assert_array_equal(gcam.despeckleImage(dev(ycc), va, vb).cpu().numpy(),
                                   oracle.despeckle(ycc, va, vb))
+    # luma extremes with NaN neighbours (ignored by the running extremes), exact
+    # zeros of both signs, negative luma and ties
+    odd = ycc.copy()
+    odd[10, 10:14, 0] = np.nan
+    odd[20:23, 30:33, 0] = 0.0
+    odd[21, 31, 0] = -0.0
+    odd[40:43, 50:53, 0] = -0.0
+    odd[60:63, 70:73, 0] = -0.25
+    odd[80:83, 90:93, 0] = 0.5
+    got = gcam.despeckleImage(dev(odd), va, vb).cpu().numpy()
+    np.testing.assert_array_equal(got, oracle.despeckle(odd, va, vb))
 
 
 def params_for(bayer=2, despeckle=True):
@@ -157,11 +168,11 @@ def check_per_stage_chain(raw_d, p, ycc, grad):
     assert torch.equal(y2, ycc)
 
 
-# The front end's demosaic runs as one pass over 64 x 32 tiles with every
-# intermediate in LDS; these sizes cover images smaller than a tile, partial
-# tiles at the right and bottom edges and every Bayer pattern.
+# The front end's demosaic runs as one pass over 48 x 32 tiles with every
+# intermediate in LDS; these sizes cover images smaller than a tile, exact and
+# partial tiles at the right and bottom edges and every Bayer pattern.
 @pytest.mark.parametrize("w,h,bayer,despeckle", [
-    (2, 2, 0, True), (6, 4, 1, False), (66, 34, 3, True), (64, 32, 2, False),
+    (2, 2, 0, True), (6, 4, 1, False), (50, 34, 3, True), (48, 32, 2, False), (96, 64, 1, True),
     (130, 98, 0, False), (1030, 770, 1, True), (4032, 96, 2, True), (200, 1000, 3, False)])
 def test_tiled_demosaic_equals_the_per_stage_chain(w, h, bayer, despeckle):
     print(f"tiled demosaic {w}x{h} bayer {bayer} despeckle {despeckle}", flush=True)
