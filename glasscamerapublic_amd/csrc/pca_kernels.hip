@@ -677,12 +677,20 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
         double* u1 = S.u1[j];
         double* u2 = S.u2[j];
         double* bb = S.Y[j];
-        auto start = [&](int i) { return 1.0 + 0.25 * (double)((i * 7 + j * 3) % 11) / 11.0; };
-        double c0 = S.d[0] - lam, c1 = S.e[0], cb = start(0);
+        // Start vector b_i = 1 + m_i / 44, m_i = (7 i + 3 j) mod 11: fixed, positive and
+        // different per eigenvector (one solve from any such vector suffices), stepped
+        // incrementally so that no division or modulo enters the elimination.
+        int m = 3 * j >= 11 ? 3 * j - 11 : 3 * j;
+        auto next_start = [&]() {
+            const double b = fma((double)m, 1.0 / 44.0, 1.0);
+            m = m >= 4 ? m - 4 : m + 7;
+            return b;
+        };
+        double c0 = S.d[0] - lam, c1 = S.e[0], cb = next_start();
 #pragma unroll
         for (int i = 0; i < N - 1; i++) {
             const double nl = S.e[i], n0 = S.d[i + 1] - lam, n1 = i + 1 < N - 1 ? S.e[i + 1] : 0.0;
-            const double nb = start(i + 1);
+            const double nb = next_start();
             const bool sw = fabs(c0) < fabs(nl);
             const double pv = sw ? nl : (fabs(c0) < tiny ? copysign(tiny, c0) : c0);
             const double rp = rcp_refined(pv);
