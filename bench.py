@@ -51,8 +51,8 @@ BURST_SIZES = [(3264, 2448), (4032, 3024), (5472, 3648), (8064, 6048)]
 # (9x9 window), levels 0..4: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
 # what the counter reads for a plain 256 MiB copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
 MEASURED_BM_TRAFFIC = {
-    ("12mp", "fp16"): [58.1, 52.1, 40.3, 40.7, 43.9],
-    ("12mp", "fp32"): [111.6, 103.7, 81.8, 79.7, 78.7],
+    ("12mp", "fp16"): [32.5, 33.0, 34.1, 36.7, 42.0],
+    ("12mp", "fp32"): [84.2, 65.9, 67.8, 71.6, 80.2],
 }
 # Wave-level VALU instructions per block-matching dispatch on the same workload, levels 0..4:
 # rocprofv3 --pmc SQ_INSTS_VALU, in millions.

@@ -57,12 +57,14 @@ template <int R, int PV, int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void k_block_match(View<const float4> in, View<const float2> grad,
                                                         const float* __restrict__ feat, int feat_stride,
                                                         const NlfDev* __restrict__ nlf, BmParams p, DirTable dir,
-                                                        View<float4> out) {
+                                                        View<float4> out, XcdTiles order) {
     using T = BmTile<R, PV, TW, TH>;
     __shared__ float4 sY[T::LN];   // Y, Cb, Cr, (w)
     __shared__ float4 sF0[T::LN];  // features 0..3
     __shared__ float4 sF1[T::LN];  // features 4..7
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH * PV;
+    int tx, ty;
+    if (!order.tile(tx, ty)) return;  // tiles in XCD-contiguous order
+    const int x0 = tx * TW, y0 = ty * TH * PV;
     const int tid = threadIdx.y * TW + threadIdx.x;
     for (int e = tid; e < T::LN; e += T::THREADS) {
         const int ly = e / T::LW, lx = e % T::LW;
@@ -165,8 +167,9 @@ __global__ __launch_bounds__(TW* TH) void k_block_match(View<const float4> in, V
 template <int R, int PV, int TW, int TH>
 gcam_status launch_shape(View<const float4> in, View<const float2> grad, View<const float> feat8, const NlfDev* nlf,
                          const BmParams& p, const DirTable& dir, View<float4> out, hipStream_t s) {
-    dim3 grid((out.w + TW - 1) / TW, (out.h + TH * PV - 1) / (TH * PV));
-    k_block_match<R, PV, TW, TH><<<grid, dim3(TW, TH), 0, s>>>(in, grad, feat8.ptr, feat8.stride, nlf, p, dir, out);
+    const XcdTiles order = xcd_tiles(out.w, out.h, TW, TH * PV);
+    k_block_match<R, PV, TW, TH><<<order.grid(), dim3(TW, TH), 0, s>>>(in, grad, feat8.ptr, feat8.stride, nlf, p, dir,
+                                                                       out, order);
     GCAM_LAUNCH_CHECK("blockMatchingDenoiseImage");
     return GCAM_OK;
 }

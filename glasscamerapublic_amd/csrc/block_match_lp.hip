@@ -337,12 +337,15 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
     }
 }
 
+// One workgroup per TW x (TH * PV) output tile, tiles in XCD-contiguous order.
 template <int R, int PV, int TW, int TH, class H>
 __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in, View<const float2> grad,
                                                            const float* __restrict__ basis,
                                                            const NlfDev* __restrict__ nlf, BmParams p,
-                                                           DirTable dir, View<float4> out) {
-    bm_lp_tile<R, PV, TW, TH, H>(in, grad, basis, nlf, p, dir, out, blockIdx.x * TW, blockIdx.y * TH * PV);
+                                                           DirTable dir, View<float4> out, XcdTiles order) {
+    int tx, ty;
+    if (!order.tile(tx, ty)) return;
+    bm_lp_tile<R, PV, TW, TH, H>(in, grad, basis, nlf, p, dir, out, tx * TW, ty * TH * PV);
 }
 
 // Burst form: a persistent grid walks the tiles of every frame of a level.
@@ -375,8 +378,8 @@ __global__ __launch_bounds__(TW* TH) void k_burst_block_match_lp(const BurstTabl
 template <int R, int PV, int TW, int TH, class H>
 gcam_status launch_lp_shape(View<const float4> in, View<const float2> grad, const float* basis, const NlfDev* nlf,
                             const BmParams& p, const DirTable& dir, View<float4> out, hipStream_t s) {
-    dim3 grid((out.w + TW - 1) / TW, (out.h + TH * PV - 1) / (TH * PV));
-    k_block_match_lp<R, PV, TW, TH, H><<<grid, dim3(TW, TH), 0, s>>>(in, grad, basis, nlf, p, dir, out);
+    const XcdTiles order = xcd_tiles(out.w, out.h, TW, TH * PV);
+    k_block_match_lp<R, PV, TW, TH, H><<<order.grid(), dim3(TW, TH), 0, s>>>(in, grad, basis, nlf, p, dir, out, order);
     GCAM_LAUNCH_CHECK("blockMatchingDenoiseImage (low precision)");
     return GCAM_OK;
 }

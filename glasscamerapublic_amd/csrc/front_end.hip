@@ -681,20 +681,17 @@ struct TileGeometry {
     static_assert(sizeof(GradDir) == 2 * sizeof(float), "a GradDir is two floats");
 };
 
-// One workgroup per tile.  The dispatcher deals workgroups to the 8 XCDs in
-// turn, so workgroup b runs on XCD b % 8; tile (b % 8) * per_xcd + b / 8 gives
-// each XCD a contiguous run of tiles in row order, whose halos then meet in
-// that XCD's L2.
+// One workgroup per tile, tiles in XCD-contiguous order (their halos meet in
+// one XCD's L2).
 template <int TW, int TH, int NT, bool kDespeckle>
 __global__ __launch_bounds__(NT) void k_demosaic_tile(View<const float> raw, View<const float2> grad,
                                                                 View<float4> out, Quad q, Var2 rv, Var2 gv, Var2 bv,
-                                                                float clip, Mat3 t, Var2 yv, int tiles_x, int tiles,
-                                                                int per_xcd) {
+                                                                float clip, Mat3 t, Var2 yv, XcdTiles order) {
     using G = TileGeometry<TW, TH, kDespeckle>;
     __shared__ float lds[G::kFloats];
-    const int tile = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (tile >= tiles) return;
-    const int x0 = (tile % tiles_x) * TW, y0 = (tile / tiles_x) * TH;
+    int tx, ty;
+    if (!order.tile(tx, ty)) return;
+    const int x0 = tx * TW, y0 = ty * TH;
     const int W = out.w, H = out.h;
     const Tile<float> green{lds, x0 - G::kH1, y0 - G::kH1, G::kW1};
     const Tile<float2> rb{reinterpret_cast<float2*>(lds + G::kGreenFloats), x0 - G::kH2, y0 - G::kH2, G::kW2};
@@ -1249,11 +1246,9 @@ gcam_status demosaic_tiles(View<const float> scaled, View<const float2> grad, Vi
     // 48 x 32 tiles, 512 threads: two workgroups (32 wavefronts) per CU; measured
     // against 32-64 x 16-64 tiles of 256-1024 threads on the 12 MP frame.
     constexpr int TW = 48, TH = 32, NT = 512;
-    const int tiles_x = (out.w + TW - 1) / TW, tiles = tiles_x * ((out.h + TH - 1) / TH);
-    const int per_xcd = (tiles + 7) / 8;
-    k_demosaic_tile<TW, TH, NT, kDespeckle><<<8 * per_xcd, NT, 0, s>>>(
-        scaled, grad, out, q, Var2{rv[0], rv[1]}, Var2{rv[2], rv[3]}, Var2{rv[4], rv[5]}, clip, t, yv, tiles_x, tiles,
-        per_xcd);
+    const XcdTiles order = xcd_tiles(out.w, out.h, TW, TH);
+    k_demosaic_tile<TW, TH, NT, kDespeckle><<<order.grid(), NT, 0, s>>>(
+        scaled, grad, out, q, Var2{rv[0], rv[1]}, Var2{rv[2], rv[3]}, Var2{rv[4], rv[5]}, clip, t, yv, order);
     return launch_status(kDespeckle ? "demosaic + despeckle" : "demosaic");
 }
 }  // namespace

@@ -36,6 +36,31 @@ inline View<P> view(P* p, int w, int h) {
     return View<P>{p, w, h, w};
 }
 
+// XCD-contiguous tile order for a 1-D grid of 8 * per_xcd workgroups over
+// tiles_x x (tiles / tiles_x) tiles.  The dispatcher deals workgroups to the
+// 8 XCDs in turn (workgroup b runs on XCD b % 8), so tile (b % 8) * per_xcd +
+// b / 8 gives each XCD a contiguous run of tiles in row order: the aprons that
+// neighbouring tiles share, left-right and up-down, then meet in that XCD's
+// L2 instead of being fetched once per XCD.
+struct XcdTiles {
+    int tiles_x, tiles, per_xcd;
+    // The grid size of the launch.
+    int grid() const { return 8 * per_xcd; }
+    // This workgroup's tile (column tx, row ty); false for the few workgroups
+    // past the last tile, which must return before any barrier.
+    __device__ __forceinline__ bool tile(int& tx, int& ty) const {
+        const int t = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+        if (t >= tiles) return false;
+        tx = t % tiles_x;
+        ty = t / tiles_x;
+        return true;
+    }
+};
+inline XcdTiles xcd_tiles(int w, int h, int tile_w, int tile_h) {
+    const int tx = (w + tile_w - 1) / tile_w, n = tx * ((h + tile_h - 1) / tile_h);
+    return XcdTiles{tx, n, n > 0 ? (n + 7) / 8 : 1};
+}
+
 // Per-level noise model on the device: var = a + b * Y per channel.
 struct NlfDev {
     float a[3];

@@ -80,8 +80,12 @@ __device__ __forceinline__ void downsample4_px(const View<const float4>& in, con
     out.at(ox, oy) = make_float4(0.25f * acc.x, 0.25f * acc.y, 0.25f * acc.z, 0.0f);
 }
 
-__global__ __launch_bounds__(256) void k_downsample4(View<const float4> in, View<float4> out) {
-    const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
+// The per-pixel kernels run 64 x 4 blocks in XCD-contiguous order: the input
+// rows that vertically adjacent blocks share stay in one XCD's L2.
+__global__ __launch_bounds__(256) void k_downsample4(View<const float4> in, View<float4> out, XcdTiles order) {
+    int tx, ty;
+    if (!order.tile(tx, ty)) return;
+    const int ox = tx * 64 + threadIdx.x, oy = ty * 4 + threadIdx.y;
     if (ox < out.w && oy < out.h) downsample4_px(in, out, ox, oy);
 }
 
@@ -99,8 +103,10 @@ __device__ __forceinline__ void downsample2_px(const View<const float2>& in, con
     out.at(ox, oy) = make_float2(0.25f * acc.x, 0.25f * acc.y);
 }
 
-__global__ __launch_bounds__(256) void k_downsample2(View<const float2> in, View<float2> out) {
-    const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
+__global__ __launch_bounds__(256) void k_downsample2(View<const float2> in, View<float2> out, XcdTiles order) {
+    int tx, ty;
+    if (!order.tile(tx, ty)) return;
+    const int ox = tx * 64 + threadIdx.x, oy = ty * 4 + threadIdx.y;
     if (ox < out.w && oy < out.h) downsample2_px(in, out, ox, oy);
 }
 
@@ -179,8 +185,11 @@ __global__ __launch_bounds__(256) void k_subtract_noise(View<const float4> in, V
                                                         View<const float4> den1, View<const float2> grad,
                                                         float luma_weight, float sharpening, float nlf_x,
                                                         float nlf_y, const NlfDev* nlf_level, float thr_y,
-                                                        View<float4> out, float* __restrict__ patches) {
-    const int ox = blockIdx.x * 64 + threadIdx.x, oy = blockIdx.y * 4 + threadIdx.y;
+                                                        View<float4> out, float* __restrict__ patches,
+                                                        XcdTiles order) {
+    int tx, ty;
+    if (!order.tile(tx, ty)) return;
+    const int ox = tx * 64 + threadIdx.x, oy = ty * 4 + threadIdx.y;
     if (ox < out.w && oy < out.h) {
         const float y = subtract_noise_px(in, in1, den1, grad, luma_weight, sharpening, nlf_x, nlf_y, nlf_level,
                                           thr_y, out, ox, oy);
@@ -430,13 +439,15 @@ inline dim3 grid64x4(int w, int h) { return dim3((w + 63) / 64, (h + 3) / 4); }
 }  // namespace
 
 gcam_status launch_downsample4(View<const float4> in, View<float4> out, hipStream_t s) {
-    k_downsample4<<<grid64x4(out.w, out.h), dim3(64, 4), 0, s>>>(in, out);
+    const XcdTiles order = xcd_tiles(out.w, out.h, 64, 4);
+    k_downsample4<<<order.grid(), dim3(64, 4), 0, s>>>(in, out, order);
     GCAM_LAUNCH_CHECK("downsampleImageXYZ");
     return GCAM_OK;
 }
 
 gcam_status launch_downsample2(View<const float2> in, View<float2> out, hipStream_t s) {
-    k_downsample2<<<grid64x4(out.w, out.h), dim3(64, 4), 0, s>>>(in, out);
+    const XcdTiles order = xcd_tiles(out.w, out.h, 64, 4);
+    k_downsample2<<<order.grid(), dim3(64, 4), 0, s>>>(in, out, order);
     GCAM_LAUNCH_CHECK("downsampleImageXY");
     return GCAM_OK;
 }
@@ -445,8 +456,9 @@ gcam_status launch_subtract_noise(View<const float4> in, View<const float4> in1,
                                   View<const float2> grad, float luma_weight, float sharpening, float nlf_x,
                                   float nlf_y, const NlfDev* nlf_level, float thr_y, View<float4> out,
                                   float* patches, hipStream_t s) {
-    k_subtract_noise<<<grid64x4(out.w, out.h), dim3(64, 4), 0, s>>>(in, in1, den1, grad, luma_weight, sharpening,
-                                                                    nlf_x, nlf_y, nlf_level, thr_y, out, patches);
+    const XcdTiles order = xcd_tiles(out.w, out.h, 64, 4);
+    k_subtract_noise<<<order.grid(), dim3(64, 4), 0, s>>>(in, in1, den1, grad, luma_weight, sharpening, nlf_x, nlf_y,
+                                                          nlf_level, thr_y, out, patches, order);
     GCAM_LAUNCH_CHECK("subtractNoiseImage");
     return GCAM_OK;
 }
