@@ -297,12 +297,10 @@ __global__ __launch_bounds__(kBThreads) void k_blur_sobel(View<const float> raw,
         const int ay = e / kBAW, ax = e % kBAW;
         const int cx = clampi(x0 - kBApL + ax, 0, W - 1), cy = clampi(y0 - kBApL + ay, 0, H - 1);
         if (kFromRaw) {
-            // raw at the clamped neighbours of (cx, cy), which lie in rw's region
-            sb[ay][ax] = sobel_at(
-                [&](int u, int v) {
-                    return rw[clampi(v, 0, H - 1) - (y0 - kBApL - 1)][clampi(u, 0, W - 1) - (x0 - kBApL - 1)];
-                },
-                cx, cy);
+            // raw at the neighbours of (cx, cy): cell (u, v) of rw holds the raw at
+            // the clamped position, and (cx +- 1, cy +- 1) lie in rw's region.
+            sb[ay][ax] = sobel_at([&](int u, int v) { return rw[v - (y0 - kBApL - 1)][u - (x0 - kBApL - 1)]; },
+                                  cx, cy);
         } else {
             sb[ay][ax] = sobel.at(cx, cy);
         }
@@ -324,12 +322,14 @@ __global__ __launch_bounds__(kBThreads) void k_blur_sobel(View<const float> raw,
         any = any || wide[k];
     }
     if (any) blur_sep<HW>(sb, lx, r0 + kBApL, wide_k, wid);
-    const float nw = wide_k.norm;
+    // The separable wide sums already differ from the tap-order sums by FP32
+    // rounding; they are normalized by the reciprocal of the weight sum.
+    const float rnw = 1.0f / wide_k.norm;
 #pragma unroll
     for (int k = 0; k < kBRows; k++) {
         const int y = y0 + r0 + k;
         if (y >= H) break;
-        const float4 res = wide[k] ? make_float4(wid[k].x / nw, wid[k].y / nw, wid[k].z / nw, wid[k].w / nw) : nar[k];
+        const float4 res = wide[k] ? make_float4(wid[k].x * rnw, wid[k].y * rnw, wid[k].z * rnw, wid[k].w * rnw) : nar[k];
         out.at(x, y) = make_float2(copysignf(res.z, res.x), copysignf(res.w, res.y));
     }
 }
@@ -340,6 +340,20 @@ struct Var2 {
     float a, b;
 };
 
+// The demosaic passes (interpolateGreen, interpolateRedBlue,
+// interpolateRedBlueAtGreen) are held to the oracle within FP32 tolerance, not
+// bit for bit: their gradient direction comes from the device atan2f.  Their
+// quotients and square roots use the hardware reciprocal and square root
+// (1 ulp) instead of the correctly rounded sequences.  Divisions by 2, 4 and 8
+// stay exact products.
+__device__ __forceinline__ float dm_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float dm_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float dm_smoothstep(float e0, float e1, float x) {
+    float t = dm_div(x - e0, e1 - e0);
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
 // The gradient as the demosaic passes use it: t = 2 atan2(|gy|, |gx|) / pi
 // (0: horizontal gradient, 1: vertical) and its length.
 struct GradDir {
@@ -347,7 +361,7 @@ struct GradDir {
 };
 __device__ __forceinline__ GradDir grad_dir(float2 g) {
     const float ax = fabsf(g.x), ay = fabsf(g.y);
-    return GradDir{2 * atan2f(ay, ax) / kPi, sqrtf(ax * ax + ay * ay)};
+    return GradDir{dm_div(2 * atan2f(ay, ax), kPi), dm_sqrt(ax * ax + ay * ay)};
 }
 
 __device__ __forceinline__ bool is_site(const Quad& q, int c, int x, int y) {
@@ -358,7 +372,7 @@ __device__ __forceinline__ bool is_site(const Quad& q, int c, int x, int y) {
 // scaled raw (an image or an LDS tile), gdir is the site's gradient.
 template <class RawR>
 __device__ __forceinline__ float green_at_site(const RawR& R, GradDir gdir, Var2 gv, int x, int y) {
-    const float low_noise = 1.0f - smoothstepf(3.5e-4f, 2e-3f, gv.b);
+    const float low_noise = 1.0f - dm_smoothstep(3.5e-4f, 2e-3f, gv.b);
     const float gl = fetch(R, x - 1, y), gr = fetch(R, x + 1, y), gu = fetch(R, x, y - 1), gd = fetch(R, x, y + 1);
     const float c = R.at(x, y);
     const float cl = fetch(R, x - 2, y), cr = fetch(R, x + 2, y), cu = fetch(R, x, y - 2), cd = fetch(R, x, y + 2);
@@ -366,19 +380,19 @@ __device__ __forceinline__ float green_at_site(const RawR& R, GradDir gdir, Var2
     const float g_ave = (gl + gr + gu + gd) / 4;
     const float lf_h = (gl + gr) / 2, lf_v = (gu + gd) / 2;
     const float hf_h = ((cl + cr) - 2 * c) / 4, hf_v = ((cu + cd) - 2 * c) / 4;
-    const float sd = sqrtf(gv.a + gv.b * g_ave);
+    const float sd = dm_sqrt(gv.a + gv.b * g_ave);
     const float glen = gdir.len;
-    const float thr = smoothstepf(sd, 4 * sd, glen);
-    const float low_thr = 1.0f - smoothstepf(2 * sd, 8 * sd, glen);
+    const float thr = dm_smoothstep(sd, 4 * sd, glen);
+    const float low_thr = 1.0f - dm_smoothstep(2 * sd, 8 * sd, glen);
     const float sharpening = (0.5f + 0.5f * low_noise) * thr * (1 + low_noise * low_thr);
-    const float highlights = 1.0f - smoothstepf(0.25f, 1.0f, fmaxf(c, fmaxf(fmaxf(cl, cr), fmaxf(cu, cd))));
+    const float highlights = 1.0f - dm_smoothstep(0.25f, 1.0f, fmaxf(c, fmaxf(fmaxf(cl, cr), fmaxf(cu, cd))));
     float dir = gdir.t;
     if (gv.b < 1e-3f) {
-        dir = dir < 0.5f ? mixf(dir, 0.0f, 1.0f - smoothstepf(0.3f, 0.45f, dir))
-                         : mixf(dir, 1.0f, smoothstepf(1.0f - 0.45f, 1.0f - 0.3f, dir));
+        dir = dir < 0.5f ? mixf(dir, 0.0f, 1.0f - dm_smoothstep(0.3f, 0.45f, dir))
+                         : mixf(dir, 1.0f, dm_smoothstep(1.0f - 0.45f, 1.0f - 0.3f, dir));
     }
     const float cmin = fminf(c, fminf(g_ave, d_ave)), cmax = fmaxf(c, fmaxf(g_ave, d_ave));
-    const float white = cmin / cmax;
+    const float white = dm_div(cmin, cmax);
     const float k = highlights * white * sharpening;
     const float est_h = lf_h - k * hf_h, est_v = lf_v - k * hf_v;
     const float g = mixf(est_v, est_h, dir);
@@ -418,11 +432,11 @@ __device__ __forceinline__ float4 red_blue_site(const RawR& R, const GreenR& Gn,
     const float d2_tl = g2_tl - fetch(R, x - 2, y - 2), d2_tr = g2_tr - fetch(R, x + 2, y - 2);
     const float d2_bl = g2_bl - fetch(R, x - 2, y + 2), d2_br = g2_br - fetch(R, x + 2, y + 2);
     const Var2 v = red_site ? rv : bv;
-    const float sd = sqrtf(v.a + v.b * c2_ave);
+    const float sd = dm_sqrt(v.a + v.b * c2_ave);
     const float dir = 1.0f - gdir.t;
-    const float thr = smoothstepf(sd, 4 * sd, gdir.len);
+    const float thr = dm_smoothstep(sd, 4 * sd, gdir.len);
     const float alpha = mixf(0.5f, dir, thr);
-    const float highlights = 1.0f - smoothstepf(0.25f, 1.0f, fmaxf(g, fmaxf(fmaxf(g2_tr, g2_bl), fmaxf(g2_tl, g2_br))));
+    const float highlights = 1.0f - dm_smoothstep(0.25f, 1.0f, fmaxf(g, fmaxf(fmaxf(g2_tr, g2_bl), fmaxf(g2_tl, g2_br))));
     const float centre = 2 * (g - c1);
     const float anti = (d_tr + d_bl) / 2 + highlights * (d2_tr + d2_bl - centre) / 8;
     const float diag = (d_tl + d_br) / 2 + highlights * (d2_tl + d2_br - centre) / 8;
@@ -463,10 +477,10 @@ __device__ __forceinline__ float4 red_blue_at_green_site(const RgbR& C, GradDir 
     const float db_l3 = l3.y - l3.z, db_r3 = r3.y - r3.z, db_u3 = u3.y - u3.z, db_d3 = d3.y - d3.z;
     const float dir = gdir.t;
     const float glen = gdir.len;
-    const float rsd = sqrtf(rv.a + rv.b * red_ave), bsd = sqrtf(bv.a + bv.b * blue_ave);
-    const float ralpha = mixf(0.5f, dir, smoothstepf(rsd, 4 * rsd, glen));
-    const float balpha = mixf(0.5f, dir, smoothstepf(bsd, 4 * bsd, glen));
-    const float highlights = 1.0f - smoothstepf(0.25f, 1.0f, fmaxf(gc, fmaxf(fmaxf(r.y, l.y), fmaxf(d.y, u.y))));
+    const float rsd = dm_sqrt(rv.a + rv.b * red_ave), bsd = dm_sqrt(bv.a + bv.b * blue_ave);
+    const float ralpha = mixf(0.5f, dir, dm_smoothstep(rsd, 4 * rsd, glen));
+    const float balpha = mixf(0.5f, dir, dm_smoothstep(bsd, 4 * bsd, glen));
+    const float highlights = 1.0f - dm_smoothstep(0.25f, 1.0f, fmaxf(gc, fmaxf(fmaxf(r.y, l.y), fmaxf(d.y, u.y))));
     const float red = gc - mixf((dr_u + dr_d) / 2 - highlights * ((dr_d3 - dr_u) - (dr_d - dr_u3)) / 8,
                                 (dr_l + dr_r) / 2 - highlights * ((dr_r3 - dr_l) - (dr_r - dr_l3)) / 8, ralpha);
     const float blue = gc - mixf((db_u + db_d) / 2 - highlights * ((db_d3 - db_u) - (db_d - db_u3)) / 8,

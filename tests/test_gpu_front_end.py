@@ -11,9 +11,10 @@ which equals the tap evaluation to FP32 rounding: magnitudes within 1e-5
 relative, and the sign (that of the blurred signed gradient) may differ only
 where that sum is within rounding of zero, which the demosaic never sees (it
 reads |gx|, |gy|).  The demosaic takes the gradient direction from atan2f,
-whose device and host implementations differ in the last bits; the direction
-enters continuously (through mix weights), so the demosaic is held to 1e-5
-absolute on values in [0.1, 1].
+whose device and host implementations differ in the last bits, and its
+quotients and square roots from the hardware reciprocal and square root
+(1 ulp); both enter continuously (through mix weights and smoothstep ramps), so
+the demosaic is held to 1e-5 absolute on values in [0.1, 1].
 """
 import numpy as np
 import pytest
