@@ -197,10 +197,14 @@ __global__ __launch_bounds__(256) void k_subtract_noise(View<const float4> in, V
     }
 }
 
-// Burst forms: one launch for the level of every frame; workgroup b covers
-// the 64 x 4 block b - block0 of its frame (row-major, bx blocks a row).
-__global__ __launch_bounds__(256) void k_burst_downsample(const BurstTable<BurstDownFrame>* __restrict__ tab) {
-    const int b = blockIdx.x;
+// Burst forms: one launch for the level of every frame; block b covers the
+// 64 x 4 block b - block0 of its frame (row-major, bx blocks a row).  Blocks
+// run in XCD-contiguous order (`order` spans all frames' blocks), so the rows
+// that vertically adjacent blocks share stay in one XCD's L2.
+__global__ __launch_bounds__(256) void k_burst_downsample(const BurstTable<BurstDownFrame>* __restrict__ tab,
+                                                          XcdTiles order) {
+    int b, unused;
+    if (!order.tile(b, unused)) return;
     const BurstDownFrame& f = tab->frame[burst_frame_of(tab, b)];
     const int lb = b - f.block0;
     const int ox = (lb % f.bx) * 64 + threadIdx.x, oy = (lb / f.bx) * 4 + threadIdx.y;
@@ -210,8 +214,10 @@ __global__ __launch_bounds__(256) void k_burst_downsample(const BurstTable<Burst
     }
 }
 
-__global__ __launch_bounds__(256) void k_burst_subtract(const BurstTable<BurstSubFrame>* __restrict__ tab) {
-    const int b = blockIdx.x;
+__global__ __launch_bounds__(256) void k_burst_subtract(const BurstTable<BurstSubFrame>* __restrict__ tab,
+                                                        XcdTiles order) {
+    int b, unused;
+    if (!order.tile(b, unused)) return;
     const BurstSubFrame& f = tab->frame[burst_frame_of(tab, b)];
     const int lb = b - f.block0;
     const int ox = (lb % f.bx) * 64 + threadIdx.x, oy = (lb / f.bx) * 4 + threadIdx.y;
@@ -500,14 +506,16 @@ gcam_status launch_set_nlf(const gcam_ycbcr_nlf* host_levels, int levels, NlfDev
 
 gcam_status launch_burst_downsample(const BurstTable<BurstDownFrame>* dev, int total_blocks, hipStream_t s) {
     if (total_blocks < 1) return GCAM_OK;
-    k_burst_downsample<<<total_blocks, dim3(64, 4), 0, s>>>(dev);
+    const XcdTiles order = xcd_tiles(total_blocks, 1, 1, 1);
+    k_burst_downsample<<<order.grid(), dim3(64, 4), 0, s>>>(dev, order);
     GCAM_LAUNCH_CHECK("burst downsampleImageXYZ");
     return GCAM_OK;
 }
 
 gcam_status launch_burst_subtract(const BurstTable<BurstSubFrame>* dev, int total_blocks, hipStream_t s) {
     if (total_blocks < 1) return GCAM_OK;
-    k_burst_subtract<<<total_blocks, dim3(64, 4), 0, s>>>(dev);
+    const XcdTiles order = xcd_tiles(total_blocks, 1, 1, 1);
+    k_burst_subtract<<<order.grid(), dim3(64, 4), 0, s>>>(dev, order);
     GCAM_LAUNCH_CHECK("burst subtractNoiseImage");
     return GCAM_OK;
 }
