@@ -87,7 +87,8 @@ class SynthParamsC(ctypes.Structure):
 
 class BurstTuningC(ctypes.Structure):
     _fields_ = [("rows_per_thread", ctypes.c_int32 * 8), ("grid", ctypes.c_int32 * 8),
-                ("schedule", ctypes.c_int32 * 8), ("level_ms", ctypes.c_float * 8), ("tuned", ctypes.c_int32)]
+                ("schedule", ctypes.c_int32 * 8), ("tile_width", ctypes.c_int32 * 8), ("level_ms", ctypes.c_float * 8),
+                ("tuned", ctypes.c_int32)]
 
 
 class LtmParamsC(ctypes.Structure):

@@ -110,12 +110,17 @@ class BurstProcessor:
         check(lib().gcam_burst_get_tuning(self._h, ctypes.byref(t)))
         L = self.levels
         return {"rows_per_thread": list(t.rows_per_thread)[:L], "grid": list(t.grid)[:L],
-                "schedule": list(t.schedule)[:L], "level_ms": list(t.level_ms)[:L], "tuned": bool(t.tuned)}
+                "schedule": list(t.schedule)[:L], "tile_width": list(t.tile_width)[:L],
+                "level_ms": list(t.level_ms)[:L], "tuned": bool(t.tuned)}
 
-    def set_tuning(self, rows_per_thread, grid, schedule) -> None:
+    def set_tuning(self, rows_per_thread, grid, schedule, tile_width=None) -> None:
+        """Fix the block-matching launch per level: rows per thread (1, 2, 4),
+        persistent grid (0: one workgroup per tile), tile order (0, 1) and tile
+        width (32, or 64 with two rows per thread; default 32)."""
         t = BurstTuningC()
         for i in range(self.levels):
             t.rows_per_thread[i] = int(rows_per_thread[i])
             t.grid[i] = int(grid[i])
             t.schedule[i] = int(schedule[i])
+            t.tile_width[i] = 32 if tile_width is None else int(tile_width[i])
         check(lib().gcam_burst_set_tuning(self._h, ctypes.byref(t)))

@@ -109,7 +109,7 @@ struct LpTile {
     static constexpr int LW = FW + 4, LH = FH + 4, LN = LW * LH;
     static constexpr int THREADS = TW * TH, WAVES = THREADS / 64, GROUPS = (FN + 31) / 32;
     static_assert(THREADS % 64 == 0, "whole wavefronts");
-    static_assert(LN * 8 + FN * 36 <= 64 * 1024, "LDS tile too large");
+    static_assert(LN * 8 + FN * 36 <= 160 * 1024, "LDS tile larger than a CU's");
 };
 
 // One output tile with origin (x0, y0); every thread of the workgroup calls
@@ -445,6 +445,7 @@ gcam_status launch_burst_r(const BurstTable<BurstBmFrame>* dev, int total, const
                            const BurstBmConfig& cfg, hipStream_t s) {
     if constexpr (R <= 6) {
         if (cfg.rows == 4) return launch_burst_shape<R, 4, 32, 4, H>(dev, total, dir, cfg, s);
+        if (cfg.rows == 2 && cfg.tile_w == 64) return launch_burst_shape<R, 2, 64, 8, H>(dev, total, dir, cfg, s);
         if (cfg.rows == 2) return launch_burst_shape<R, 2, 32, 8, H>(dev, total, dir, cfg, s);
         return launch_burst_shape<R, 1, 32, 8, H>(dev, total, dir, cfg, s);
     } else {
@@ -473,10 +474,10 @@ gcam_status burst_dispatch(const BurstTable<BurstBmFrame>* dev, int total, int r
 
 }  // namespace
 
-void burst_bm_tile(int radius, int rows, int* tw, int* th) {
+void burst_bm_tile(int radius, int rows, int tile_w, int* tw, int* th) {
     // the shapes of launch_burst_r: 32 x 4 threads (radius > 6, one row each),
-    // 32 x 4 x 4 rows, 32 x 8 x 2 rows, 32 x 8 x 1 row
-    *tw = 32;
+    // 32 x 4 x 4 rows, 32 (or 64) x 8 x 2 rows, 32 x 8 x 1 row
+    *tw = tile_w == 64 && rows == 2 && radius <= 6 ? 64 : 32;
     if (radius > 6) {
         *th = 4;
     } else {

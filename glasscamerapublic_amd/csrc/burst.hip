@@ -119,14 +119,15 @@ gcam_status ensure_gradient(gcam_pyramid* p) {
 }
 
 BurstBmConfig level_config(const gcam_burst* b, int i) {
-    return BurstBmConfig{b->tuning.rows_per_thread[i], b->tuning.grid[i], b->tuning.schedule[i]};
+    return BurstBmConfig{b->tuning.rows_per_thread[i], b->tuning.grid[i], b->tuning.schedule[i],
+                         b->tuning.tile_width[i]};
 }
 
 // Block-matching table of level i for `rows` rows per thread (tile geometry).
-void fill_bm_table(gcam_burst* b, BurstTables* t, int i, int rows, const gcam_denoise_params* params,
+void fill_bm_table(gcam_burst* b, BurstTables* t, int i, int rows, int tile_w, const gcam_denoise_params* params,
                    const gcam_image* frames, const gcam_image* gradients, float lens) {
     int tw = 0, th = 0;
-    burst_bm_tile(b->radius, rows, &tw, &th);
+    burst_bm_tile(b->radius, rows, tile_w, &tw, &th);
     BurstTable<BurstBmFrame>& bt = t->bm[i];
     const BurstTables* d = b->dev;
     int tiles = 0;
@@ -236,7 +237,7 @@ void fill_tables(gcam_burst* b, BurstTables* t, const gcam_denoise_params* param
         b->down_blocks[i] = down;
         b->sub_blocks[i] = sub;
         b->mom_blocks[i] = mom;
-        fill_bm_table(b, t, i, b->tuning.rows_per_thread[i], params, frames, gradients, lens);
+        fill_bm_table(b, t, i, b->tuning.rows_per_thread[i], b->tuning.tile_width[i], params, frames, gradients, lens);
     }
 }
 
@@ -286,7 +287,8 @@ gcam_status time_bm(gcam_burst* b, int i, const BurstBmConfig& cfg, int reps, hi
 // Autotuner: run the burst once with the default launch, then per level time
 // the candidate block-matching launches on the level's real inputs (the
 // launch is idempotent) and keep the fastest: rows per thread first (one
-// workgroup per tile), then the persistent grid size, then the tile order.
+// workgroup per tile, 32-wide tiles), then 64-wide tiles for two rows per
+// thread, then the persistent grid size, then the tile order.
 gcam_status autotune(gcam_burst* b, const gcam_denoise_params* params, const gcam_image* frames,
                      const gcam_image* gradients, const gcam_ycbcr_nlf* nlf, float lens, hipStream_t s) {
     const int L = b->levels;
@@ -295,18 +297,19 @@ gcam_status autotune(gcam_burst* b, const gcam_denoise_params* params, const gca
         b->tuning.rows_per_thread[i] = def_rows;
         b->tuning.grid[i] = 0;
         b->tuning.schedule[i] = 0;
+        b->tuning.tile_width[i] = 32;
     }
     gcam_status st = stage(b, params, frames, gradients, nlf, lens, s);
     if (st == GCAM_OK) st = enqueue(b, s);
     if (st != GCAM_OK) return st;
     for (int i = 0; i < L; i++) {
         const int reps = i == 0 ? 2 : 4;
-        BurstBmConfig best{def_rows, 0, 0};
+        BurstBmConfig best{def_rows, 0, 0, 32};
         float best_ms = 1e30f;
         auto consider = [&](const BurstBmConfig& c) -> gcam_status {
             // the tile table depends on the rows per thread
             GCAM_HIP_TRY(hipEventSynchronize(b->staged));
-            fill_bm_table(b, b->host, i, c.rows, params, frames, gradients, lens);
+            fill_bm_table(b, b->host, i, c.rows, c.tile_w, params, frames, gradients, lens);
             GCAM_HIP_TRY(hipMemcpyAsync(&b->dev->bm[i], &b->host->bm[i], sizeof(b->host->bm[i]),
                                         hipMemcpyHostToDevice, s));
             GCAM_HIP_TRY(hipEventRecord(b->staged, s));
@@ -322,16 +325,18 @@ gcam_status autotune(gcam_burst* b, const gcam_denoise_params* params, const gca
         const int rows_set[3] = {1, 2, 4};
         for (int k = 0; k < 3 && st == GCAM_OK; k++) {
             if (b->radius > 6 && rows_set[k] != 1) continue;
-            st = consider(BurstBmConfig{rows_set[k], 0, 0});
+            st = consider(BurstBmConfig{rows_set[k], 0, 0, 32});
         }
-        const int rows = best.rows;
+        if (st == GCAM_OK && best.rows == 2 && b->radius <= 6) st = consider(BurstBmConfig{2, 0, 0, 64});
+        const int rows = best.rows, tile_w = best.tile_w;
         const int grids[3] = {b->cus, 2 * b->cus, 4 * b->cus};
-        for (int k = 0; k < 3 && st == GCAM_OK; k++) st = consider(BurstBmConfig{rows, grids[k], 0});
-        if (st == GCAM_OK) st = consider(BurstBmConfig{rows, best.grid, 1});
+        for (int k = 0; k < 3 && st == GCAM_OK; k++) st = consider(BurstBmConfig{rows, grids[k], 0, tile_w});
+        if (st == GCAM_OK) st = consider(BurstBmConfig{rows, best.grid, 1, tile_w});
         if (st != GCAM_OK) return st;
         b->tuning.rows_per_thread[i] = best.rows;
         b->tuning.grid[i] = best.grid;
         b->tuning.schedule[i] = best.schedule;
+        b->tuning.tile_width[i] = best.tile_w;
         b->tuning.level_ms[i] = best_ms;
     }
     b->tuning.tuned = 1;
@@ -485,6 +490,10 @@ gcam_status gcam_burst_set_tuning(gcam_burst* b, const gcam_burst_tuning* tuning
         if (tuning->grid[i] < 0) return set_error(GCAM_ERR_INVALID_ARGUMENT, "grid must be >= 0");
         if (tuning->schedule[i] != 0 && tuning->schedule[i] != 1)
             return set_error(GCAM_ERR_INVALID_ARGUMENT, "schedule must be 0 or 1");
+        const int tw = tuning->tile_width[i];
+        if (tw != 32 && tw != 64) return set_error(GCAM_ERR_INVALID_ARGUMENT, "tile_width must be 32 or 64");
+        if (tw == 64 && (r != 2 || b->radius > 6))
+            return set_error(GCAM_ERR_INVALID_ARGUMENT, "64-wide tiles run two rows per thread at radius <= 6");
     }
     DeviceGuard g(b->device);
     if (b->staged) (void)hipEventSynchronize(b->staged);
@@ -493,6 +502,7 @@ gcam_status gcam_burst_set_tuning(gcam_burst* b, const gcam_burst_tuning* tuning
         b->tuning.rows_per_thread[i] = 0;
         b->tuning.grid[i] = 0;
         b->tuning.schedule[i] = 0;
+        b->tuning.tile_width[i] = 0;
     }
     for (int i = 0; i < GCAM_MAX_LEVELS; i++) b->tuning.level_ms[i] = 0.0f;
     b->tuning.tuned = 1;

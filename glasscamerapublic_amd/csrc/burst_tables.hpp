@@ -93,6 +93,7 @@ struct BurstBmConfig {
     int rows;      // output rows per thread: 1, 2 or 4
     int grid;      // workgroups; <= 0: one workgroup per tile
     int schedule;  // 0: tiles in frame order, strided over the grid; 1: XCD-chunked
+    int tile_w;    // output tile width: 32, or 64 (two rows per thread, radius <= 6)
 };
 
 gcam_status launch_burst_downsample(const BurstTable<BurstDownFrame>* dev, int total_blocks, hipStream_t s);
@@ -102,7 +103,7 @@ gcam_status launch_burst_pca(const BurstTable<BurstMomFrame>* dev, int total_blo
 // Workgroups of the moment reduction for npatch patches (1 .. kMomentBlocks).
 int moment_block_count(int64_t npatch);
 // Output tile of the persistent kernel for `rows` rows per thread: tw x th pixels.
-void burst_bm_tile(int radius, int rows, int* tw, int* th);
+void burst_bm_tile(int radius, int rows, int tile_w, int* tw, int* th);
 gcam_status launch_burst_block_match_lp(const BurstTable<BurstBmFrame>* dev, int total_tiles, int radius,
                                         int precision, const BurstBmConfig& cfg, hipStream_t s);
 

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define GCAM_ABI_VERSION 2
+#define GCAM_ABI_VERSION 3
 #define GCAM_MAX_LEVELS 8
 #define GCAM_MAX_SEARCH_RADIUS 10
 #define GCAM_MAX_BURST 64
@@ -204,6 +204,8 @@ typedef struct gcam_burst_tuning {
     int32_t grid[GCAM_MAX_LEVELS];            /* persistent workgroups; 0 = one per tile */
     int32_t schedule[GCAM_MAX_LEVELS];        /* 0: tiles in frame order, strided over the
                                                  grid; 1: contiguous chunks per XCD     */
+    int32_t tile_width[GCAM_MAX_LEVELS];      /* output tile width: 32, or 64 with 2 rows
+                                                 per thread (radius <= 6)               */
     float level_ms[GCAM_MAX_LEVELS];          /* measured block-matching time (output)  */
     int32_t tuned;                            /* 1 once chosen (output)                  */
 } gcam_burst_tuning;

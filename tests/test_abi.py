@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     L = gcam.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    assert L.gcam_abi_version() == 2
+    assert L.gcam_abi_version() == 3
 
 
 def test_status_strings():

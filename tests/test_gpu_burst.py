@@ -106,6 +106,37 @@ def test_autotuned_burst_and_graph_replays():
                 assert err < 1e-3, f"replay {rep} frame {k} level {i}: {err}"
 
 
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_64_wide_tiles(precision):
+    """64-wide block-matching tiles (a tuner candidate for two rows per thread):
+    the persistent XCD-chunked launch equals the one-workgroup-per-tile launch
+    bit for bit, and both stay within the FP16/BF16 feature rounding of the
+    32-wide tiles (features are centred on each tile's first pixel)."""
+    gains = [2e-3, 7e-3, 4e-3, 3e-3]
+    frames = frames_for(250, gains)
+    params = [frame_params(k) for k in range(len(SIZES))]
+    nlfs = [frame_nlf(a) for a in gains]
+    burst = gcam.BurstProcessor(SIZES, levels=LEVELS, search_radius=RADIUS, precision=precision)
+    burst.set_tuning([2] * LEVELS, [0] * LEVELS, [0] * LEVELS)
+    burst.denoise(params, frames, nlfs)
+    torch.cuda.synchronize()
+    narrow = [[burst.output(k, i).clone() for i in range(LEVELS)] for k in range(len(SIZES))]
+    burst.set_tuning([2] * LEVELS, [0] * LEVELS, [0] * LEVELS, [64] * LEVELS)
+    assert burst.tuning["tile_width"] == [64] * LEVELS
+    burst.denoise(params, frames, nlfs)
+    torch.cuda.synchronize()
+    wide = [[burst.output(k, i).clone() for i in range(LEVELS)] for k in range(len(SIZES))]
+    burst.set_tuning([2] * LEVELS, [512] * LEVELS, [1] * LEVELS, [64] * LEVELS)
+    burst.denoise(params, frames, nlfs)
+    torch.cuda.synchronize()
+    tol = 1e-3 if precision == "fp16" else 3e-3
+    for k in range(len(SIZES)):
+        for i in range(LEVELS):
+            assert torch.equal(burst.output(k, i), wide[k][i]), f"frame {k} level {i}"
+            err = (wide[k][i][..., :3] - narrow[k][i][..., :3]).abs().max().item()
+            assert err < tol, f"frame {k} level {i}: {err}"
+
+
 def test_bf16_burst_matches_single_frame():
     gains = [4e-3] * 4
     frames = frames_for(600, gains)
@@ -134,6 +165,10 @@ def test_burst_argument_errors():
         burst.denoise(params, bad, nlfs)
     with pytest.raises(gcam.GcamError):
         burst.set_tuning([3] * LEVELS, [0] * LEVELS, [0] * LEVELS)
+    with pytest.raises(gcam.GcamError, match="tile_width"):
+        burst.set_tuning([2] * LEVELS, [0] * LEVELS, [0] * LEVELS, [48] * LEVELS)
+    with pytest.raises(gcam.GcamError, match="64-wide"):
+        burst.set_tuning([1] * LEVELS, [0] * LEVELS, [0] * LEVELS, [64] * LEVELS)
     with pytest.raises(ValueError):
         burst.denoise(params[:2], frames, nlfs)
 
