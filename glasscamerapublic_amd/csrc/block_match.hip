@@ -21,12 +21,22 @@
 #include "gcam_internal.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace gcam {
 
 namespace {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// f(std::integral_constant<int, i>) for i = I .. E (nothing when E < I).
+template <int I, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I <= E) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, E>(f);
+    }
+}
 
 __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
     float t = (x - e0) / (e1 - e0);
@@ -121,17 +131,27 @@ __global__ __launch_bounds__(TW* TH) void k_block_match(View<const float4> in, V
     // staged neighbour is loaded once and used by every owned pixel whose
     // window contains it.  Summation order per pixel: rows ascending, columns
     // ascending, as the reference's loops (demosaic.metal:858-859).
-#pragma unroll 1
-    for (int dy = -R; dy <= R + PV - 1; dy++) {
+    // One window row dy against the owned outputs k = KLO .. KHI (those whose
+    // window holds the row).
+    auto window_row = [&](auto klo, auto khi, int dy, auto check) {
+        constexpr int KLO = decltype(klo)::value, KHI = decltype(khi)::value;
         const int rowq = (threadIdx.y * PV + R + dy) * T::LW + threadIdx.x + R;
+        // Operands of position dx are loaded while position dx - 1 is computed;
+        // the scheduling barrier keeps the compiler from hoisting the loads of
+        // the whole row (registers, and so occupancy).
+        float4 na = sF0[rowq - R], nb = sF1[rowq - R], ny = sY[rowq - R];
 #pragma unroll
         for (int dx = -R; dx <= R; dx++) {
-            const int q = rowq + dx;
-            const float4 qa = sF0[q], qb = sF1[q], qy = sY[q];
+            const float4 qa = na, qb = nb, qy = ny;
+            if (dx < R) {
+                na = sF0[rowq + dx + 1];
+                nb = sF1[rowq + dx + 1];
+                ny = sY[rowq + dx + 1];
+            }
 #pragma unroll
-            for (int k = 0; k < PV; k++) {
+            for (int k = KLO; k <= KHI; k++) {
                 const int ddy = dy - k;
-                if (ddy < -R || ddy > R) continue;  // wave-uniform
+                if (decltype(check)::value && (ddy < -R || ddy > R)) continue;  // wave-uniform
                 const int o2 = dx * dx + ddy * ddy;
                 const float wd = dir.w[o2], lwd = dir.lw[o2];
                 float t, d2;
@@ -156,7 +176,38 @@ __global__ __launch_bounds__(TW* TH) void k_block_match(View<const float4> in, V
                 fCr[k] = fmaf(wC, qy.z, fCr[k]);
                 nC[k] += wC;
             }
+            __builtin_amdgcn_sched_barrier(0);
         }
+    };
+    // Window rows dy = -R .. R + PV - 1 of the PV outputs; output k takes the
+    // rows with dy - k in [-R, R].  With one or two outputs per thread the
+    // first and last PV - 1 rows (where only some outputs take part) are
+    // unrolled with their outputs fixed at compile time and the band between
+    // them, where all take part, is a loop with no per-output test.  With
+    // four, the unrolled edge rows would hold too many registers: every row
+    // runs through the loop and tests its outputs (a wave-uniform test).
+    if constexpr (PV <= 2) {
+        constexpr int kMidLo = -R + PV - 1;
+        constexpr int kBotLo = R + 1 > kMidLo ? R + 1 : kMidLo;
+        static_for<-R, (kMidLo - 1 < R ? kMidLo - 1 : R)>([&](auto d) {
+            constexpr int dy = decltype(d)::value;
+            window_row(std::integral_constant<int, (dy - R > 0 ? dy - R : 0)>{},
+                       std::integral_constant<int, (dy + R < PV - 1 ? dy + R : PV - 1)>{}, dy,
+                           std::false_type{});
+        });
+#pragma unroll 1
+        for (int dy = kMidLo; dy <= R; dy++)
+            window_row(std::integral_constant<int, 0>{}, std::integral_constant<int, PV - 1>{}, dy, std::false_type{});
+        static_for<kBotLo, R + PV - 1>([&](auto d) {
+            constexpr int dy = decltype(d)::value;
+            window_row(std::integral_constant<int, (dy - R > 0 ? dy - R : 0)>{},
+                       std::integral_constant<int, (dy + R < PV - 1 ? dy + R : PV - 1)>{}, dy,
+                           std::false_type{});
+        });
+    } else {
+#pragma unroll 1
+        for (int dy = -R; dy <= R + PV - 1; dy++)
+            window_row(std::integral_constant<int, 0>{}, std::integral_constant<int, PV - 1>{}, dy, std::true_type{});
     }
 #pragma unroll
     for (int k = 0; k < PV; k++) {

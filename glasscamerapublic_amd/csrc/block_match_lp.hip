@@ -82,6 +82,15 @@ struct Prec<__bf16> {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// f(std::integral_constant<int, i>) for i = I .. E (nothing when E < I).
+template <int I, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I <= E) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, E>(f);
+    }
+}
+
 // Two 16-bit values in one word, the first in the low half.
 template <class H>
 __device__ __forceinline__ uint32_t pack2(H a, H b) {
@@ -291,43 +300,86 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
             fY[k] = fCb[k] = fCr[k] = nY[k] = nC[k] = 0.0f;
         }
 
-#pragma unroll 1
-        for (int dy = -R; dy <= R + PV - 1; dy++) {
+        // One window row dy: every position q of the row against the outputs
+        // k = KLO .. KHI of this thread (those whose window holds the row).
+        auto window_row = [&](auto klo, auto khi, int dy, auto check) {
+            constexpr int KLO = decltype(klo)::value, KHI = decltype(khi)::value;
             const int rowq = (threadIdx.y * PV + R + dy) * T::FW + threadIdx.x + R;
+            // Operands of position dx are loaded while position dx - 1 is
+            // computed; the scheduling barrier keeps the compiler from hoisting
+            // the loads of the whole row (registers, and so occupancy).
+            h8 fq = fh[rowq - R];
+            float4 qy = ycn[rowq - R];
+            float f0q = f0s[rowq - R];
 #pragma unroll
             for (int dx = -R; dx <= R; dx++) {
-                const int q = rowq + dx;
-                const h8 fq = fh[q];
-                const float4 qy = ycn[q];
-                const float f0q = f0s[q];
+                const h8 fq_c = fq;
+                const float4 qy_c = qy;
+                const float f0q_c = f0q;
+                if (dx < R) {
+                    fq = fh[rowq + dx + 1];
+                    qy = ycn[rowq + dx + 1];
+                    f0q = f0s[rowq + dx + 1];
+                }
 #pragma unroll
-                for (int k = 0; k < PV; k++) {
+                for (int k = KLO; k <= KHI; k++) {
                     const int ddy = dy - k;
-                    if (ddy < -R || ddy > R) continue;  // wave-uniform
+                    if (decltype(check)::value && (ddy < -R || ddy > R)) continue;  // wave-uniform
                     const int o2 = dx * dx + ddy * ddy;
                     const float wd = dir.w[o2], lwd = dir.lw[o2];
                     // acc = f_p . f_q - |f_q|^2 / 2 over components 1..7, so that
                     // k2 acc + kl |f_p|^2 = kl |f_q - f_p|^2 (Gram form)
-                    float acc = P::dot2(h2{fq[0], fq[1]}, h2{fp[k][0], fp[k][1]}, qy.w);
-                    acc = P::dot2(h2{fq[2], fq[3]}, h2{fp[k][2], fp[k][3]}, acc);
-                    acc = P::dot2(h2{fq[4], fq[5]}, h2{fp[k][4], fp[k][5]}, acc);
-                    acc = P::dot2(h2{fq[6], fq[7]}, h2{fp[k][6], fp[k][7]}, acc);
-                    const float t0 = fmaf(f0q, r1[k], pr[k]);
+                    float acc = P::dot2(h2{fq_c[0], fq_c[1]}, h2{fp[k][0], fp[k][1]}, qy_c.w);
+                    acc = P::dot2(h2{fq_c[2], fq_c[3]}, h2{fp[k][2], fp[k][3]}, acc);
+                    acc = P::dot2(h2{fq_c[4], fq_c[5]}, h2{fp[k][4], fp[k][5]}, acc);
+                    acc = P::dot2(h2{fq_c[6], fq_c[7]}, h2{fp[k][6], fp[k][7]}, acc);
+                    const float t0 = fmaf(f0q_c, r1[k], pr[k]);
                     // wl' = sqrt(2 log2 e) wl; wd holds w / sqrt(2 log2 e)
                     const float wl = __builtin_amdgcn_exp2f(fmaf(k2[k], acc, fmaf(-t0, t0, cp[k])));
                     const float wY = wd * wl;
-                    const float tb = fmaf(sb[k], qy.y, -sbc[k]), tr = fmaf(sr[k], qy.z, -src[k]);
+                    const float tb = fmaf(sb[k], qy_c.y, -sbc[k]), tr = fmaf(sr[k], qy_c.z, -src[k]);
                     float e = fmaf(-tb, tb, lwd);
                     e = fmaf(-tr, tr, e);
                     e = fmaf(-wl, wl, e);
                     const float wC = __builtin_amdgcn_exp2f(e);
-                    fY[k] = fmaf(wY, qy.x, fY[k]);
+                    fY[k] = fmaf(wY, qy_c.x, fY[k]);
                     nY[k] += wY;
-                    fCb[k] = fmaf(wC, qy.y, fCb[k]);
-                    fCr[k] = fmaf(wC, qy.z, fCr[k]);
+                    fCb[k] = fmaf(wC, qy_c.y, fCb[k]);
+                    fCr[k] = fmaf(wC, qy_c.z, fCr[k]);
                     nC[k] += wC;
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
+        };
+        // Window rows dy = -R .. R + PV - 1 of the PV outputs; output k takes the
+        // rows with dy - k in [-R, R].  With one or two outputs per thread the
+        // first and last PV - 1 rows (where only some outputs take part) are
+        // unrolled with their outputs fixed at compile time and the band between
+        // them, where all take part, is a loop with no per-output test.  With
+        // four, the unrolled edge rows would hold too many registers: every row
+        // runs through the loop and tests its outputs (a wave-uniform test).
+        if constexpr (PV <= 2) {
+            constexpr int kMidLo = -R + PV - 1;                  // first row of every output
+            constexpr int kBotLo = R + 1 > kMidLo ? R + 1 : kMidLo;  // first row after the band
+            static_for<-R, (kMidLo - 1 < R ? kMidLo - 1 : R)>([&](auto d) {
+                constexpr int dy = decltype(d)::value;
+                window_row(std::integral_constant<int, (dy - R > 0 ? dy - R : 0)>{},
+                           std::integral_constant<int, (dy + R < PV - 1 ? dy + R : PV - 1)>{}, dy,
+                           std::false_type{});
+            });
+#pragma unroll 1
+            for (int dy = kMidLo; dy <= R; dy++)
+                window_row(std::integral_constant<int, 0>{}, std::integral_constant<int, PV - 1>{}, dy, std::false_type{});
+            static_for<kBotLo, R + PV - 1>([&](auto d) {
+                constexpr int dy = decltype(d)::value;
+                window_row(std::integral_constant<int, (dy - R > 0 ? dy - R : 0)>{},
+                           std::integral_constant<int, (dy + R < PV - 1 ? dy + R : PV - 1)>{}, dy,
+                           std::false_type{});
+            });
+        } else {
+#pragma unroll 1
+            for (int dy = -R; dy <= R + PV - 1; dy++)
+                window_row(std::integral_constant<int, 0>{}, std::integral_constant<int, PV - 1>{}, dy, std::true_type{});
         }
 #pragma unroll
         for (int k = 0; k < PV; k++) {
@@ -348,15 +400,28 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
     bm_lp_tile<R, PV, TW, TH, H>(in, grad, basis, nlf, p, dir, out, tx * TW, ty * TH * PV);
 }
 
-// Burst form: a persistent grid walks the tiles of every frame of a level.
-// schedule 0: workgroup b takes tiles b, b + G, b + 2G, ... (G = grid size);
-// schedule 1: workgroups run on XCD b % 8, so XCD x takes the contiguous
-// x-th eighth of the tiles (neighbouring tiles, whose aprons overlap, share
-// that XCD's L2) and its workgroups stride through it.  Every workgroup
-// leaves the loop when its tiles run out.
-template <int R, int PV, int TW, int TH, class H>
+// Burst form over the tiles of every frame of a level.  kLoop false: one
+// workgroup per tile (workgroup b takes tile b).  kLoop true, a persistent
+// grid: schedule 0: workgroup b takes tiles b, b + G, b + 2G, ... (G = grid
+// size); schedule 1: workgroups run on XCD b % 8, so XCD x takes the
+// contiguous x-th eighth of the tiles (neighbouring tiles, whose aprons
+// overlap, share that XCD's L2) and its workgroups stride through it.  Every
+// workgroup leaves the loop when its tiles run out.  The one-tile form is a
+// separate instance because across a tile loop the compiler keeps the
+// direction-table values of the unrolled window rows in registers, which
+// costs wavefronts per SIMD.
+template <int R, int PV, int TW, int TH, class H, bool kLoop>
 __global__ __launch_bounds__(TW* TH) void k_burst_block_match_lp(const BurstTable<BurstBmFrame>* __restrict__ tab,
                                                                  DirTable dir, int schedule) {
+    if constexpr (!kLoop) {
+        const int item = blockIdx.x;
+        if (item >= tab->total) return;
+        const BurstBmFrame& f = tab->frame[burst_tile_frame(tab, item)];
+        const int lt = item - f.tile0;
+        bm_lp_tile<R, PV, TW, TH, H>(f.in, f.grad, f.basis, f.nlf, f.p, dir, f.out, (lt % f.tiles_x) * TW,
+                                     (lt / f.tiles_x) * (TH * PV));
+        return;
+    }
     const int total = tab->total, G = gridDim.x;
     int first = blockIdx.x, step = G, end = total;
     if (schedule == 1) {
@@ -435,7 +500,10 @@ gcam_status launch_burst_shape(const BurstTable<BurstBmFrame>* dev, int total, c
     int grid = cfg.grid > 0 ? std::min(cfg.grid, total) : total;
     if (cfg.schedule == 1 && grid < 8) grid = std::min(8, total);  // every XCD chunk needs a workgroup
     if (grid < 1) return GCAM_OK;
-    k_burst_block_match_lp<R, PV, TW, TH, H><<<grid, dim3(TW, TH), 0, s>>>(dev, dir, cfg.schedule);
+    if (grid == total && cfg.schedule == 0)
+        k_burst_block_match_lp<R, PV, TW, TH, H, false><<<grid, dim3(TW, TH), 0, s>>>(dev, dir, cfg.schedule);
+    else
+        k_burst_block_match_lp<R, PV, TW, TH, H, true><<<grid, dim3(TW, TH), 0, s>>>(dev, dir, cfg.schedule);
     GCAM_LAUNCH_CHECK("burst blockMatchingDenoiseImage");
     return GCAM_OK;
 }
