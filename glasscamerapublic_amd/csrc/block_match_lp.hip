@@ -103,10 +103,7 @@ __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
     return t * t * (3.0f - 2.0f * t);
 }
 
-struct DirTable {
-    float w[2 * GCAM_MAX_SEARCH_RADIUS * GCAM_MAX_SEARCH_RADIUS + 1];
-    float lw[2 * GCAM_MAX_SEARCH_RADIUS * GCAM_MAX_SEARCH_RADIUS + 1];
-};
+using DirTable = BmDirTable;
 
 constexpr float kLog2e = 1.4426950408889634f;
 
@@ -127,7 +124,7 @@ template <int R, int PV, int TW, int TH, class H>
 __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const View<const float2>& grad,
                                            const float* __restrict__ basis, const NlfDev* __restrict__ nlf,
                                            const BmParams& p, const DirTable& dir, const View<float4>& out, int x0,
-                                           int y0) {
+                                           int y0, int tx = threadIdx.x, int ty = threadIdx.y) {
     using T = LpTile<R, PV, TW, TH>;
     using P = Prec<H>;
     using h2 = typename P::h2;
@@ -141,7 +138,7 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
     __shared__ float4 ycn[T::FN];  // Y, Cb, Cr, -|f_1..7|^2 / 2 (scaled, rounded)
     __shared__ float f0s[T::FN];   // component 0, FP32
     __shared__ h8 fh[T::FN];       // components 1..7 rounded, 0
-    const int tid = threadIdx.y * TW + threadIdx.x;
+    const int tid = ty * TW + tx;
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     // Level scale ~ 1/sigma of mid-grey luma; centre = the tile's first pixel.
@@ -253,7 +250,7 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
     }
     __syncthreads();
 
-    const int x = x0 + threadIdx.x, yb = y0 + threadIdx.y * PV;
+    const int x = x0 + tx, yb = y0 + ty * PV;
     if (x < out.w && yb < out.h) {
         h8 fp[PV];
         float r1[PV], pr[PV], kl[PV], k2[PV], cp[PV], sb[PV], sr[PV], sbc[PV], src[PV];
@@ -265,7 +262,7 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
 #pragma unroll
         for (int k = 0; k < PV; k++) {
             const int y = min(yb + k, out.h - 1);
-            const int c = (threadIdx.y * PV + k + R) * T::FW + threadIdx.x + R;
+            const int c = (ty * PV + k + R) * T::FW + tx + R;
             const float4 v = ycn[c];
             fp[k] = fh[c];
             const float f0p = f0s[c];
@@ -304,7 +301,7 @@ __device__ __forceinline__ void bm_lp_tile(const View<const float4>& in, const V
         // k = KLO .. KHI of this thread (those whose window holds the row).
         auto window_row = [&](auto klo, auto khi, int dy, auto check) {
             constexpr int KLO = decltype(klo)::value, KHI = decltype(khi)::value;
-            const int rowq = (threadIdx.y * PV + R + dy) * T::FW + threadIdx.x + R;
+            const int rowq = (ty * PV + R + dy) * T::FW + tx + R;
             // Operands of position dx are loaded while position dx - 1 is
             // computed; the scheduling barrier keeps the compiler from hoisting
             // the loads of the whole row (registers, and so occupancy).
@@ -406,19 +403,21 @@ __global__ __launch_bounds__(TW* TH) void k_block_match_lp(View<const float4> in
 // size); schedule 1: workgroups run on XCD b % 8, so XCD x takes the
 // contiguous x-th eighth of the tiles (neighbouring tiles, whose aprons
 // overlap, share that XCD's L2) and its workgroups stride through it.  Every
-// workgroup leaves the loop when its tiles run out.  The one-tile form is a
-// separate instance because across a tile loop the compiler keeps the
-// direction-table values of the unrolled window rows in registers, which
-// costs wavefronts per SIMD.
+// workgroup leaves the loop when its tiles run out.  Across a tile loop the
+// compiler would keep what it can compute once (addresses from the thread
+// coordinates, the direction-table values of the unrolled window rows) in
+// registers for the whole loop, which costs wavefronts per SIMD: an empty
+// asm re-defines the thread coordinates and the table pointer per tile, and
+// the one-tile form is an instance of its own.
 template <int R, int PV, int TW, int TH, class H, bool kLoop>
 __global__ __launch_bounds__(TW* TH) void k_burst_block_match_lp(const BurstTable<BurstBmFrame>* __restrict__ tab,
-                                                                 DirTable dir, int schedule) {
+                                                                 const DirTable* __restrict__ dirp, int schedule) {
     if constexpr (!kLoop) {
         const int item = blockIdx.x;
         if (item >= tab->total) return;
         const BurstBmFrame& f = tab->frame[burst_tile_frame(tab, item)];
         const int lt = item - f.tile0;
-        bm_lp_tile<R, PV, TW, TH, H>(f.in, f.grad, f.basis, f.nlf, f.p, dir, f.out, (lt % f.tiles_x) * TW,
+        bm_lp_tile<R, PV, TW, TH, H>(f.in, f.grad, f.basis, f.nlf, f.p, *dirp, f.out, (lt % f.tiles_x) * TW,
                                      (lt / f.tiles_x) * (TH * PV));
         return;
     }
@@ -432,10 +431,12 @@ __global__ __launch_bounds__(TW* TH) void k_burst_block_match_lp(const BurstTabl
         end = min(total, (x + 1) * chunk);
     }
     for (int item = first; item < end; item += step) {
+        int tx = threadIdx.x, ty = threadIdx.y;
+        asm volatile("" : "+s"(dirp), "+v"(tx), "+v"(ty));
         const BurstBmFrame& f = tab->frame[burst_tile_frame(tab, item)];
         const int lt = item - f.tile0;
-        bm_lp_tile<R, PV, TW, TH, H>(f.in, f.grad, f.basis, f.nlf, f.p, dir, f.out, (lt % f.tiles_x) * TW,
-                                     (lt / f.tiles_x) * (TH * PV));
+        bm_lp_tile<R, PV, TW, TH, H>(f.in, f.grad, f.basis, f.nlf, f.p, *dirp, f.out, (lt % f.tiles_x) * TW,
+                                     (lt / f.tiles_x) * (TH * PV), tx, ty);
         __syncthreads();
     }
 }
@@ -461,24 +462,10 @@ gcam_status launch_lp_r(View<const float4> in, View<const float2> grad, const fl
     }
 }
 
-// w: the direction weight divided by sqrt(2 log2 e) (the luma weight enters
-// pre-scaled by that factor); lw: log2 of the direction weight itself.
-DirTable make_dir_table() {
-    DirTable t{};
-    const float kroot = std::sqrt(2.0f * kLog2e);
-    for (int d2 = 0; d2 < (int)(sizeof(t.w) / sizeof(float)); d2++) {
-        const float a = 0.1f * std::sqrt((float)d2);
-        const float w = std::exp(-2.0f * a * a);
-        t.w[d2] = w / kroot;
-        t.lw[d2] = std::log2(w);
-    }
-    return t;
-}
-
 template <class H>
 gcam_status dispatch(View<const float4> in, View<const float2> grad, const float* basis, const NlfDev* nlf,
                      const BmParams& p, int radius, int rows, View<float4> out, hipStream_t s) {
-    static const DirTable dir = make_dir_table();
+    static const DirTable dir = make_bm_dir_table();
     switch (radius) {
         case 1: return launch_lp_r<1, H>(in, grad, basis, nlf, p, dir, rows, out, s);
         case 2: return launch_lp_r<2, H>(in, grad, basis, nlf, p, dir, rows, out, s);
@@ -495,7 +482,7 @@ gcam_status dispatch(View<const float4> in, View<const float2> grad, const float
 }
 
 template <int R, int PV, int TW, int TH, class H>
-gcam_status launch_burst_shape(const BurstTable<BurstBmFrame>* dev, int total, const DirTable& dir,
+gcam_status launch_burst_shape(const BurstTable<BurstBmFrame>* dev, int total, const DirTable* dir,
                                const BurstBmConfig& cfg, hipStream_t s) {
     int grid = cfg.grid > 0 ? std::min(cfg.grid, total) : total;
     if (cfg.schedule == 1 && grid < 8) grid = std::min(8, total);  // every XCD chunk needs a workgroup
@@ -509,7 +496,7 @@ gcam_status launch_burst_shape(const BurstTable<BurstBmFrame>* dev, int total, c
 }
 
 template <int R, class H>
-gcam_status launch_burst_r(const BurstTable<BurstBmFrame>* dev, int total, const DirTable& dir,
+gcam_status launch_burst_r(const BurstTable<BurstBmFrame>* dev, int total, const DirTable* dir,
                            const BurstBmConfig& cfg, hipStream_t s) {
     if constexpr (R <= 6) {
         if (cfg.rows == 4) return launch_burst_shape<R, 4, 32, 4, H>(dev, total, dir, cfg, s);
@@ -522,9 +509,8 @@ gcam_status launch_burst_r(const BurstTable<BurstBmFrame>* dev, int total, const
 }
 
 template <class H>
-gcam_status burst_dispatch(const BurstTable<BurstBmFrame>* dev, int total, int radius, const BurstBmConfig& cfg,
-                           hipStream_t s) {
-    static const DirTable dir = make_dir_table();
+gcam_status burst_dispatch(const BurstTable<BurstBmFrame>* dev, const DirTable* dir, int total, int radius,
+                           const BurstBmConfig& cfg, hipStream_t s) {
     switch (radius) {
         case 1: return launch_burst_r<1, H>(dev, total, dir, cfg, s);
         case 2: return launch_burst_r<2, H>(dev, total, dir, cfg, s);
@@ -553,10 +539,24 @@ void burst_bm_tile(int radius, int rows, int tile_w, int* tw, int* th) {
     }
 }
 
-gcam_status launch_burst_block_match_lp(const BurstTable<BurstBmFrame>* dev, int total_tiles, int radius,
-                                        int precision, const BurstBmConfig& cfg, hipStream_t s) {
-    if (precision == GCAM_PRECISION_FP16) return burst_dispatch<_Float16>(dev, total_tiles, radius, cfg, s);
-    if (precision == GCAM_PRECISION_BF16) return burst_dispatch<__bf16>(dev, total_tiles, radius, cfg, s);
+// w: the direction weight divided by sqrt(2 log2 e) (the luma weight enters
+// pre-scaled by that factor); lw: log2 of the direction weight itself.
+BmDirTable make_bm_dir_table() {
+    BmDirTable t{};
+    const float kroot = std::sqrt(2.0f * kLog2e);
+    for (int d2 = 0; d2 < (int)(sizeof(t.w) / sizeof(float)); d2++) {
+        const float a = 0.1f * std::sqrt((float)d2);
+        const float w = std::exp(-2.0f * a * a);
+        t.w[d2] = w / kroot;
+        t.lw[d2] = std::log2(w);
+    }
+    return t;
+}
+
+gcam_status launch_burst_block_match_lp(const BurstTable<BurstBmFrame>* dev, const BmDirTable* dir, int total_tiles,
+                                        int radius, int precision, const BurstBmConfig& cfg, hipStream_t s) {
+    if (precision == GCAM_PRECISION_FP16) return burst_dispatch<_Float16>(dev, dir, total_tiles, radius, cfg, s);
+    if (precision == GCAM_PRECISION_BF16) return burst_dispatch<__bf16>(dev, dir, total_tiles, radius, cfg, s);
     return set_error(GCAM_ERR_INVALID_ARGUMENT, "the burst block matching needs FP16 or BF16");
 }
 

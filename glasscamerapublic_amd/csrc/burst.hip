@@ -27,6 +27,7 @@ struct BurstTables {
     BurstTable<BurstSubFrame> sub[GCAM_MAX_LEVELS];
     BurstTable<BurstMomFrame> mom[GCAM_MAX_LEVELS];
     BurstTable<BurstBmFrame> bm[GCAM_MAX_LEVELS];
+    BmDirTable dir;  // constant; set once at creation
 };
 
 }  // namespace
@@ -251,7 +252,8 @@ gcam_status enqueue(gcam_burst* b, hipStream_t s) {
         if (i < L - 1) st = launch_burst_subtract(&d->sub[i], b->sub_blocks[i], s);
         if (st == GCAM_OK) st = launch_burst_pca(&d->mom[i], b->mom_blocks[i], b->n, s);
         if (st == GCAM_OK)
-            st = launch_burst_block_match_lp(&d->bm[i], b->bm_tiles[i], b->radius, b->precision, level_config(b, i), s);
+            st = launch_burst_block_match_lp(&d->bm[i], &d->dir, b->bm_tiles[i], b->radius, b->precision,
+                                             level_config(b, i), s);
     }
     return st;
 }
@@ -270,10 +272,12 @@ gcam_status time_bm(gcam_burst* b, int i, const BurstBmConfig& cfg, int reps, hi
     hipEvent_t e0, e1;
     GCAM_HIP_TRY(hipEventCreate(&e0));
     GCAM_HIP_TRY(hipEventCreate(&e1));
-    gcam_status st = launch_burst_block_match_lp(&b->dev->bm[i], b->bm_tiles[i], b->radius, b->precision, cfg, s);
+    gcam_status st =
+        launch_burst_block_match_lp(&b->dev->bm[i], &b->dev->dir, b->bm_tiles[i], b->radius, b->precision, cfg, s);
     if (st == GCAM_OK && hipEventRecord(e0, s) != hipSuccess) st = set_error(GCAM_ERR_HIP, "hipEventRecord");
     for (int r = 0; r < reps && st == GCAM_OK; r++)
-        st = launch_burst_block_match_lp(&b->dev->bm[i], b->bm_tiles[i], b->radius, b->precision, cfg, s);
+        st = launch_burst_block_match_lp(&b->dev->bm[i], &b->dev->dir, b->bm_tiles[i], b->radius, b->precision, cfg,
+                                         s);
     if (st == GCAM_OK && hipEventRecord(e1, s) != hipSuccess) st = set_error(GCAM_ERR_HIP, "hipEventRecord");
     if (st == GCAM_OK && hipEventSynchronize(e1) != hipSuccess) st = set_error(GCAM_ERR_HIP, "hipEventSynchronize");
     float t = 0.0f;
@@ -430,6 +434,7 @@ gcam_status gcam_burst_create(int32_t n_frames, const int32_t* widths, const int
     if (hipHostMalloc((void**)&b->host, sizeof(BurstTables), hipHostMallocDefault) != hipSuccess)
         return fail(set_error(GCAM_ERR_OUT_OF_MEMORY, "burst staging tables"));
     std::memset(b->host, 0, sizeof(BurstTables));
+    b->host->dir = make_bm_dir_table();
     if (hipEventCreateWithFlags(&b->staged, hipEventDisableTiming) != hipSuccess)
         return fail(set_error(GCAM_ERR_HIP, "hipEventCreate"));
     if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess)

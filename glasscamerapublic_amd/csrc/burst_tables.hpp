@@ -54,6 +54,16 @@ struct BurstBmFrame {
     int tiles_x, tile0;
 };
 
+// Direction weights of the precision-form block matching, indexed by the
+// squared offset |q - p|^2 (0 .. 2 R^2): w = the weight divided by
+// sqrt(2 log2 e) (the luma weight enters pre-scaled by that factor), lw =
+// log2 of the weight.  The burst reads it from its device table buffer.
+struct BmDirTable {
+    float w[2 * GCAM_MAX_SEARCH_RADIUS * GCAM_MAX_SEARCH_RADIUS + 1];
+    float lw[2 * GCAM_MAX_SEARCH_RADIUS * GCAM_MAX_SEARCH_RADIUS + 1];
+};
+BmDirTable make_bm_dir_table();
+
 template <class F>
 struct BurstTable {
     int nframes;
@@ -104,7 +114,8 @@ gcam_status launch_burst_pca(const BurstTable<BurstMomFrame>* dev, int total_blo
 int moment_block_count(int64_t npatch);
 // Output tile of the persistent kernel for `rows` rows per thread: tw x th pixels.
 void burst_bm_tile(int radius, int rows, int tile_w, int* tw, int* th);
-gcam_status launch_burst_block_match_lp(const BurstTable<BurstBmFrame>* dev, int total_tiles, int radius,
-                                        int precision, const BurstBmConfig& cfg, hipStream_t s);
+// dir: the direction table in device memory (make_bm_dir_table).
+gcam_status launch_burst_block_match_lp(const BurstTable<BurstBmFrame>* dev, const BmDirTable* dir, int total_tiles,
+                                        int radius, int precision, const BurstBmConfig& cfg, hipStream_t s);
 
 }  // namespace gcam
