@@ -476,6 +476,17 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// v plus the value of the same lane in the other 32-lane half of the
+// wavefront (v_permlane32_swap on both words); bit-identical in both halves.
+__device__ __forceinline__ double sum_halves(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+    const double a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    const double c = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+    return a + c;
+}
+
 // Eigenvalues of T below x: sign changes along p_0 = 1, p_1, .., p_25
 // (p_{-1} = 0); d[i] = T_ii, e2[i] = T_{i,i-1}^2 (e2[0] = 0), held in
 // registers; the pair is rescaled by a power of two every five terms.
@@ -529,92 +540,106 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
     for (int e = tid; e < N * N; e += 256) S.W[e / N][e % N] = A0[(e / N) * LD + e % N] * rscale;
     __syncthreads();
 
-    // Householder tridiagonalisation on wave 0, every step unrolled; lane r
-    // holds row r of W (w) and row r of Q (q) in registers.  Step k: lane k
-    // holds x = W[k][k+1 ..] (= column k below the diagonal) and forms
-    // alpha = -sign(x_0) |x|, v_0 = x_0 - alpha and f = sqrt(2 / |v|^2),
-    // broadcast with readlane; lane r > k forms u_r = v_r f from its own
-    // W[r][k].  Two exchanges through LDS: u, then p = W u; with K = u . p,
+    // Householder tridiagonalisation on wave 0, every step unrolled.  Lane l
+    // holds row r = l % 32 (r < N) of W and of Q, columns c = 2 j + h (h = l / 32,
+    // the lane's half, j = 0 .. 12) in registers: the halves split every row's
+    // columns by parity, so the active columns of a step (c > k) fall evenly on
+    // both and a lane does half of its row's work; sums over a row's columns
+    // are completed by one v_permlane32_swap exchange between the halves (the
+    // same value in both).  Step k: the lanes of row k form
+    // alpha = -sign(x_0) |x|, v_0 = x_0 - alpha and f = sqrt(2 / |v|^2)
+    // (x = W[k][k+1 ..] = column k below the diagonal), broadcast with
+    // readlane; the half holding column k forms u_r = v_r f from its W[r][k].
+    // Two exchanges through LDS: u, then p = W u; with K = u . p,
     // W <- W - u p^T - p u^T + K u u^T (row r: w_c -= u_r p_c + (p_r - K u_r)
-    // u_c) and Q <- Q - (Q u) u^T.
+    // u_c) and Q <- Q - (Q u) u^T.  Entries of columns c <= k are stale and
+    // are masked where they would enter a sum (u_c = p_c = 0 there, so the
+    // updates leave them unchanged).
     if (wave == 0) {
-        double w[N], q[N];
+        constexpr int J = (N + 1) / 2;
+        const int h = lane >> 5, r = lane & 31;
+        const bool row_ok = r < N;
+        double w[J], q[J];
 #pragma unroll
-        for (int c = 0; c < N; c++) {
-            w[c] = lane < N ? S.W[lane][c] : 0.0;
-            q[c] = lane == c ? 1.0 : 0.0;
+        for (int j = 0; j < J; j++) {
+            const int c = 2 * j + h;
+            w[j] = row_ok && c < N ? S.W[r][c] : 0.0;
+            q[j] = row_ok && c == r ? 1.0 : 0.0;
         }
 #pragma unroll
         for (int k = 0; k < N - 2; k++) {
-            double na = 0.0, nb = 0.0, nc = 0.0, nd = 0.0;
+            const int j0 = k / 2;  // first register slot with an active column in either half
+            // |x|^2 of row k over c > k (every row forms its own; row k's is used)
+            double na = 0.0, nb = 0.0;
 #pragma unroll
-            for (int c = k + 1; c < N; c += 4) {
-                na = fma(w[c], w[c], na);
-                if (c + 1 < N) nb = fma(w[c + 1], w[c + 1], nb);
-                if (c + 2 < N) nc = fma(w[c + 2], w[c + 2], nc);
-                if (c + 3 < N) nd = fma(w[c + 3], w[c + 3], nd);
+            for (int j = j0; j < J; j++) {
+                const double x = (j > j0 || 2 * j + h > k) ? w[j] : 0.0;
+                if ((j - j0) & 1) nb = fma(x, x, nb);
+                else na = fma(x, x, na);
             }
-            const double n2 = (na + nb) + (nc + nd), x0 = w[k + 1];
+            const double n2 = sum_halves(na + nb);
+            const int kx = k + 1, xl = k + 32 * (kx & 1);  // lane holding x_0 = W[k][k + 1]
+            const double x0 = readlane_f64(w[kx >> 1], xl);
             const double nrm = n2 > 0.0 ? n2 * rsq_full(n2) : 0.0;
             const double al = x0 < 0.0 ? nrm : -nrm;
             const double v0l = x0 - al;
             const double nv = fma(v0l, v0l, fma(-x0, x0, n2));
             const double fl = nv > 0.0 ? 1.4142135623730951 * rsq_full(nv) : 0.0;
             const double alpha = readlane_f64(al, k), v0 = readlane_f64(v0l, k), f = readlane_f64(fl, k);
-            const bool below = lane > k && lane < N;
-            const double u = below ? (lane == k + 1 ? v0 : w[k]) * f : 0.0;
-            S.bu[lane] = u;
-            if (lane == k) S.d[k] = w[k];
+            // column k lives in half k & 1, slot k / 2
+            const bool below = row_ok && r > k;
+            if (h == (k & 1) && row_ok) {
+                S.bu[r] = below ? (r == k + 1 ? v0 : w[k >> 1]) * f : 0.0;
+                if (r == k) S.d[k] = w[k >> 1];
+            }
             if (lane == 0) S.e[k] = alpha;
             wave_sync();
-            double uc[N];
-            double pa = 0.0, pb = 0.0, pc = 0.0, pd = 0.0, qa = 0.0, qb = 0.0;
+            double uc[J];
+            double pa = 0.0, pb = 0.0, qa = 0.0, qb = 0.0;
 #pragma unroll
-            for (int c = k + 1; c < N; c++) {
-                uc[c] = S.bu[c];
-                switch ((c - k - 1) & 3) {
-                    case 0: pa = fma(w[c], uc[c], pa); break;
-                    case 1: pb = fma(w[c], uc[c], pb); break;
-                    case 2: pc = fma(w[c], uc[c], pc); break;
-                    default: pd = fma(w[c], uc[c], pd); break;
+            for (int j = j0; j < J; j++) {
+                uc[j] = 2 * j + h < N ? S.bu[2 * j + h] : 0.0;
+                if ((j - j0) & 1) {
+                    pb = fma(w[j], uc[j], pb);
+                    qb = fma(q[j], uc[j], qb);
+                } else {
+                    pa = fma(w[j], uc[j], pa);
+                    qa = fma(q[j], uc[j], qa);
                 }
-                if ((c - k - 1) & 1) qb = fma(q[c], uc[c], qb);
-                else qa = fma(q[c], uc[c], qa);
             }
-            const double p = below ? (pa + pb) + (pc + pd) : 0.0;
-            const double qu = qa + qb;
-            S.bp[lane] = p;
+            const double u = row_ok ? S.bu[r] : 0.0;
+            const double psum = sum_halves(pa + pb);
+            const double qu = sum_halves(qa + qb);
+            const double p = below ? psum : 0.0;
+            if (h == 0 && row_ok) S.bp[r] = p;
             wave_sync();
-            double pv[N];
-            double ka = 0.0, kb = 0.0, kc = 0.0, kd = 0.0;
+            double pv[J];
+            double ka = 0.0, kb = 0.0;
 #pragma unroll
-            for (int c = k + 1; c < N; c++) {
-                pv[c] = S.bp[c];
-                switch ((c - k - 1) & 3) {
-                    case 0: ka = fma(uc[c], pv[c], ka); break;
-                    case 1: kb = fma(uc[c], pv[c], kb); break;
-                    case 2: kc = fma(uc[c], pv[c], kc); break;
-                    default: kd = fma(uc[c], pv[c], kd); break;
-                }
+            for (int j = j0; j < J; j++) {
+                pv[j] = 2 * j + h < N ? S.bp[2 * j + h] : 0.0;
+                if ((j - j0) & 1) kb = fma(uc[j], pv[j], kb);
+                else ka = fma(uc[j], pv[j], ka);
             }
-            const double K = (ka + kb) + (kc + kd);
+            const double K = sum_halves(ka + kb);
             const double pp = fma(-K, u, p);
 #pragma unroll
-            for (int c = k + 1; c < N; c++) {
-                w[c] = fma(-u, pv[c], fma(-pp, uc[c], w[c]));
-                q[c] = fma(-qu, uc[c], q[c]);
+            for (int j = j0; j < J; j++) {
+                w[j] = fma(-u, pv[j], fma(-pp, uc[j], w[j]));
+                q[j] = fma(-qu, uc[j], q[j]);
             }
             wave_sync();
         }
-        if (lane == N - 2) S.d[N - 2] = w[N - 2];
-        if (lane == N - 1) {
-            S.d[N - 1] = w[N - 1];
-            S.e[N - 2] = w[N - 2];
-        }
+        // d[N-2] = W[N-2][N-2] (half 1, slot 11), d[N-1] = W[N-1][N-1] (half 0,
+        // slot 12), e[N-2] = W[N-1][N-2] (half 1, slot 11)
+        if (lane == (N - 2) + 32) S.d[N - 2] = w[(N - 2) >> 1];
+        if (lane == N - 1) S.d[N - 1] = w[(N - 1) >> 1];
+        if (lane == (N - 1) + 32) S.e[N - 2] = w[(N - 2) >> 1];
         // Q replaces W in LDS (W is no longer needed).
-        if (lane < N) {
+        if (row_ok) {
 #pragma unroll
-            for (int c = 0; c < N; c++) S.W[lane][c] = q[c];
+            for (int j = 0; j < J; j++)
+                if (2 * j + h < N) S.W[r][2 * j + h] = q[j];
         }
         wave_sync();
         if (lane == 0) {
