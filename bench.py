@@ -57,8 +57,8 @@ MEASURED_BM_TRAFFIC = {
 # Wave-level VALU instructions per block-matching dispatch on the same workload, levels 0..4:
 # rocprofv3 --pmc SQ_INSTS_VALU, in millions.
 MEASURED_BM_VALU = {
-    ("12mp", "fp16"): [395.661, 99.062, 25.232, 6.754, 1.704],
-    ("12mp", "fp32"): [534.553, 133.651, 33.950, 8.639, 2.171],
+    ("12mp", "fp16"): [391.316, 97.987, 24.964, 6.781, 1.712],
+    ("12mp", "fp32"): [531.124, 132.795, 33.733, 8.654, 2.175],
 }
 # Matrix-core work per dispatch on the same workload, levels 0..4 (rocprofv3 --pmc
 # SQ_INSTS_VALU_MFMA_MOPS_F16 / _F64, which count flops / 512): the FP16 projection inside the precision
