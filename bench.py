@@ -63,10 +63,11 @@ MEASURED_BM_VALU = {
 # Matrix-core work per dispatch on the same workload, levels 0..4 (rocprofv3 --pmc
 # SQ_INSTS_VALU_MFMA_MOPS_F16 / _F64, which count flops / 512): the FP16 projection inside the precision
 # form's block matching (v_mfma_f32_32x32x16_f16) and the FP64 patch moments (v_mfma_f64_16x16x4f64,
-# levels 0-3 in k_sample_moments, level 4 in k_image_moments).
+# levels 0-3 in the fused k_subtract_moments, level 4 in k_image_moments; levels 0-2 share a grid, so
+# the counter pass gives their mean, 250172, and the per-level values are its tiles' groups x 3 MFMAs x 4).
 MEASURED_MFMA_MOPS = {
     ("12mp", "fp16"): {"block_match_f16": [91445760, 22982400, 5898240, 1966080, 491520],
-                       "moments_f64": [786432, 286080, 71424, 18048, 4608]},
+                       "moments_f64": [571536, 142884, 36096, 9024, 4608]},
 }
 MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0, "f64": 78.6}
 DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}
@@ -81,15 +82,18 @@ BOX_CPU_SHARE = 16
 
 def mfma_block(workload, precision, stages_mean5, levels):
     """Matrix-core utilisation: measured MFMA flops per dispatch over the live per-level stage durations
-    (block matching for the FP16 projection, the moment kernel for the FP64 moments), against the dense
-    MFMA peaks."""
+    (block matching for the FP16 projection; for the FP64 moments the subtract, which forms them below
+    the top level, and the moment kernel of the top level), against the dense MFMA peaks."""
     m = MEASURED_MFMA_MOPS.get((workload, precision))
     if m is None:
         return None
     out = {}
     for name, key, dt in (("block_match_f16_projection", "block_match", "f16"), ("moments_f64", "moments", "f64")):
         mops = m["block_match_f16" if dt == "f16" else "moments_f64"]
-        ms = stages_mean5.get(key, [0.0] * levels)
+        ms = list(stages_mean5.get(key, [0.0] * levels))
+        if dt == "f64":
+            sub = stages_mean5.get("subtract", [0.0] * levels)
+            ms = [sub[i] if i < levels - 1 else ms[i] for i in range(levels)]
         flops = [512.0 * x for x in mops[:levels]]
         rates = [f / (t / 1e3) / 1e12 if t > 0 else None for f, t in zip(flops, ms)]
         tot_t = sum(ms[:levels])

@@ -135,9 +135,6 @@ gcam_status gcam_pyramid_create(int32_t width, int32_t height, int32_t levels, c
     // form projects inside the block-matching tiles).
     if (cfg.precision == GCAM_PRECISION_FP32 && !cfg.bilateral)
         parts.push_back({(void**)&p->feat, sizeof(float) * 8 * (size_t)width * height});
-    // The subtract writes the PCA sample of its level; level 0 is the largest.
-    if (levels > 1 && !cfg.bilateral)
-        parts.push_back({(void**)&p->patches, sizeof(float) * kPatch * (size_t)(width / 8) * (height / 8)});
     parts.push_back({(void**)&p->moment_partials, sizeof(double) * kMomentSlots * kMoments});
     parts.push_back({(void**)&p->nlf_partials, sizeof(double) * kNlfBlocks * 13});
     parts.push_back({(void**)&p->nlf_state, sizeof(NlfFitState)});
@@ -265,13 +262,20 @@ static gcam_status denoise_impl(gcam_pyramid* p, const gcam_denoise_params* para
     // Loop B (pyramid_processor.cpp:123-169): coarse to fine.
     for (int i = L - 1; i >= 0; i--) {
         View<const float4> layer = noisy(i);
+        int sub_blocks = 0;
         if (i < L - 1) {
             GCAM_MARK(i, GCAM_STAGE_SUBTRACT, 0);
             const View<float4> sv = view<float4>(p->sub[i], p->W[i], p->H[i]);
-            st = launch_subtract_noise(noisy(i), noisy(i + 1),
-                                       view<const float4>(p->den[i + 1], p->W[i + 1], p->H[i + 1]), gradv(i),
-                                       /*luma_weight=*/1.0f, params[i].sharpening, 0.0f, 0.0f, p->nlf_dev + i,
-                                       params[i].luma, sv, p->patches, s);
+            const View<const float4> den1 = view<const float4>(p->den[i + 1], p->W[i + 1], p->H[i + 1]);
+            // With patch similarity the subtract also forms the level's PCA
+            // moment partials (the sample is never written).
+            if (p->bilateral)
+                st = launch_subtract_noise(noisy(i), noisy(i + 1), den1, gradv(i), /*luma_weight=*/1.0f,
+                                           params[i].sharpening, 0.0f, 0.0f, p->nlf_dev + i, params[i].luma, sv, s);
+            else
+                st = launch_subtract_moments(noisy(i), noisy(i + 1), den1, gradv(i), /*luma_weight=*/1.0f,
+                                             params[i].sharpening, 0.0f, 0.0f, p->nlf_dev + i, params[i].luma, sv,
+                                             p->moment_partials, &sub_blocks, s);
             if (st != GCAM_OK) return st;
             layer = View<const float4>{sv.ptr, sv.w, sv.h, sv.stride};
             GCAM_MARK(i, GCAM_STAGE_SUBTRACT, 1);
@@ -293,13 +297,10 @@ static gcam_status denoise_impl(gcam_pyramid* p, const gcam_denoise_params* para
         }
         GCAM_MARK(i, GCAM_STAGE_PCA, 0);
         const int64_t n = ((int64_t)p->W[i] * p->H[i]) / 64;  // :142
-        int nblocks = 0;
-        // Levels below the top were sampled by their subtract.
+        // Levels below the top: their subtract wrote the moment partials.
+        int nblocks = sub_blocks;
         GCAM_MARK(i, GCAM_STAGE_MOMENTS, 0);
-        if (i < L - 1)
-            st = launch_sample_moments(p->patches, p->W[i], p->H[i], p->moment_partials, &nblocks, s);
-        else
-            st = launch_image_moments(layer, p->moment_partials, &nblocks, s);
+        if (i == L - 1) st = launch_image_moments(layer, p->moment_partials, &nblocks, s);
         if (st != GCAM_OK) return st;
         GCAM_MARK(i, GCAM_STAGE_MOMENTS, 1);
         st = launch_pca_solve(p->moment_partials, nblocks, n, p->basis + 200 * i, p->eigvals + 25 * i,
@@ -543,7 +544,7 @@ gcam_status gcam_subtract_noise_image(const gcam_image* input, const gcam_image*
                                  view<const float4>(*input_denoised1),
                                  gradient ? view<const float2>(*gradient) : View<const float2>{nullptr, 0, 0, 0},
                                  luma_weight, sharpening, nlf_x, nlf_y, nullptr, 0.0f, view<float4>(*output),
-                                 nullptr, as_stream(stream));
+                                 as_stream(stream));
 }
 
 gcam_status gcam_collect_patches(const gcam_image* input, float* patches, int64_t capacity, int64_t* count,

@@ -14,6 +14,7 @@
 #include "burst_tables.hpp"
 #include "gcam_internal.hpp"
 #include "handle.hpp"
+#include "moments.hpp"
 
 using namespace gcam;
 
@@ -211,21 +212,18 @@ void fill_tables(gcam_burst* b, BurstTables* t, const gcam_denoise_params* param
                 sb.sharpening = params[f * L + i].sharpening;
                 sb.thr_y = params[f * L + i].luma;
                 sb.out = View<float4>{p->sub[i], w, h, w};
-                sb.patches = p->patches;
-                sb.bx = (w + 63) / 64;
+                sb.partials = p->moment_partials;
+                sb.nblk = submom_blocks(w, h);
+                sub = (sub + 7) & ~7;  // the frame's workgroup b runs on XCD b % 8, as in the single-frame pass
                 sb.block0 = sub;
-                sub += blocks64x4(w, h);
+                sub += sb.nblk;
             }
+            // Levels below the top: the fused pass wrote the partials; the top
+            // level is sampled by the moment kernel.
             BurstMomFrame& m = t->mom[i].frame[f];
-            if (i < L - 1) {
-                m.in = View<const float4>{p->sub[i], w, h, w};
-                m.patches = p->patches;
-            } else {
-                m.in = noisy;
-                m.patches = nullptr;
-            }
+            m.in = noisy;
             m.partials = p->moment_partials;
-            m.nblocks = moment_block_count((int64_t)(w / 8) * (h / 8));
+            m.nblocks = i < L - 1 ? submom_blocks(w, h) : moment_block_count((int64_t)(w / 8) * (h / 8));
             m.block0 = mom;
             m.n = (double)(((int64_t)w * h) / 64);  // pyramid_processor.cpp:142
             m.basis = p->basis + 200 * i;
@@ -249,8 +247,12 @@ gcam_status enqueue(gcam_burst* b, hipStream_t s) {
     gcam_status st = GCAM_OK;
     for (int i = 0; i < L - 1 && st == GCAM_OK; i++) st = launch_burst_downsample(&d->down[i], b->down_blocks[i], s);
     for (int i = L - 1; i >= 0 && st == GCAM_OK; i--) {
-        if (i < L - 1) st = launch_burst_subtract(&d->sub[i], b->sub_blocks[i], s);
-        if (st == GCAM_OK) st = launch_burst_pca(&d->mom[i], b->mom_blocks[i], b->n, s);
+        if (i < L - 1) {
+            st = launch_burst_subtract_moments(&d->sub[i], b->sub_blocks[i], s);
+            if (st == GCAM_OK) st = launch_burst_pca_solve(&d->mom[i], b->n, s);
+        } else {
+            st = launch_burst_pca(&d->mom[i], b->mom_blocks[i], b->n, s);
+        }
         if (st == GCAM_OK)
             st = launch_burst_block_match_lp(&d->bm[i], &d->dir, b->bm_tiles[i], b->radius, b->precision,
                                              level_config(b, i), s);

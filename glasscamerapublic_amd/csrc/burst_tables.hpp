@@ -30,13 +30,12 @@ struct BurstSubFrame {
     const NlfDev* nlf;  // this frame's noise model of the level
     float sharpening, thr_y;
     View<float4> out;
-    float* patches;  // PCA sample of out, (w / 8) x (h / 8) x 25 (nullptr: none)
-    int bx, block0;
+    double* partials;  // this frame's moment partials (nblk x kMoments)
+    int nblk, block0;  // workgroups of the fused subtract-and-moments pass; block0 a multiple of 8
 };
 
 struct BurstMomFrame {
-    View<const float4> in;
-    const float* patches;  // the sample of `in` written by the subtract (nullptr: sample `in`)
+    View<const float4> in;  // the level the moment kernel samples (top level)
     double* partials;  // kMomentSlots x kMoments of this frame
     int nblocks, block0;
     double n;          // patch slots, w * h / 64
@@ -107,9 +106,12 @@ struct BurstBmConfig {
 };
 
 gcam_status launch_burst_downsample(const BurstTable<BurstDownFrame>* dev, int total_blocks, hipStream_t s);
-gcam_status launch_burst_subtract(const BurstTable<BurstSubFrame>* dev, int total_blocks, hipStream_t s);
+// The fused subtract-and-moments pass of every frame of a level.
+gcam_status launch_burst_subtract_moments(const BurstTable<BurstSubFrame>* dev, int total_blocks, hipStream_t s);
 // Moments, their reduction and the eigensolve of every frame of a level.
 gcam_status launch_burst_pca(const BurstTable<BurstMomFrame>* dev, int total_blocks, int nframes, hipStream_t s);
+// The reduction and eigensolve only, of partials written by the fused subtract-and-moments pass.
+gcam_status launch_burst_pca_solve(const BurstTable<BurstMomFrame>* dev, int nframes, hipStream_t s);
 // Workgroups of the moment reduction for npatch patches (1 .. kMomentBlocks).
 int moment_block_count(int64_t npatch);
 // Output tile of the persistent kernel for `rows` rows per thread: tw x th pixels.

@@ -105,14 +105,20 @@ gcam_status set_error(gcam_status st, const std::string& msg);
 gcam_status launch_downsample4(View<const float4> in, View<float4> out, hipStream_t s);
 gcam_status launch_downsample2(View<const float2> in, View<float2> out, hipStream_t s);
 // nlf_level: device noise model of this level (or nullptr to use nlf_x/nlf_y as given);
-// when set, nlf = (a.x * thr_y, b.x * thr_y).  patches (nullable): also write
-// the PCA sample of the output, (w / 8) * (h / 8) patches x 25 luma values
-// (the 5 x 5 around pixel (8 px, 8 py), clamp-to-edge) in row planes: value
-// (i, j) of patch (px, py) at ((py * 5 + j) * (w / 8) + px) * 5 + i, i, j in 0..4.
+// when set, nlf = (a.x * thr_y, b.x * thr_y).
 gcam_status launch_subtract_noise(View<const float4> in, View<const float4> in1, View<const float4> den1,
                                   View<const float2> grad, float luma_weight, float sharpening, float nlf_x,
-                                  float nlf_y, const NlfDev* nlf_level, float thr_y, View<float4> out, float* patches,
+                                  float nlf_y, const NlfDev* nlf_level, float thr_y, View<float4> out,
                                   hipStream_t s);
+// The subtract of launch_subtract_noise and the PCA moments of its output in
+// one pass: writes out and *nblocks per-workgroup moment partials of the
+// (w / 8) x (h / 8) patches (the 5 x 5 luma around pixel (8 px, 8 py),
+// clamp-to-edge), the sample collectPatches would take; the sample itself is
+// never materialized.
+gcam_status launch_subtract_moments(View<const float4> in, View<const float4> in1, View<const float4> den1,
+                                    View<const float2> grad, float luma_weight, float sharpening, float nlf_x,
+                                    float nlf_y, const NlfDev* nlf_level, float thr_y, View<float4> out,
+                                    double* partials, int* nblocks, hipStream_t s);
 gcam_status launch_noise_stats(View<const float4> in, View<float4> out, hipStream_t s);
 gcam_status launch_nlf_fit(View<const float4> stats, float exposure_multiplier, double* partials,
                            NlfFitState* state, NlfDev* nlf_out, hipStream_t s);
@@ -127,8 +133,6 @@ constexpr double kJacobiTol = 1e-22;
 // Moments of the patches sampled from an image (never materialized); *nblocks
 // receives the number of per-workgroup partials written.
 gcam_status launch_image_moments(View<const float4> in, double* partials, int* nblocks, hipStream_t s);
-// Moments of the PCA sample of a w x h level written by launch_subtract_noise.
-gcam_status launch_sample_moments(const float* sample, int w, int h, double* partials, int* nblocks, hipStream_t s);
 // Moments of a device patch buffer (n x 25).
 gcam_status launch_buffer_moments(const float* patches, int64_t n, double* partials, int* nblocks, hipStream_t s);
 // Reduce the partials, form the covariance, eigensolve, write basis (25 x 8 f32) and eigenvalues.

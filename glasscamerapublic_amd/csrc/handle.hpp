@@ -16,7 +16,6 @@ struct gcam_pyramid {
     float4* sub[GCAM_MAX_LEVELS] = {};    // subtractedImagePyramid
     float4* den[GCAM_MAX_LEVELS] = {};    // denoisedImagePyramid
     float* feat = nullptr;                // PCA features of one level (reused per level)
-    float* patches = nullptr;             // PCA sample of one subtracted level (reused per level)
     double* moment_partials = nullptr;
     double* nlf_partials = nullptr;
     gcam::NlfFitState* nlf_state = nullptr;

@@ -16,6 +16,7 @@
 // whole denoise call stays stream-ordered.
 #include "gcam_internal.hpp"
 #include "burst_tables.hpp"
+#include "moments.hpp"
 
 namespace gcam {
 
@@ -28,28 +29,6 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // leaves the sign free; fixing it makes the features reproducible).
 __device__ __forceinline__ double sign_ref(int r) { return 1.0 + (double)((r * r * 7 + r * 3 + 5) % 23) / 16.0; }
 
-// Row/column of moment entry e: entries 0..24 are sum x_r (r = e, s = -1),
-// entries 25.. enumerate the upper triangle r <= s row by row.
-// Row r of triangle entry t = e - 25 is the largest r with start(r) <= t,
-// start(r) = 25 r - r (r - 1) / 2 (the entries of rows 0 .. r - 1): the root
-// of the quadratic, corrected by one step either way.
-__device__ __forceinline__ void moment_rs(int e, int& r, int& s) {
-    if (e < kPatch) {
-        r = e;
-        s = -1;
-        return;
-    }
-    const int t = e - kPatch;
-    auto start = [](int q) { return kPatch * q - q * (q - 1) / 2; };
-    int q = (int)((51.0f - sqrtf(2601.0f - 8.0f * (float)t)) * 0.5f);
-    q = q < 0 ? 0 : (q > kPatch - 1 ? kPatch - 1 : q);
-    if (q < kPatch - 1 && start(q + 1) <= t) q++;
-    if (start(q) > t) q--;
-    r = q;
-    s = q + (t - start(q));
-}
-
-typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // Moments of workgroup `blk` of `nblk` as a Gram product on the FP64 matrix
 // cores: with X the npatch x 32 patch matrix (25 luma values, 7 zero
@@ -82,37 +61,9 @@ __device__ void moments_body(int blk, int nblk, int64_t npatch, double* partial,
             hi[u] = fetch(k, f + 16);
         }
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const double dl = lo[u], dh = hi[u];
-            slo += dl;
-            shi += dh;
-            c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(dl, dl, c00, 0, 0, 0);
-            c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(dl, dh, c01, 0, 0, 0);
-            c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(dh, dh, c11, 0, 0, 0);
-        }
+        for (int u = 0; u < U; u++) moments_group(lo[u], hi[u], c00, c01, c11, slo, shi);
     }
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int i = kq + 4 * r;
-        gram[wave][i][f] = c00[r];
-        gram[wave][i][16 + f] = c01[r];
-        gram[wave][16 + i][16 + f] = c11[r];
-    }
-    sums[wave][lane][0] = slo;
-    sums[wave][lane][1] = shi;
-    __syncthreads();
-    for (int e = tid; e < kMoments; e += 256) {
-        int r, s;
-        moment_rs(e, r, s);
-        double acc = 0.0;
-        if (s < 0) {
-            for (int w = 0; w < 4; w++)
-                for (int q = 0; q < 4; q++) acc += sums[w][16 * q + (r & 15)][r >> 4];
-        } else {
-            for (int w = 0; w < 4; w++) acc += gram[w][r][s];
-        }
-        partial[e] = acc;
-    }
+    moments_store<4>(tid, 256, c00, c01, c11, slo, shi, gram, sums, partial);
 }
 
 // Patches sampled from the image at 8*(x, y), x < w/8, y < h/8, luma only,
@@ -132,24 +83,6 @@ __global__ __launch_bounds__(256) void k_image_moments(View<const float4> in, do
     const int64_t npatch = (int64_t)gw * gh;
     moments_body(blockIdx.x, gridDim.x, npatch, partials + (size_t)blockIdx.x * kMoments,
                  [&](int64_t k, int f) { return image_patch_value(in, gw, npatch, k, f); });
-}
-
-// Value f of patch k (px = k % gw, py = k / gw) of a sample written by the
-// subtract kernel (row planes: value (i, j) at ((py * 5 + j) * gw + px) * 5 + i,
-// i, j in 0..4), the same value image_patch_value reads from the level.
-__device__ __forceinline__ float plane_patch_value(const float* __restrict__ s, int gw, int64_t npatch, int64_t k,
-                                                   int f) {
-    if (k >= npatch || f >= kPatch) return 0.0f;
-    const int kk = (int)k;  // npatch <= 16384^2 / 64
-    const int py = kk / gw, px = kk - py * gw;
-    const int j = f / 5, i = f - 5 * j;
-    return s[((size_t)(py * 5 + j) * gw + px) * 5 + i];
-}
-
-__global__ __launch_bounds__(256) void k_sample_moments(const float* sample, int gw, int gh, double* partials) {
-    const int64_t npatch = (int64_t)gw * gh;
-    moments_body(blockIdx.x, gridDim.x, npatch, partials + (size_t)blockIdx.x * kMoments,
-                 [&](int64_t k, int f) { return plane_patch_value(sample, gw, npatch, k, f); });
 }
 
 __global__ __launch_bounds__(256) void k_buffer_moments(const float* patches, int64_t n, double* partials) {
@@ -881,11 +814,8 @@ __global__ __launch_bounds__(256) void k_burst_moments(const BurstTable<BurstMom
     const View<const float4> in = f.in;
     const int gw = in.w / 8, gh = in.h / 8;
     const int64_t npatch = (int64_t)gw * gh;
-    const float* patches = f.patches;
-    moments_body(lb, f.nblocks, npatch, f.partials + (size_t)lb * kMoments, [&](int64_t k, int c) {
-        if (patches) return plane_patch_value(patches, gw, npatch, k, c);
-        return image_patch_value(in, gw, npatch, k, c);
-    });
+    moments_body(lb, f.nblocks, npatch, f.partials + (size_t)lb * kMoments,
+                 [&](int64_t k, int c) { return image_patch_value(in, gw, npatch, k, c); });
 }
 
 __global__ __launch_bounds__(256) void k_burst_moments_reduce(const BurstTable<BurstMomFrame>* __restrict__ tab) {
@@ -957,12 +887,6 @@ gcam_status launch_image_moments(View<const float4> in, double* partials, int* n
     return GCAM_OK;
 }
 
-gcam_status launch_sample_moments(const float* sample, int w, int h, double* partials, int* nblocks, hipStream_t s) {
-    *nblocks = moment_blocks((int64_t)(w / 8) * (h / 8));
-    k_sample_moments<<<*nblocks, 256, 0, s>>>(sample, w / 8, h / 8, partials);
-    GCAM_LAUNCH_CHECK("patch moments");
-    return GCAM_OK;
-}
 
 gcam_status launch_buffer_moments(const float* patches, int64_t n, double* partials, int* nblocks, hipStream_t s) {
     *nblocks = moment_blocks(n);
@@ -977,6 +901,14 @@ gcam_status launch_pca_solve(double* partials, int nblocks, int64_t n, float* ba
     k_moments_reduce<<<kMoments / kRedEntries, 256, 0, s>>>(partials, nblocks, mom);
     k_pca_solve<<<1, 256, 0, s>>>(mom, (double)n, basis, eigvals, eig_full);
     GCAM_LAUNCH_CHECK("PCA eigensolve");
+    return GCAM_OK;
+}
+
+gcam_status launch_burst_pca_solve(const BurstTable<BurstMomFrame>* dev, int nframes, hipStream_t s) {
+    if (nframes < 1) return GCAM_OK;
+    k_burst_moments_reduce<<<dim3(kMoments / kRedEntries, nframes), 256, 0, s>>>(dev);
+    k_burst_pca_solve<<<nframes, 256, 0, s>>>(dev);
+    GCAM_LAUNCH_CHECK("burst PCA basis");
     return GCAM_OK;
 }
 

@@ -10,6 +10,7 @@
 // or 8-byte (float2) access; the neighbouring taps hit L1/L2.
 #include "gcam_internal.hpp"
 #include "burst_tables.hpp"
+#include "moments.hpp"
 
 namespace gcam {
 
@@ -113,11 +114,11 @@ __global__ __launch_bounds__(256) void k_downsample2(View<const float2> in, View
 // subtractNoiseImage: Laplacian-style reconstruct of level i from the noisy
 // and the denoised level i+1 (both upsampled bilinearly), with optional
 // gradient-driven sharpening.
-__device__ __forceinline__ float subtract_noise_px(const View<const float4>& in, const View<const float4>& in1,
-                                                  const View<const float4>& den1, const View<const float2>& grad,
-                                                  float luma_weight, float sharpening, float nlf_x, float nlf_y,
-                                                  const NlfDev* nlf_level, float thr_y, const View<float4>& out,
-                                                  int ox, int oy) {
+__device__ __forceinline__ float4 subtract_noise_value(const View<const float4>& in, const View<const float4>& in1,
+                                                      const View<const float4>& den1, const View<const float2>& grad,
+                                                      float luma_weight, float sharpening, float nlf_x, float nlf_y,
+                                                      const NlfDev* nlf_level, float thr_y, const View<float4>& out,
+                                                      int ox, int oy) {
     const Tap tyy = lin_tap((2 * oy + 1) * in1.h, 2 * out.h, in1.h);
     const Tap txx = lin_tap((2 * ox + 1) * in1.w, 2 * out.w, in1.w);
     const float4 ip = in.at(ox, oy);
@@ -145,56 +146,101 @@ __device__ __forceinline__ float subtract_noise_px(const View<const float4>& in,
     o.y = d1.y + (dp.y - d1.y) * alpha;
     o.z = d1.z + (dp.z - d1.z) * alpha;
     o.w = ip.w;
+    return o;
+}
+
+__device__ __forceinline__ float subtract_noise_px(const View<const float4>& in, const View<const float4>& in1,
+                                                  const View<const float4>& den1, const View<const float2>& grad,
+                                                  float luma_weight, float sharpening, float nlf_x, float nlf_y,
+                                                  const NlfDev* nlf_level, float thr_y, const View<float4>& out,
+                                                  int ox, int oy) {
+    const float4 o = subtract_noise_value(in, in1, den1, grad, luma_weight, sharpening, nlf_x, nlf_y, nlf_level,
+                                          thr_y, out, ox, oy);
     out.at(ox, oy) = o;
     return o.x;
 }
 
-// The PCA sample of the level written as it is produced: patch (px, py)
-// (px < gw = w / 8, py < gh = h / 8) is the 5 x 5 luma around (8 px, 8 py),
-// offset (i, j) in -2..2 at pixel (8 px + i, 8 py + j) clamped to the image --
-// the collectPatches sample.  Stored in row planes: value (i, j) of patch
-// (px, py) at ((py * 5 + j + 2) * gw + px) * 5 + i + 2, so the 40 values a
-// wavefront's 64-pixel row holds are one contiguous 160-byte run.  Along x, a pixel with ox mod 8 = r in {0, 1, 2} is offset
-// i = r of patch column ox / 8 and one with r in {6, 7} offset r - 8 of the
-// next column; column 0 is also the clamped offsets -2 and -1 of patch
-// column 0; rows likewise.  Offsets +1, +2 never clamp (8 (gw - 1) + 2 <= w - 6).
-__device__ __forceinline__ int patch_axis_slots(int o, int g, int* cell, int* off) {
-    int n = 0;
-    const int r = o & 7;
-    if (r <= 2 && (o >> 3) < g) cell[n] = o >> 3, off[n++] = r;
-    if (r >= 6 && (o >> 3) + 1 < g) cell[n] = (o >> 3) + 1, off[n++] = r - 8;
-    if (o == 0 && g > 0) {
-        cell[n] = 0, off[n++] = -2;
-        cell[n] = 0, off[n++] = -1;
+// Subtract and PCA moments in one pass over a level (the sample is never
+// written): 64 x 8-pixel tiles anchored at (64 tx - 2, 8 ty - 2) hold whole
+// patches -- patch (8 tx + p, ty), p = 0..7, is tile rows 0..4, columns
+// 8 p .. 8 p + 4 -- and a fixed grid of nblk workgroups (one moment partial
+// each) walks the tiles.  Workgroup b belongs to the tile range of XCD b % 8
+// (a contiguous eighth of the tiles, so neighbouring tiles, which share rows
+// of the coarse levels, meet in that XCD's L2) and takes every nw-th tile of
+// it, nw the number of workgroups of that XCD.  A thread computes two pixels
+// of a tile (rows y and y + 4); pixels outside the image are computed at the
+// clamped position (which is what the clamp-to-edge patch sampling reads) and
+// not written.  The luma of tile rows 0..4 goes to LDS, in two buffers: the
+// barrier that publishes a tile's rows also retires the reads of the tile
+// before.  Wavefronts 0 and 1 then add patches 0-3 and 4-7 of the tile to
+// their FP64 Gram accumulators.  The tiles of a workgroup are taken in
+// increasing order and the wavefronts' sums added in a fixed order, so the
+// partial of a workgroup is deterministic, and the burst form, which gives a
+// frame the same workgroups and walk, produces the same partials.
+struct SubMomArgs {
+    View<const float4> in, in1, den1;
+    View<const float2> grad;
+    float luma_weight, sharpening, nlf_x, nlf_y;
+    const NlfDev* nlf_level;
+    float thr_y;
+    View<float4> out;
+};
+
+__device__ __forceinline__ void subtract_moments_body(const SubMomArgs& a, int b, int nblk,
+                                                      double* __restrict__ partial) {
+    __shared__ float lum[2][5][64];
+    __shared__ double gram[2][32][33];
+    __shared__ double sums[2][64][2];
+    const int tid = threadIdx.y * 64 + threadIdx.x, lane = tid & 63, wave = threadIdx.y;
+    const int W = a.out.w, H = a.out.h, gw = W / 8, gh = H / 8;
+    const int tiles_x = submom_tiles_x(W), tiles = submom_tiles(W, H);
+    const int v = b & 7, chunk = (tiles + 7) >> 3, nw = (nblk - v + 7) >> 3;
+    const int end = min(tiles, (v + 1) * chunk);
+    // this lane's patch values: feature f (row f / 5, column f % 5) and f + 16
+    const int f = lane & 15, kq = lane >> 4;
+    const int r0 = f / 5, c0 = f - 5 * r0, r1 = (f + 16) / 5, c1 = (f + 16) - 5 * r1;
+    f64x4 c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+    double slo = 0.0, shi = 0.0;
+    int buf = 0;
+    for (int t = v * chunk + (b >> 3); t < end; t += nw) {
+        const int tx = t % tiles_x, ty = t / tiles_x;
+        const int cx = 64 * tx - 2 + (int)threadIdx.x, sx = clampi(cx, 0, W - 1);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int r = (int)threadIdx.y + 4 * h;
+            const int cy = 8 * ty - 2 + r, sy = clampi(cy, 0, H - 1);
+            const float4 o = subtract_noise_value(a.in, a.in1, a.den1, a.grad, a.luma_weight, a.sharpening, a.nlf_x,
+                                                  a.nlf_y, a.nlf_level, a.thr_y, a.out, sx, sy);
+            if (cx == sx && cy == sy) a.out.at(sx, sy) = o;
+            if (r < 5) lum[buf][r][threadIdx.x] = o.x;
+        }
+        __syncthreads();
+        if (wave < 2 && ty < gh && 8 * tx + 4 * wave < gw) {  // the group holds a patch of the sample
+            const int p = 4 * wave + kq;  // patch of this lane's row of the group
+            const bool ok = 8 * tx + p < gw;
+            const double lo = ok ? (double)lum[buf][r0][8 * p + c0] : 0.0;
+            const double hi = ok && f < kPatch - 16 ? (double)lum[buf][r1][8 * p + c1] : 0.0;
+            moments_group(lo, hi, c00, c01, c11, slo, shi);
+        }
+        buf ^= 1;
     }
-    return n;
+    moments_store<2>(tid, 256, c00, c01, c11, slo, shi, gram, sums, partial);
 }
 
-__device__ __forceinline__ void write_patch_slots(float* __restrict__ patches, int gw, int gh, int ox, int oy,
-                                                  float v) {
-    int cx[3], ix[3], cy[3], jy[3];
-    const int ny = patch_axis_slots(oy, gh, cy, jy);
-    if (ny == 0) return;
-    const int nx = patch_axis_slots(ox, gw, cx, ix);
-    for (int b = 0; b < ny; b++)
-        for (int a = 0; a < nx; a++)
-            patches[((size_t)(cy[b] * 5 + jy[b] + 2) * gw + cx[a]) * 5 + ix[a] + 2] = v;
+__global__ __launch_bounds__(256) void k_subtract_moments(SubMomArgs a, double* partials) {
+    subtract_moments_body(a, blockIdx.x, gridDim.x, partials + (size_t)blockIdx.x * kMoments);
 }
 
 __global__ __launch_bounds__(256) void k_subtract_noise(View<const float4> in, View<const float4> in1,
                                                         View<const float4> den1, View<const float2> grad,
                                                         float luma_weight, float sharpening, float nlf_x,
                                                         float nlf_y, const NlfDev* nlf_level, float thr_y,
-                                                        View<float4> out, float* __restrict__ patches,
-                                                        XcdTiles order) {
+                                                        View<float4> out, XcdTiles order) {
     int tx, ty;
     if (!order.tile(tx, ty)) return;
     const int ox = tx * 64 + threadIdx.x, oy = ty * 4 + threadIdx.y;
-    if (ox < out.w && oy < out.h) {
-        const float y = subtract_noise_px(in, in1, den1, grad, luma_weight, sharpening, nlf_x, nlf_y, nlf_level,
-                                          thr_y, out, ox, oy);
-        if (patches) write_patch_slots(patches, out.w / 8, out.h / 8, ox, oy, y);
-    }
+    if (ox < out.w && oy < out.h)
+        subtract_noise_px(in, in1, den1, grad, luma_weight, sharpening, nlf_x, nlf_y, nlf_level, thr_y, out, ox, oy);
 }
 
 // Burst forms: one launch for the level of every frame; block b covers the
@@ -214,18 +260,16 @@ __global__ __launch_bounds__(256) void k_burst_downsample(const BurstTable<Burst
     }
 }
 
-__global__ __launch_bounds__(256) void k_burst_subtract(const BurstTable<BurstSubFrame>* __restrict__ tab,
-                                                        XcdTiles order) {
-    int b, unused;
-    if (!order.tile(b, unused)) return;
-    const BurstSubFrame& f = tab->frame[burst_frame_of(tab, b)];
-    const int lb = b - f.block0;
-    const int ox = (lb % f.bx) * 64 + threadIdx.x, oy = (lb / f.bx) * 4 + threadIdx.y;
-    if (ox < f.out.w && oy < f.out.h) {
-        const float y =
-            subtract_noise_px(f.in, f.in1, f.den1, f.grad, 1.0f, f.sharpening, 0.0f, 0.0f, f.nlf, f.thr_y, f.out, ox, oy);
-        if (f.patches) write_patch_slots(f.patches, f.out.w / 8, f.out.h / 8, ox, oy, y);
-    }
+// Burst form of the fused pass: frame f owns workgroups block0 .. block0 +
+// nblk - 1 (block0 a multiple of 8, so a workgroup's XCD is that of the
+// single-frame pass's workgroup of the same index); workgroups between
+// frames have no frame and return.
+__global__ __launch_bounds__(256) void k_burst_subtract_moments(const BurstTable<BurstSubFrame>* __restrict__ tab) {
+    const BurstSubFrame& f = tab->frame[burst_frame_of(tab, blockIdx.x)];
+    const int lb = (int)blockIdx.x - f.block0;
+    if (lb >= f.nblk) return;
+    const SubMomArgs a{f.in, f.in1, f.den1, f.grad, 1.0f, f.sharpening, 0.0f, 0.0f, f.nlf, f.thr_y, f.out};
+    subtract_moments_body(a, lb, f.nblk, f.partials + (size_t)lb * kMoments);
 }
 
 // basicNoiseStatistics: 5x5 mean of Y and population variances of Y, Cb, Cr.
@@ -461,10 +505,10 @@ gcam_status launch_downsample2(View<const float2> in, View<float2> out, hipStrea
 gcam_status launch_subtract_noise(View<const float4> in, View<const float4> in1, View<const float4> den1,
                                   View<const float2> grad, float luma_weight, float sharpening, float nlf_x,
                                   float nlf_y, const NlfDev* nlf_level, float thr_y, View<float4> out,
-                                  float* patches, hipStream_t s) {
+                                  hipStream_t s) {
     const XcdTiles order = xcd_tiles(out.w, out.h, 64, 4);
     k_subtract_noise<<<order.grid(), dim3(64, 4), 0, s>>>(in, in1, den1, grad, luma_weight, sharpening, nlf_x, nlf_y,
-                                                          nlf_level, thr_y, out, patches, order);
+                                                          nlf_level, thr_y, out, order);
     GCAM_LAUNCH_CHECK("subtractNoiseImage");
     return GCAM_OK;
 }
@@ -512,12 +556,23 @@ gcam_status launch_burst_downsample(const BurstTable<BurstDownFrame>* dev, int t
     return GCAM_OK;
 }
 
-gcam_status launch_burst_subtract(const BurstTable<BurstSubFrame>* dev, int total_blocks, hipStream_t s) {
-    if (total_blocks < 1) return GCAM_OK;
-    const XcdTiles order = xcd_tiles(total_blocks, 1, 1, 1);
-    k_burst_subtract<<<order.grid(), dim3(64, 4), 0, s>>>(dev, order);
-    GCAM_LAUNCH_CHECK("burst subtractNoiseImage");
+gcam_status launch_subtract_moments(View<const float4> in, View<const float4> in1, View<const float4> den1,
+                                    View<const float2> grad, float luma_weight, float sharpening, float nlf_x,
+                                    float nlf_y, const NlfDev* nlf_level, float thr_y, View<float4> out,
+                                    double* partials, int* nblocks, hipStream_t s) {
+    *nblocks = submom_blocks(out.w, out.h);
+    const SubMomArgs a{in, in1, den1, grad, luma_weight, sharpening, nlf_x, nlf_y, nlf_level, thr_y, out};
+    k_subtract_moments<<<*nblocks, dim3(64, 4), 0, s>>>(a, partials);
+    GCAM_LAUNCH_CHECK("subtractNoiseImage with PCA moments");
     return GCAM_OK;
 }
+
+gcam_status launch_burst_subtract_moments(const BurstTable<BurstSubFrame>* dev, int total_blocks, hipStream_t s) {
+    if (total_blocks < 1) return GCAM_OK;
+    k_burst_subtract_moments<<<total_blocks, dim3(64, 4), 0, s>>>(dev);
+    GCAM_LAUNCH_CHECK("burst subtractNoiseImage with PCA moments");
+    return GCAM_OK;
+}
+
 
 }  // namespace gcam
