@@ -141,6 +141,7 @@ gcam_status gcam_pyramid_create(int32_t width, int32_t height, int32_t levels, c
     parts.push_back({(void**)&p->nlf_dev, sizeof(NlfDev) * GCAM_MAX_LEVELS});
     parts.push_back({(void**)&p->basis, sizeof(float) * 200 * levels});
     parts.push_back({(void**)&p->eigvals, sizeof(double) * 25 * levels});
+    parts.push_back({(void**)&p->tri, sizeof(double) * 50 * levels});
     size_t total = 0;
     for (auto& pr : parts) total += align256(pr.second);
     hipError_t e = hipMalloc((void**)&p->arena, total);
@@ -304,7 +305,7 @@ static gcam_status denoise_impl(gcam_pyramid* p, const gcam_denoise_params* para
         if (st != GCAM_OK) return st;
         GCAM_MARK(i, GCAM_STAGE_MOMENTS, 1);
         st = launch_pca_solve(p->moment_partials, nblocks, n, p->basis + 200 * i, p->eigvals + 25 * i,
-                              keep_eig ? p->eig_full + kEigStride * i : nullptr, s);
+                              keep_eig ? p->eig_full + kEigStride * i : nullptr, p->tri + 50 * i, s);
         if (st != GCAM_OK) return st;
         const View<float> fv = view<float>(p->feat, p->W[i], p->H[i]);
         if (p->precision == GCAM_PRECISION_FP32) {
@@ -508,8 +509,14 @@ gcam_status gcam_pyramid_basis(const gcam_pyramid* p, int32_t level, float* basi
     DeviceGuard g(p->device);
     GCAM_HIP_TRY(hipDeviceSynchronize());
     if (basis) GCAM_HIP_TRY(hipMemcpy(basis, p->basis + 200 * level, sizeof(float) * 200, hipMemcpyDeviceToHost));
-    if (eigenvalues)
+    if (eigenvalues) {
+        // The denoise's solve brackets only the 8 leading eigenvalues; the rest
+        // come from the tridiagonal form it kept.
+        const gcam_status st = launch_tridiag_eigvals(p->tri + 50 * level, p->eigvals + 25 * level, nullptr);
+        if (st != GCAM_OK) return st;
+        GCAM_HIP_TRY(hipDeviceSynchronize());
         GCAM_HIP_TRY(hipMemcpy(eigenvalues, p->eigvals + 25 * level, sizeof(double) * 25, hipMemcpyDeviceToHost));
+    }
     return GCAM_OK;
 }
 
@@ -569,7 +576,7 @@ gcam_status gcam_build_pca_space(const float* patches, int64_t n, float* basis, 
     GCAM_HIP_TRY(hipMallocAsync((void**)&partials, sizeof(double) * kMomentSlots * kMoments, s));
     int nblocks = 0;
     gcam_status st = launch_buffer_moments(patches, n, partials, &nblocks, s);
-    if (st == GCAM_OK) st = launch_pca_solve(partials, nblocks, n, basis, eigenvalues, nullptr, s);
+    if (st == GCAM_OK) st = launch_pca_solve(partials, nblocks, n, basis, eigenvalues, nullptr, nullptr, s);
     GCAM_HIP_TRY(hipFreeAsync(partials, s));
     return st;
 }

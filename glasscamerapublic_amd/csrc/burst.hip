@@ -228,6 +228,7 @@ void fill_tables(gcam_burst* b, BurstTables* t, const gcam_denoise_params* param
             m.n = (double)(((int64_t)w * h) / 64);  // pyramid_processor.cpp:142
             m.basis = p->basis + 200 * i;
             m.eigvals = p->eigvals + 25 * i;
+            m.tri = p->tri + 50 * i;
             mom += m.nblocks;
         }
         t->down[i].total = down;

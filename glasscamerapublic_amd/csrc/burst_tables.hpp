@@ -41,6 +41,7 @@ struct BurstMomFrame {
     double n;          // patch slots, w * h / 64
     float* basis;      // 25 x 8
     double* eigvals;   // 25
+    double* tri;       // 50: the tridiagonal form (eigenvalues 9..25 on demand)
 };
 
 struct BurstBmFrame {

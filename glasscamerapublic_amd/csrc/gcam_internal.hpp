@@ -136,8 +136,14 @@ gcam_status launch_image_moments(View<const float4> in, double* partials, int* n
 // Moments of a device patch buffer (n x 25).
 gcam_status launch_buffer_moments(const float* patches, int64_t n, double* partials, int* nblocks, hipStream_t s);
 // Reduce the partials, form the covariance, eigensolve, write basis (25 x 8 f32) and eigenvalues.
+// tri (nullable, 2 x 25 doubles): bracket only the 8 leading eigenvalues,
+// leave eigvals[8..24] NaN and keep the tridiagonal form there for
+// launch_tridiag_eigvals (the basis needs only the 8 leading eigenpairs).
 gcam_status launch_pca_solve(double* partials, int nblocks, int64_t n, float* basis, double* eigvals,
-                             double* eig_full, hipStream_t s);
+                             double* eig_full, double* tri, hipStream_t s);
+// Completes eigvals[8..24] of a solve that kept its tridiagonal form in tri
+// (no-op when they are already set).
+gcam_status launch_tridiag_eigvals(const double* tri, double* eigvals, hipStream_t s);
 gcam_status launch_pca_project(View<const float4> in, const float* basis, View<float> feat8, hipStream_t s);
 
 // Default output rows per thread of block matching (rows_per_thread == 0):

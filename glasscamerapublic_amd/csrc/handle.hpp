@@ -21,7 +21,8 @@ struct gcam_pyramid {
     gcam::NlfFitState* nlf_state = nullptr;
     gcam::NlfDev* nlf_dev = nullptr;
     float* basis = nullptr;    // levels x 25 x 8
-    double* eigvals = nullptr; // levels x 25
+    double* eigvals = nullptr; // levels x 25 (9..25 NaN until gcam_pyramid_basis completes them)
+    double* tri = nullptr;     // levels x 50: each level's tridiagonal form (diagonal, off-diagonal, scale)
     int feature_level = -1;
     // differentiation form (allocated by the first gcam_pyramid_denoise_fwd)
     char* adj_arena = nullptr;

@@ -73,5 +73,5 @@ extern "C" gcam_status gcam_build_pca_space_noalloc(const float* patches, int64_
     int nblocks = 0;
     st = launch_buffer_moments(patches, n, partials, &nblocks, s);
     if (st != GCAM_OK) return st;
-    return launch_pca_solve(partials, nblocks, n, basis, eigenvalues, nullptr, s);
+    return launch_pca_solve(partials, nblocks, n, basis, eigenvalues, nullptr, nullptr, s);
 }

@@ -393,6 +393,7 @@ struct Top8 {
     double X[kComponents][kPatch];      // eigenvectors of C
     double red[4];
     double lo, hi;
+    double scale;  // C = scale * W on entry (W's largest entry is 1)
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -455,9 +456,57 @@ __device__ __forceinline__ double rsq_full(double x) {
     return y * fma(x * y, -0.5 * y, 1.5);
 }
 
+// The Gershgorin interval of the tridiagonal T (diagonal d, off-diagonal e),
+// widened by a relative 1e-12, and the Sturm recurrence terms
+// de2[i] = (d_i, e_{i-1}^2) (e_{-1} = 0).
+__device__ __forceinline__ void tridiag_bracket(const double* d, const double* e, double2* de2, double& lo,
+                                                double& hi) {
+    constexpr int N = kPatch;
+    lo = 1e300;
+    hi = -1e300;
+    for (int i = 0; i < N; i++) {
+        const double rad = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < N - 1 ? fabs(e[i]) : 0.0);
+        lo = fmin(lo, d[i] - rad);
+        hi = fmax(hi, d[i] + rad);
+        de2[i] = make_double2(d[i], i > 0 ? e[i - 1] * e[i - 1] : 0.0);
+    }
+    const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
+    lo -= pad;
+    hi += pad;
+}
+
+// Eigenvalue j (the j-th largest) of T by `rounds` rounds of (NL + 1)-way
+// multisection of [lo, hi] on a group of NL lanes (lane m of the group tests
+// lo + (m + 1) w / (NL + 1)); every lane of the group returns the midpoint of
+// the final bracket.  17 rounds at NL = 8 and 11 at NL = 32 both shrink the
+// bracket by more than 2^53.
+template <int NL>
+__device__ __forceinline__ double multisect(const double (&dd)[kPatch], const double (&ee)[kPatch], double lo,
+                                            double hi, int j, int m, int rounds) {
+    const int target = kPatch - 1 - j;  // x is above the j-th largest iff more than target lie below x
+    const int lane = threadIdx.x & 63;
+    constexpr uint64_t kMask = NL >= 64 ? ~0ull : ((1ull << NL) - 1);
+    const double at = (double)(m + 1) / (double)(NL + 1);
+    for (int it = 0; it < rounds; it++) {
+        const double w = hi - lo;
+        const double x = fma(w, at, lo);
+        const bool above = sturm_below(dd, ee, x) > target;
+        const uint64_t grp = (__ballot(above) >> (lane & ~(NL - 1))) & kMask;
+        const int ms = grp ? __builtin_ctzll(grp) : NL;
+        const double nlo = ms == 0 ? lo : fma(w, (double)ms / (double)(NL + 1), lo);
+        const double nhi = ms == NL ? hi : fma(w, (double)(ms + 1) / (double)(NL + 1), lo);
+        lo = nlo;
+        hi = nhi;
+    }
+    return 0.5 * (lo + hi);
+}
+
 // Writes S.lam (scaled back: eigenvalues of C, descending) and S.X (the 8
-// leading eigenvectors); returns whether they pass the checks.  A0: the
+// leading eigenvectors); returns whether they pass the checks.  kAll: all 25
+// eigenvalues (8 lanes each, 17 rounds); otherwise the 8 leading only (32
+// lanes each, 11 rounds) and S.lam[8..24] = NaN.  A0: the
 // covariance (leading dimension kLD).  Every thread of the workgroup calls it.
+template <bool kAll>
 __device__ bool top8_eigen(const double* A0, Top8& S) {
     constexpr int N = kPatch, LD = kLD;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -468,6 +517,7 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
     if (lane == 0) S.red[wave] = mx;
     __syncthreads();
     const double scale = fmax(fmax(S.red[0], S.red[1]), fmax(S.red[2], S.red[3]));
+    if (tid == 0) S.scale = scale;
     if (!(scale > 0.0) || !(scale < 1e300)) return false;
     const double rscale = 1.0 / scale;
     for (int e = tid; e < N * N; e += 256) S.W[e / N][e % N] = A0[(e / N) * LD + e % N] * rscale;
@@ -575,27 +625,14 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
                 if (2 * j + h < N) S.W[r][2 * j + h] = q[j];
         }
         wave_sync();
-        if (lane == 0) {
-            double lo = 1e300, hi = -1e300;
-            for (int i = 0; i < N; i++) {
-                const double rad = (i > 0 ? fabs(S.e[i - 1]) : 0.0) + (i < N - 1 ? fabs(S.e[i]) : 0.0);
-                lo = fmin(lo, S.d[i] - rad);
-                hi = fmax(hi, S.d[i] + rad);
-                S.de2[i] = make_double2(S.d[i], i > 0 ? S.e[i - 1] * S.e[i - 1] : 0.0);
-            }
-            const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
-            S.lo = lo - pad;
-            S.hi = hi + pad;
-        }
+        if (lane == 0) tridiag_bracket(S.d, S.e, S.de2, S.lo, S.hi);
     }
     __syncthreads();
 
-    // Multisection: eigenvalue j (j-th largest) on lanes 8j' .. 8j' + 7 of
-    // wave j / 8; lane m tests lo + (m + 1) w / 9.  17 rounds shrink the
-    // Gershgorin interval by 9^17 > 2^53.
+    // Multisection, all eigenvalues: eigenvalue j on lanes 8j' .. 8j' + 7 of
+    // wave j / 8; the 8 leading only: eigenvalue j on lanes 32 (j % 2) ..
+    // 32 (j % 2) + 31 of wave j / 2.
     {
-        const int j = wave * 8 + (lane >> 3), m = lane & 7;
-        const int target = N - 1 - j;  // x is above the j-th largest iff more than target lie below x
         double dd[N], ee[N];
 #pragma unroll
         for (int i = 0; i < N; i++) {
@@ -603,20 +640,16 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
             dd[i] = t.x;
             ee[i] = t.y;
         }
-        double lo = S.lo, hi = S.hi;
-        for (int it = 0; it < 17; it++) {
-            const double w = hi - lo;
-            const double x = fma(w, (double)(m + 1) / 9.0, lo);
-            const bool above = sturm_below(dd, ee, x) > target;
-            const uint64_t bal = __ballot(above);
-            const uint32_t grp = (uint32_t)(bal >> (lane & ~7)) & 0xffu;
-            const int ms = grp ? __builtin_ctz(grp) : 8;
-            const double nlo = ms == 0 ? lo : fma(w, (double)ms / 9.0, lo);
-            const double nhi = ms == 8 ? hi : fma(w, (double)(ms + 1) / 9.0, lo);
-            lo = nlo;
-            hi = nhi;
+        if constexpr (kAll) {
+            const int j = wave * 8 + (lane >> 3), m = lane & 7;
+            const double lam = multisect<8>(dd, ee, S.lo, S.hi, j, m, 17);
+            if (m == 0 && j < N) S.lam[j] = lam;
+        } else {
+            const int j = wave * 2 + (lane >> 5), m = lane & 31;
+            const double lam = multisect<32>(dd, ee, S.lo, S.hi, j, m, 11);
+            if (m == 0) S.lam[j] = lam;
+            if (tid >= kComponents && tid < N) S.lam[tid] = __longlong_as_double(0x7ff8000000000000ll);
         }
-        if (m == 0 && j < N) S.lam[j] = 0.5 * (lo + hi);
     }
     __syncthreads();
 
@@ -731,8 +764,13 @@ __device__ bool top8_eigen(const double* A0, Top8& S) {
 // The 8 largest eigenpairs, largest first (PCA.cpp:49-55), are written as
 // basis[r * 8 + c]; ties in eigenvalue keep index order; each vector is
 // oriented to a non-negative dot product with sign_ref.
+// eigvals (nullable): the 25 eigenvalues.  tri (nullable): with eigvals set,
+// the solve brackets only the 8 leading eigenvalues (the basis needs no more),
+// writes NaN for eigenvalues 9..25 and keeps the tridiagonal form in tri
+// (diagonal 0..24, off-diagonal 25..48, scale 49) for k_tridiag_eigvals to
+// complete them on demand; without tri the solve brackets all 25.
 __device__ __forceinline__ void pca_solve_body(const double* moments, double n, float* basis, double* eigvals,
-                                               double* eig_full) {
+                                               double* eig_full, double* tri) {
     constexpr int N = kPatch;
     constexpr int LD = kLD;
     __shared__ double mom[kMoments];
@@ -758,9 +796,15 @@ __device__ __forceinline__ void pca_solve_body(const double* moments, double n, 
     // its checks pass; the Jacobi sweeps run when they fail, and for the full
     // decomposition of the reverse pass (which then only fills eig_full).
     __shared__ Top8 t8;
-    const bool fast = top8_eigen(A0, t8);
+    const bool all = eigvals && !tri;
+    const bool fast = all ? top8_eigen<true>(A0, t8) : top8_eigen<false>(A0, t8);
     if (fast) {
         if (tid < N && eigvals) eigvals[tid] = t8.lam[tid];
+        if (tri && eigvals) {
+            if (tid < N) tri[tid] = t8.d[tid];
+            if (tid < N - 1) tri[N + tid] = t8.e[tid];
+            if (tid == 0) tri[2 * N - 1] = t8.scale;
+        }
         if (tid < kComponents) {
             double dot = 0.0;
             for (int r = 0; r < N; r++) dot += t8.X[tid][r] * sign_ref(r);
@@ -801,8 +845,35 @@ __device__ __forceinline__ void pca_solve_body(const double* moments, double n, 
 }
 
 __global__ __launch_bounds__(256) void k_pca_solve(const double* moments, double n, float* basis, double* eigvals,
-                                                   double* eig_full) {
-    pca_solve_body(moments, n, basis, eigvals, eig_full);
+                                                   double* eig_full, double* tri) {
+    pca_solve_body(moments, n, basis, eigvals, eig_full, tri);
+}
+
+// Eigenvalues 9..25 of a level whose solve kept the tridiagonal form and left
+// them NaN: the multisection of the all-eigenvalue solve on the same T.
+// Nothing to do when eigvals[8] is a number (the all-eigenvalue solve or the
+// Jacobi sweeps wrote all 25).
+__global__ __launch_bounds__(256) void k_tridiag_eigvals(const double* __restrict__ tri, double* __restrict__ eigvals) {
+    constexpr int N = kPatch;
+    if (!isnan(eigvals[kComponents])) return;
+    __shared__ double d[N], e[N];
+    __shared__ double2 de2[N];
+    __shared__ double lohi[2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < N) d[tid] = tri[tid];
+    if (tid < N - 1) e[tid] = tri[N + tid];
+    __syncthreads();
+    if (tid == 0) tridiag_bracket(d, e, de2, lohi[0], lohi[1]);
+    __syncthreads();
+    double dd[N], ee[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        dd[i] = de2[i].x;
+        ee[i] = de2[i].y;
+    }
+    const int j = wave * 8 + (lane >> 3), m = lane & 7;
+    const double lam = multisect<8>(dd, ee, lohi[0], lohi[1], j, m, 17);
+    if (m == 0 && j >= kComponents && j < N) eigvals[j] = lam * tri[2 * N - 1];
 }
 
 // Burst forms: moments of every frame of a level in one launch (frame f owns
@@ -825,7 +896,7 @@ __global__ __launch_bounds__(256) void k_burst_moments_reduce(const BurstTable<B
 
 __global__ __launch_bounds__(256) void k_burst_pca_solve(const BurstTable<BurstMomFrame>* __restrict__ tab) {
     const BurstMomFrame& f = tab->frame[blockIdx.x];
-    pca_solve_body(f.partials + (size_t)kMomentBlocks * kMoments, f.n, f.basis, f.eigvals, nullptr);
+    pca_solve_body(f.partials + (size_t)kMomentBlocks * kMoments, f.n, f.basis, f.eigvals, nullptr, f.tri);
 }
 
 // pcaProjection: f[c] = sum_row basis[row][c] * Y(p + off(row)), rows j-outer /
@@ -896,11 +967,17 @@ gcam_status launch_buffer_moments(const float* patches, int64_t n, double* parti
 }
 
 gcam_status launch_pca_solve(double* partials, int nblocks, int64_t n, float* basis, double* eigvals,
-                             double* eig_full, hipStream_t s) {
+                             double* eig_full, double* tri, hipStream_t s) {
     double* mom = partials + (size_t)kMomentBlocks * kMoments;
     k_moments_reduce<<<kMoments / kRedEntries, 256, 0, s>>>(partials, nblocks, mom);
-    k_pca_solve<<<1, 256, 0, s>>>(mom, (double)n, basis, eigvals, eig_full);
+    k_pca_solve<<<1, 256, 0, s>>>(mom, (double)n, basis, eigvals, eig_full, tri);
     GCAM_LAUNCH_CHECK("PCA eigensolve");
+    return GCAM_OK;
+}
+
+gcam_status launch_tridiag_eigvals(const double* tri, double* eigvals, hipStream_t s) {
+    k_tridiag_eigvals<<<1, 256, 0, s>>>(tri, eigvals);
+    GCAM_LAUNCH_CHECK("PCA eigenvalues from the tridiagonal form");
     return GCAM_OK;
 }
 
