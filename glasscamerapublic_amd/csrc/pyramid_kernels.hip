@@ -81,13 +81,43 @@ __device__ __forceinline__ void downsample4_px(const View<const float4>& in, con
     out.at(ox, oy) = make_float4(0.25f * acc.x, 0.25f * acc.y, 0.25f * acc.z, 0.0f);
 }
 
+// RO vertically adjacent outputs oy0 .. oy0 + RO - 1 of column ox.  The
+// second vertical sample of output oy is the first of output oy + 1 (both at
+// texel row (oy + 1) * in.h / out.h), so RO outputs take RO + 1 sample rows,
+// 8 (RO + 1) texel loads instead of 16 RO; each output adds its four samples
+// in the order of downsample4_px (identical results).
+template <int RO>
+__device__ __forceinline__ void downsample4_col(const View<const float4>& in, const View<float4>& out, int ox,
+                                                int oy0) {
+    const Tap tx0 = lin_tap(ox * in.w, out.w, in.w), tx1 = lin_tap((ox + 1) * in.w, out.w, in.w);
+    float3 s0[RO + 1], s1[RO + 1];
+#pragma unroll
+    for (int b = 0; b <= RO; b++) {
+        const Tap ty = lin_tap((oy0 + b) * in.h, out.h, in.h);
+        s0[b] = bilerp3(in, tx0, ty);
+        s1[b] = bilerp3(in, tx1, ty);
+    }
+#pragma unroll
+    for (int r = 0; r < RO; r++) {
+        if (oy0 + r >= out.h) break;
+        float3 acc = s0[r];
+        acc.x += s1[r].x, acc.y += s1[r].y, acc.z += s1[r].z;
+        acc.x += s0[r + 1].x, acc.y += s0[r + 1].y, acc.z += s0[r + 1].z;
+        acc.x += s1[r + 1].x, acc.y += s1[r + 1].y, acc.z += s1[r + 1].z;
+        out.at(ox, oy0 + r) = make_float4(0.25f * acc.x, 0.25f * acc.y, 0.25f * acc.z, 0.0f);
+    }
+}
+
 // The per-pixel kernels run 64 x 4 blocks in XCD-contiguous order: the input
-// rows that vertically adjacent blocks share stay in one XCD's L2.
+// rows that vertically adjacent blocks share stay in one XCD's L2.  The
+// downsample's threads take kDownRows outputs of a column each (blocks of
+// 64 x 4 kDownRows outputs).
+constexpr int kDownRows = 4;
 __global__ __launch_bounds__(256) void k_downsample4(View<const float4> in, View<float4> out, XcdTiles order) {
     int tx, ty;
     if (!order.tile(tx, ty)) return;
-    const int ox = tx * 64 + threadIdx.x, oy = ty * 4 + threadIdx.y;
-    if (ox < out.w && oy < out.h) downsample4_px(in, out, ox, oy);
+    const int ox = tx * 64 + threadIdx.x, oy = (ty * 4 + threadIdx.y) * kDownRows;
+    if (ox < out.w && oy < out.h) downsample4_col<kDownRows>(in, out, ox, oy);
 }
 
 // downsampleImageXY, the 2-channel gradient version.
@@ -489,7 +519,7 @@ inline dim3 grid64x4(int w, int h) { return dim3((w + 63) / 64, (h + 3) / 4); }
 }  // namespace
 
 gcam_status launch_downsample4(View<const float4> in, View<float4> out, hipStream_t s) {
-    const XcdTiles order = xcd_tiles(out.w, out.h, 64, 4);
+    const XcdTiles order = xcd_tiles(out.w, out.h, 64, 4 * kDownRows);
     k_downsample4<<<order.grid(), dim3(64, 4), 0, s>>>(in, out, order);
     GCAM_LAUNCH_CHECK("downsampleImageXYZ");
     return GCAM_OK;
