@@ -60,7 +60,7 @@ struct BmTile {
     static constexpr int LH = TH * PV + 2 * R;
     static constexpr int LN = LW * LH;
     static constexpr int THREADS = TW * TH;
-    static_assert(LN * 48 <= 64 * 1024, "LDS tile too large");
+    static_assert(LN * 48 <= 160 * 1024, "LDS tile larger than a CU's");
 };
 
 template <int R, int PV, int TW, int TH>
@@ -225,18 +225,19 @@ gcam_status launch_shape(View<const float4> in, View<const float2> grad, View<co
     return GCAM_OK;
 }
 
-// Tile shapes: 1 row per thread uses a 32 x 8 (R <= 6) or 32 x 4 (R > 6)
-// workgroup; 2 and 4 rows per thread (R <= 6) use 32 x 8 and 32 x 4.
+// Tile shapes: 1 row per thread uses a 32 x 8 workgroup; 2 and 4 rows per
+// thread (R <= 6) use 32 x 8 and 32 x 4.  Above radius 6 every setting takes
+// one row on 32 x 8 (70 KB of LDS at radius 10, two workgroups per CU; 12 MP
+// level 0 at radius 10: 4.55 ms, against 7.41 ms on 32 x 4 workgroups and
+// 4.63 ms on 32 x 16).
 template <int R>
 gcam_status launch_r(View<const float4> in, View<const float2> grad, View<const float> feat8, const NlfDev* nlf,
                      const BmParams& p, const DirTable& dir, int rows, View<float4> out, hipStream_t s) {
     if constexpr (R <= 6) {
         if (rows == 4) return launch_shape<R, 4, 32, 4>(in, grad, feat8, nlf, p, dir, out, s);
         if (rows == 2) return launch_shape<R, 2, 32, 8>(in, grad, feat8, nlf, p, dir, out, s);
-        return launch_shape<R, 1, 32, 8>(in, grad, feat8, nlf, p, dir, out, s);
-    } else {
-        return launch_shape<R, 1, 32, 4>(in, grad, feat8, nlf, p, dir, out, s);
     }
+    return launch_shape<R, 1, 32, 8>(in, grad, feat8, nlf, p, dir, out, s);
 }
 
 DirTable make_dir_table() {

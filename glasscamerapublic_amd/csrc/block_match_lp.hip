@@ -450,16 +450,26 @@ gcam_status launch_lp_shape(View<const float4> in, View<const float2> grad, cons
     return GCAM_OK;
 }
 
+// One output row per thread: 32 x 16 threads (32 x 16-output tiles) where the
+// level has at least 1024 such tiles, else 32 x 8 (smaller tiles, more
+// workgroups for the small levels).
+template <int R, class H>
+gcam_status launch_lp_one_row(View<const float4> in, View<const float2> grad, const float* basis, const NlfDev* nlf,
+                              const BmParams& p, const DirTable& dir, View<float4> out, hipStream_t s) {
+    if ((long)((out.w + 31) / 32) * ((out.h + 15) / 16) >= 1024)
+        return launch_lp_shape<R, 1, 32, 16, H>(in, grad, basis, nlf, p, dir, out, s);
+    return launch_lp_shape<R, 1, 32, 8, H>(in, grad, basis, nlf, p, dir, out, s);
+}
+
 template <int R, class H>
 gcam_status launch_lp_r(View<const float4> in, View<const float2> grad, const float* basis, const NlfDev* nlf,
                         const BmParams& p, const DirTable& dir, int rows, View<float4> out, hipStream_t s) {
+    if (rows == 2) return launch_lp_shape<R, 2, 32, 8, H>(in, grad, basis, nlf, p, dir, out, s);
     if constexpr (R <= 6) {
         if (rows == 4) return launch_lp_shape<R, 4, 32, 4, H>(in, grad, basis, nlf, p, dir, out, s);
-        if (rows == 2) return launch_lp_shape<R, 2, 32, 8, H>(in, grad, basis, nlf, p, dir, out, s);
-        return launch_lp_shape<R, 1, 32, 8, H>(in, grad, basis, nlf, p, dir, out, s);
-    } else {
-        return launch_lp_shape<R, 1, 32, 4, H>(in, grad, basis, nlf, p, dir, out, s);
     }
+    // rows 1 (and 4 above radius 6, where four rows' registers do not fit)
+    return launch_lp_one_row<R, H>(in, grad, basis, nlf, p, dir, out, s);
 }
 
 template <class H>
@@ -504,7 +514,8 @@ gcam_status launch_burst_r(const BurstTable<BurstBmFrame>* dev, int total, const
         if (cfg.rows == 2) return launch_burst_shape<R, 2, 32, 8, H>(dev, total, dir, cfg, s);
         return launch_burst_shape<R, 1, 32, 8, H>(dev, total, dir, cfg, s);
     } else {
-        return launch_burst_shape<R, 1, 32, 4, H>(dev, total, dir, cfg, s);
+        if (cfg.rows == 2) return launch_burst_shape<R, 2, 32, 8, H>(dev, total, dir, cfg, s);
+        return launch_burst_shape<R, 1, 32, 16, H>(dev, total, dir, cfg, s);
     }
 }
 
@@ -529,11 +540,12 @@ gcam_status burst_dispatch(const BurstTable<BurstBmFrame>* dev, const DirTable* 
 }  // namespace
 
 void burst_bm_tile(int radius, int rows, int tile_w, int* tw, int* th) {
-    // the shapes of launch_burst_r: 32 x 4 threads (radius > 6, one row each),
-    // 32 x 4 x 4 rows, 32 (or 64) x 8 x 2 rows, 32 x 8 x 1 row
+    // the shapes of launch_burst_r: radius up to 6: 32 x 4 threads x 4 rows,
+    // 32 (or 64) x 8 x 2 rows, 32 x 8 x 1 row; above: 32 x 8 x 2 rows, else
+    // 32 x 16 x 1 row (16 output rows either way)
     *tw = tile_w == 64 && rows == 2 && radius <= 6 ? 64 : 32;
     if (radius > 6) {
-        *th = 4;
+        *th = 16;
     } else {
         *th = rows == 4 ? 16 : 8 * rows;
     }

@@ -331,7 +331,7 @@ gcam_status autotune(gcam_burst* b, const gcam_denoise_params* params, const gca
         };
         const int rows_set[3] = {1, 2, 4};
         for (int k = 0; k < 3 && st == GCAM_OK; k++) {
-            if (b->radius > 6 && rows_set[k] != 1) continue;
+            if (b->radius > 6 && rows_set[k] == 4) continue;
             st = consider(BurstBmConfig{rows_set[k], 0, 0, 32});
         }
         if (st == GCAM_OK && best.rows == 2 && b->radius <= 6) st = consider(BurstBmConfig{2, 0, 0, 64});
@@ -493,8 +493,8 @@ gcam_status gcam_burst_set_tuning(gcam_burst* b, const gcam_burst_tuning* tuning
     for (int i = 0; i < b->levels; i++) {
         const int r = tuning->rows_per_thread[i];
         if (r != 1 && r != 2 && r != 4) return set_error(GCAM_ERR_INVALID_ARGUMENT, "rows_per_thread must be 1, 2 or 4");
-        if (b->radius > 6 && r != 1)
-            return set_error(GCAM_ERR_INVALID_ARGUMENT, "search radius > 6 runs one row per thread");
+        if (b->radius > 6 && r == 4)
+            return set_error(GCAM_ERR_INVALID_ARGUMENT, "search radius > 6 runs one or two rows per thread");
         if (tuning->grid[i] < 0) return set_error(GCAM_ERR_INVALID_ARGUMENT, "grid must be >= 0");
         if (tuning->schedule[i] != 0 && tuning->schedule[i] != 1)
             return set_error(GCAM_ERR_INVALID_ARGUMENT, "schedule must be 0 or 1");

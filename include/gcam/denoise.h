@@ -200,7 +200,7 @@ gcam_status gcam_pyramid_denoise_bwd(gcam_pyramid* pyramid, const gcam_image* gr
 typedef struct gcam_burst gcam_burst;
 
 typedef struct gcam_burst_tuning {
-    int32_t rows_per_thread[GCAM_MAX_LEVELS]; /* 1, 2 or 4 (radius > 6: 1)              */
+    int32_t rows_per_thread[GCAM_MAX_LEVELS]; /* 1, 2 or 4 (radius > 6: 1 or 2)          */
     int32_t grid[GCAM_MAX_LEVELS];            /* persistent workgroups; 0 = one per tile */
     int32_t schedule[GCAM_MAX_LEVELS];        /* 0: tiles in frame order, strided over the
                                                  grid; 1: contiguous chunks per XCD     */
