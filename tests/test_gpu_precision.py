@@ -57,8 +57,8 @@ def test_block_match_lp_close_to_fp64(precision, tol, radius):
 @pytest.mark.parametrize("radius", [1, 5, 7, 10])
 @pytest.mark.parametrize("precision,tol", [("fp16", 5e-3), ("bf16", 3e-2)])
 def test_lp_tile_shapes_vs_fp64(precision, tol, radius, rows, oracle_threads):
-    """Every tile shape of the precision-form kernel (one, two or four output rows per thread for
-    radius <= 6; 32 x 4 threads, one row each, for radius 7..10, which ignores rows) through the
+    """Every tile shape of the precision-form kernel at this size (one, two or four output rows per
+    thread for radius <= 6; one or two rows for radius 7..10, where four runs as one) through the
     whole two-level pyramid, against the FP64 oracle on both levels."""
     w, h, levels = 288, 200, 2
     ycc = gcam.synth_frame(w, h, 4e-3, 2e-3, scene_seed=11 + radius, noise_seed=12 + rows)
@@ -75,6 +75,29 @@ def test_lp_tile_shapes_vs_fp64(precision, tol, radius, rows, oracle_threads):
         print(f"{precision} r{radius} rows {rows} level {i}: max-abs {err.max():.3g} mean-abs {err.mean():.3g}")
         assert err.max() < tol
         assert err.mean() < tol / 10
+
+
+@pytest.mark.parametrize("radius,rows", [(10, 0), (7, 0), (5, 1)])
+def test_one_row_32x16_tiles_vs_fp64(radius, rows, oracle_threads):
+    """One output row per thread on 32 x 16 threads, the shape of levels with at least 1024 such
+    tiles (the default above radius 6, and an explicit one row per thread at any radius): level 0
+    of 1024 x 512 takes it, level 1 the 32 x 8 threads of the small levels; FP16 against the FP64
+    oracle on both levels."""
+    w, h, levels = 1024, 512, 2
+    ycc = gcam.synth_frame(w, h, 4e-3, 2e-3, scene_seed=21 + radius, noise_seed=22)
+    prm = camera_params(levels)
+    nlf = [(tuple(x[0]), tuple(x[1])) for x in nlf_for(levels)]
+    proc = gcam.PyramidProcessor(w, h, levels=levels, search_radius=radius, precision="fp16",
+                                 bm_rows_per_thread=rows)
+    proc.denoise([gcam.DenoiseParameters(**p) for p in prm], ycc, None, list(nlf))
+    got = [host(t) for t in proc.denoisedImagePyramid]
+    proc.close()
+    ref = oracle.pyramid_denoise(host(ycc), prm, np.array(nlf), levels=levels, radius=radius, dtype=np.float64)
+    for i in range(levels):
+        err = np.abs(got[i][..., :3] - ref.denoised[i][..., :3])
+        print(f"r{radius} rows {rows} level {i}: max-abs {err.max():.3g} mean-abs {err.mean():.3g}")
+        assert err.max() < 5e-3
+        assert err.mean() < 5e-4
 
 
 @pytest.mark.parametrize("radius", [4, 10])
