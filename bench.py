@@ -24,8 +24,9 @@ Beside the denoise the default run times the raw front end that produces its
 inputs from the frame's Bayer mosaic (RawFrontEnd, "front_end"; --no-front-end
 to skip), burst fusion (registerAndFuse of the frame, FindHomography on 1500
 matches: "registration"; --no-registration), the reference-CPU configuration
-(1024x1024, 4 levels, 7x7: both forms and the CPU oracle, "c1"; --no-c1) and
-the 32-frame burst of the batched form ("burst32"; --no-burst).
+(1024x1024, 4 levels, 7x7: both forms and the CPU oracle, "c1"; --no-c1), both
+forms with the reference's own 21x21 window on the same frame ("reference_window";
+--no-wide) and the 32-frame burst of the batched form ("burst32"; --no-burst).
 """
 from __future__ import annotations
 
@@ -488,6 +489,16 @@ def single_frame(args) -> dict:
                             "stage_ms_mean5": {k: [round(x, 4) for x in v] for k, v in ref32["stages_mean5"].items()}}
     if quality is not None:
         out["quality"] = quality
+    if not args.no_wide and args.workload == "12mp":
+        # The reference's own window (const int size = 10, demosaic.metal:853) on the same frame, both forms.
+        wide = {"window": "21x21 (radius 10)", "unit": "Mpix/s"}
+        for prec in ("fp16", "fp32"):
+            r = time_single(gcam, torch, w, h, levels, 10, prec, 0, ycc, params, nlf, min(args.steps, 10),
+                            args.warmup, dev)
+            wide[prec] = {"value_median": round(mpix / (r["step_stats"]["median_ms"] / 1e3), 2),
+                          "step_stats": r["step_stats"],
+                          "block_match_ms": [round(x, 4) for x in r["stages_mean5"]["block_match"]]}
+        out["reference_window"] = wide
     if not args.no_front_end:
         # The inputs of this denoise from the frame's raw mosaic (front end of RawConverter::demosaic and
         # its despeckle), reported beside the denoise; the despeckle uses the calibrated level-0 model.
@@ -706,6 +717,8 @@ def main() -> int:
                     help="skip timing burst registration (registerAndFuse, FindHomography) beside the denoise")
     ap.add_argument("--no-quality", action="store_true", help="skip the PSNR check against the FP64 oracle")
     ap.add_argument("--no-burst", action="store_true", help="skip the 32-frame burst leg of the default run")
+    ap.add_argument("--no-wide", action="store_true",
+                    help="skip timing both forms with the reference's 21x21 window on the same frame")
     ap.add_argument("--no-c1", action="store_true",
                     help="skip the 1024x1024, 4-level, 7x7 leg (both forms and the CPU oracle) of the default run")
     ap.add_argument("--cpu-threads", type=int, default=0,
