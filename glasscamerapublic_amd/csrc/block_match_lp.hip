@@ -450,6 +450,16 @@ gcam_status launch_lp_shape(View<const float4> in, View<const float2> grad, cons
     return GCAM_OK;
 }
 
+// Whether two workgroups of the one-row 32 x 16-thread tile (LpTile<R, 1, 32,
+// 16>: the luma region, twice with BF16's remainders, and 36 B per feature
+// pixel) fit a CU's 160 KB of LDS.
+constexpr bool lp_tall_one_row_fits(int radius, bool split) {
+    return (36 + 2 * radius) * (20 + 2 * radius) * (split ? 8 : 4) + (32 + 2 * radius) * (16 + 2 * radius) * 36 <=
+           80 * 1024;
+}
+static_assert(LpTile<10, 1, 32, 16>::LN == (36 + 20) * (20 + 20) && LpTile<10, 1, 32, 16>::FN == 52 * 36,
+              "lp_tall_one_row_fits follows LpTile");
+
 // One output row per thread: 32 x 16 threads (32 x 16-output tiles) where the
 // level has at least 1024 such tiles and two of these workgroups fit a CU's
 // LDS, else 32 x 8 (smaller tiles, more workgroups for the small levels).  The
@@ -458,10 +468,8 @@ gcam_status launch_lp_shape(View<const float4> in, View<const float2> grad, cons
 template <int R, class H>
 gcam_status launch_lp_one_row(View<const float4> in, View<const float2> grad, const float* basis, const NlfDev* nlf,
                               const BmParams& p, const DirTable& dir, View<float4> out, hipStream_t s) {
-    using T16 = LpTile<R, 1, 32, 16>;
-    constexpr bool kSplit = !std::is_same<H, _Float16>::value;
-    constexpr int kLds16 = T16::LN * (kSplit ? 8 : 4) + T16::FN * 36;
-    if (kLds16 <= 80 * 1024 && (long)((out.w + 31) / 32) * ((out.h + 15) / 16) >= 1024)
+    constexpr bool kFits = lp_tall_one_row_fits(R, !std::is_same<H, _Float16>::value);
+    if (kFits && (long)((out.w + 31) / 32) * ((out.h + 15) / 16) >= 1024)
         return launch_lp_shape<R, 1, 32, 16, H>(in, grad, basis, nlf, p, dir, out, s);
     return launch_lp_shape<R, 1, 32, 8, H>(in, grad, basis, nlf, p, dir, out, s);
 }
@@ -520,7 +528,9 @@ gcam_status launch_burst_r(const BurstTable<BurstBmFrame>* dev, int total, const
         return launch_burst_shape<R, 1, 32, 8, H>(dev, total, dir, cfg, s);
     } else {
         if (cfg.rows == 2) return launch_burst_shape<R, 2, 32, 8, H>(dev, total, dir, cfg, s);
-        return launch_burst_shape<R, 1, 32, 16, H>(dev, total, dir, cfg, s);
+        if constexpr (lp_tall_one_row_fits(R, !std::is_same<H, _Float16>::value))
+            return launch_burst_shape<R, 1, 32, 16, H>(dev, total, dir, cfg, s);
+        return launch_burst_shape<R, 1, 32, 8, H>(dev, total, dir, cfg, s);
     }
 }
 
@@ -544,13 +554,14 @@ gcam_status burst_dispatch(const BurstTable<BurstBmFrame>* dev, const DirTable* 
 
 }  // namespace
 
-void burst_bm_tile(int radius, int rows, int tile_w, int* tw, int* th) {
+void burst_bm_tile(int radius, int rows, int tile_w, int precision, int* tw, int* th) {
     // the shapes of launch_burst_r: radius up to 6: 32 x 4 threads x 4 rows,
     // 32 (or 64) x 8 x 2 rows, 32 x 8 x 1 row; above: 32 x 8 x 2 rows, else
-    // 32 x 16 x 1 row (16 output rows either way)
+    // 32 x 16 x 1 row where two such workgroups fit a CU's LDS, 32 x 8 x 1 row
+    // where not
     *tw = tile_w == 64 && rows == 2 && radius <= 6 ? 64 : 32;
     if (radius > 6) {
-        *th = 16;
+        *th = rows == 2 || lp_tall_one_row_fits(radius, precision == GCAM_PRECISION_BF16) ? 16 : 8;
     } else {
         *th = rows == 4 ? 16 : 8 * rows;
     }

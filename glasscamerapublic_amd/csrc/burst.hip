@@ -129,7 +129,7 @@ BurstBmConfig level_config(const gcam_burst* b, int i) {
 void fill_bm_table(gcam_burst* b, BurstTables* t, int i, int rows, int tile_w, const gcam_denoise_params* params,
                    const gcam_image* frames, const gcam_image* gradients, float lens) {
     int tw = 0, th = 0;
-    burst_bm_tile(b->radius, rows, tile_w, &tw, &th);
+    burst_bm_tile(b->radius, rows, tile_w, b->precision, &tw, &th);
     BurstTable<BurstBmFrame>& bt = t->bm[i];
     const BurstTables* d = b->dev;
     int tiles = 0;

@@ -116,7 +116,7 @@ gcam_status launch_burst_pca_solve(const BurstTable<BurstMomFrame>* dev, int nfr
 // Workgroups of the moment reduction for npatch patches (1 .. kMomentBlocks).
 int moment_block_count(int64_t npatch);
 // Output tile of the persistent kernel for `rows` rows per thread: tw x th pixels.
-void burst_bm_tile(int radius, int rows, int tile_w, int* tw, int* th);
+void burst_bm_tile(int radius, int rows, int tile_w, int precision, int* tw, int* th);
 // dir: the direction table in device memory (make_bm_dir_table).
 gcam_status launch_burst_block_match_lp(const BurstTable<BurstBmFrame>* dev, const BmDirTable* dir, int total_tiles,
                                         int radius, int precision, const BurstBmConfig& cfg, hipStream_t s);
